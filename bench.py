@@ -1,0 +1,211 @@
+"""Benchmark of the filtration time step on one MI355X.
+
+Default workload (the configuration the headline metric is quoted on that
+fits one GPU): the plain FP64 drop-in path on a 256^3 grid with per-cell
+log-normal permeability (sigma 1, correlation length 8 cells) and porosity
+U[0.35, 0.45], all five defines.h stages per step (ro_P2_Xi, u, roS, Newton,
+boundary), 200 steps.  Inputs are resident in HBM before the timed region.
+
+Prints one JSON line (rank 0):
+  value      cell-updates/s of the whole job (Nx*Ny*Nz*steps / wall time)
+  roofline   the dominant kernel's algorithmic bytes per launch / its mean
+             duration (HIP events around every stage inside the timed region)
+             against MI355X's 8 TB/s; traffic from rocprofv3 PMC counters when
+             a summary for this workload is committed under profiles/
+  cpu_baseline  the reference CPU path (oracle/_ref, compiled from the
+             reference sources) or the oracle port, on a bounded sample
+"""
+import argparse
+import ctypes
+import importlib
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+PKG = importlib.import_module("shmem-cuda-filtration_amd")
+
+HBM_PEAK = 8.0e12
+PROFILES = os.path.join(ROOT, "profiles")
+
+# Algorithmic bytes per cell of each plain-path kernel for this workload
+# (one medium: the media array is not read; per-cell perm / poro are):
+#   ro_P2_Xi: S2, P1, perm in (24); P2, Xi1, Xi2, ro1, ro2 out (40)     = 64
+#   u:        Xi1, Xi2, ro1, ro2, P1, P2 in (48); six u out (48)        = 96
+#   roS:      S2, P1, P2, Xi1, Xi2, ro1, ro2, poro in (64); 4 out (32)  = 96
+#   Newton:   S2, P1, roS1, roS2 in (32); P1, S2 out (16)               = 48
+PLAIN_BYTES = {"ro_P2_Xi": 64, "u": 96, "roS": 96, "newton": 48, "boundary": 0}
+PLAIN_STAGES = ("ro_P2_Xi", "u", "roS", "newton", "boundary")
+KERNEL_NAMES = {"ro_P2_Xi": "k_ro_P2_Xi", "u": "k_u", "roS": "k_roS", "newton": "k_newton",
+                "boundary": "k_bc_a+k_bc_b"}
+
+
+def cpu_baseline(args):
+    """Reference CPU path on a bounded sample: one time step of a
+    256 x 256 x 64 slab of the same grid (the reference's rate is the same
+    per cell at 256^3; its i-outer / k-inner loop order strides whole
+    xy-planes), one thread."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    nx, ny, nz = args.n, args.n, max(3, args.n // 4)
+    p = PKG.abi.default_params_py()
+    p.h1, p.h2, p.h3 = PKG.workloads.INI_H
+    p.dt, p.beta1, p.beta2 = 1e-4, 1e-4, 1e-8
+    p.NX, p.Ny, p.Nz, p.source = nx, ny, nz, 2
+    PKG.abi.finalize_patch_py(p)
+    g = PKG.grid.HostGrid(p)
+    steps = 1
+    if oracle.ref_available():
+        impl, kind = oracle.RefCPU(), "reference"
+    else:
+        impl, kind = oracle.Oracle(), "port"
+        impl.set_loop_order(1)
+    impl.initial_data(g, p)
+    t0 = time.perf_counter()
+    impl.run(g, p, steps)
+    dt = time.perf_counter() - t0
+    cpu = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": nx * ny * nz * steps / dt, "unit": "cell-updates/s", "cores": 1,
+            "kind": kind,
+            "sample": f"{steps} step of a {nx}x{ny}x{nz} slab (media table, same h/dt/beta), "
+                      f"{dt:.1f} s on 1 thread of {cpu}"}
+
+
+def load_traffic(workload_tag, kernel):
+    path = os.path.join(PROFILES, f"pmc_{kernel}_{workload_tag}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        return json.load(fh).get("bytes_per_launch")
+
+
+def run_plain(args):
+    from importlib import import_module
+    hiprt = import_module(PKG.__name__ + ".hiprt")
+    be = PKG.device.Backend()
+    p, perm, poro = PKG.workloads.c2_workload(args.n)
+    r = PKG.workloads.stability(p, perm, poro)
+    if r > 2.0:
+        raise SystemExit(f"stability guard: r = {r:.3g} > 2 for this workload")
+    g = PKG.workloads.initial_grid(p, perm, poro)
+    d = be.grid(p, perm=True, poro=True)
+    d.upload(g)
+    lib = d.lib
+    stream = lib.filt_get_stream()
+    fref, pref = ctypes.byref(d.f), ctypes.byref(d.params)
+    nx = d.nx
+    calls = (
+        lambda t: lib.filt_ro_P2_Xi_calculation(fref, pref, nx, 0, 1),
+        lambda t: lib.filt_u_calculation(fref, pref, nx, 0, 1),
+        lambda t: lib.filt_roS_calculation(fref, pref, t, nx, 0, 1),
+        lambda t: lib.filt_P1_S2_calculation(fref, pref, nx, 0, 1),
+        lambda t: lib.filt_boundary_conditions(fref, pref, nx, 0, 1),
+    )
+    for s in range(1, args.warmup + 1):
+        for c in calls:
+            PKG.abi.check(lib, c(s * p.dt), "warmup")
+    d.sync()
+    K = args.steps
+    ev = [[hiprt.Event() for _ in range(len(calls) + 1)] for _ in range(K)]
+    hiprt.device_synchronize()
+    t0 = time.perf_counter()
+    for s in range(K):
+        t = (args.warmup + s + 1) * p.dt
+        e = ev[s]
+        e[0].record(stream)
+        for q, c in enumerate(calls):
+            PKG.abi.check(lib, c(t), "step")
+            e[q + 1].record(stream)
+    hiprt.device_synchronize()
+    wall = time.perf_counter() - t0
+    stage_ms = {name: sum(ev[s][q].elapsed_ms(ev[s][q + 1]) for s in range(K)) / K
+                for q, name in enumerate(PLAIN_STAGES)}
+    out = g.copy()
+    d.download(out, ["P1", "S2"])
+    d.free()
+    import numpy as np
+    finite = bool(np.isfinite(out["P1"]).all() and np.isfinite(out["S2"]).all())
+    return p, wall, stage_ms, r, finite
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=256, help="grid edge")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(multi-GPU sharding is described, not built)"}))
+        raise SystemExit(2)
+
+    p, wall, stage_ms, r, finite = run_plain(args)
+    cells = p.NX * p.Ny * p.Nz
+    value = cells * args.steps / wall
+    dom = max((s for s in PLAIN_STAGES if PLAIN_BYTES[s]), key=lambda s: stage_ms[s])
+    dom_bytes = PLAIN_BYTES[dom] * cells
+    achieved = dom_bytes / (stage_ms[dom] * 1e-3)
+    tag = f"c2_{args.n}"
+    traffic = load_traffic(tag, KERNEL_NAMES[dom])
+    step_bytes = sum(PLAIN_BYTES.values()) * cells
+    line = {
+        "metric": "cell-updates/s",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"plain FP64 drop-in, {args.n}^3, 5 defines.h stages per step, "
+                        "per-cell log-normal K (sigma 1, corr 8 cells) and porosity",
+            "grid": [p.NX, p.Ny, p.Nz],
+            "form": "plain",
+            "deviations": "beta1=1e-4, beta2=1e-8 (stated 4.4e-10 is unstable); "
+                          "defines.ini spacing",
+            "stability_r": r,
+            "state_finite": finite,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": KERNEL_NAMES[dom],
+            "achieved": achieved / 1e9,
+            "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK,
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": dom_bytes,
+        },
+        "stages_ms": stage_ms,
+        "step_roofline": {
+            "bytes_per_cell_update": sum(PLAIN_BYTES.values()),
+            "achieved_GBps": step_bytes * args.steps / wall / 1e9,
+            "frac": step_bytes * args.steps / wall / HBM_PEAK,
+        },
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(args)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

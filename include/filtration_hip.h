@@ -1,0 +1,266 @@
+/*
+ * filtration_hip.h -- C ABI of the MI355X (gfx950) backend for the two-phase
+ * filtration time step of shmem-cuda-filtration.
+ *
+ * The reference selects its compute backend at link time: gpu.cu or cpu.cpp
+ * provides the functions declared in defines.h:66-116 (CUDA/CUDA.vcxproj:17-21,
+ * CPU/CPU.vcxproj:81-83).  This header is the language-neutral layer under that
+ * contract: extern "C", plain pointers and sizes, int status codes.  The C++
+ * shims in shmem-cuda-filtration_amd/host/backend_defines.cpp implement the
+ * defines.h functions (C++ linkage, same mangled names) on top of it, so the
+ * reference driver main.cpp links against this backend unchanged.
+ *
+ * Every entry point that launches GPU work enqueues it on the library stream
+ * (filt_set_stream, default: a stream owned by the library) and returns
+ * without waiting, unless its comment says otherwise.  filt_synchronize()
+ * waits for all of it.  Copies to and from the host are synchronous, as the
+ * reference's cudaMemcpy calls are (gpu.cu:483-501).
+ *
+ * Grid layout (defines.h:41-47, cpu.cpp throughout): one array per field,
+ * idx = i + j*localNx + k*localNx*Ny, x fastest, j vertical (j = 0 is the top
+ * face, gravity acts toward +j), k slowest.  All physics is FP64; media is int32.
+ */
+#ifndef FILTRATION_HIP_H
+#define FILTRATION_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes returned by every int-returning entry point. */
+#define FILT_OK 0
+#define FILT_ERR_INVALID 1     /* null pointer, bad size or shape */
+#define FILT_ERR_HIP 2         /* a HIP runtime call failed */
+#define FILT_ERR_NOMEM 3       /* device allocation failed */
+#define FILT_ERR_UNSTABLE 4    /* the explicit-scheme stability guard refused */
+#define FILT_ERR_IO 5          /* file could not be opened or read */
+#define FILT_ERR_UNSUPPORTED 6 /* valid request this build does not implement */
+#define FILT_ERR_NOT_FOUND 7   /* unknown slot / handle */
+
+/* Largest media table.  The reference has two media (defines.h:22-26). */
+#define FILT_MAX_MEDIA 8
+
+/*
+ * Problem parameters: a superset of consts_tag (defines.h:50-61).
+ * The first block keeps consts_tag's members and meaning (the per-media arrays
+ * are widened from 2 to FILT_MAX_MEDIA entries); the second block holds what
+ * defines.h:8-15 fixes at compile time; the third generalises the
+ * (2*source+1)^2 source patch of main.cpp:109 / cpu.cpp:346 to explicit bounds.
+ */
+typedef struct filt_params {
+    double K[FILT_MAX_MEDIA];      /* absolute permeability per medium, m^2 */
+    double lambda[FILT_MAX_MEDIA]; /* Brooks-Corey pore-size index */
+    double S_wr[FILT_MAX_MEDIA];   /* residual water saturation */
+    double m[FILT_MAX_MEDIA];      /* porosity */
+    double P_d[FILT_MAX_MEDIA];    /* entry (threshold) pressure, Pa */
+    double h1, h2, h3;             /* node spacing in x, y, z, m */
+    double dt;                     /* time step, s */
+    double tau, l_w, l_n, c;       /* divgrad-scheme constants (unused on this path) */
+    double beta1, beta2;           /* compressibility of water / NAPL, 1/Pa */
+    int NX, Ny, Nz;                /* global grid */
+    int source;                    /* half-width of the reference's source patch */
+    int save_plots, print_screen;  /* driver output cadence */
+    /* defines.h:8-15 */
+    double mu1, mu2;               /* viscosity of water / NAPL, Pa*s */
+    double ro01, ro02;             /* reference density of water / NAPL, kg/m^3 */
+    double S2_gr;                  /* NAPL saturation held on the source patch */
+    double P_atm;                  /* pressure on the top face, Pa */
+    double g_const;                /* gravity, m/s^2 */
+    int newton_iterations;         /* Newton iterations per cell and step */
+    /* source patch on j = 0: src_x0 <= I <= src_x1 and src_z0 <= k <= src_z1 */
+    int src_x0, src_x1, src_z0, src_z1;
+    int n_media;                   /* entries in use in the media table */
+} filt_params;
+
+/*
+ * Field arrays of one grid, host or device: the members of ptr_Arrays_tag
+ * (defines.h:41-47) in the same order, plus two optional per-cell property
+ * fields.  When perm (poro) is non-null it replaces K[media] (m[media]) cell
+ * by cell; lambda, S_wr and P_d always come from the media table.
+ */
+typedef struct filt_fields {
+    double *x, *y, *z;
+    double *P1, *P2, *S2, *ro1, *ro2;
+    double *u1x, *u1y, *u1z, *u2x, *u2y, *u2z;
+    double *Xi1, *Xi2, *roS1, *roS1_old, *roS2, *roS2_old;
+    int *media;
+    double *perm;
+    double *poro;
+} filt_fields;
+
+/* ---------------------------------------------------------------- setup -- */
+
+/* Fill *p with the reference's compile-time defaults: defines.h:8-15 and
+ * the media table of defines.h:22-26 (the ini-controlled members are zeroed,
+ * as read_defines leaves them before parsing, main.cpp:601-612).
+ * The source patch bounds are derived from NX, Nz, source by
+ * filt_params_finalize. */
+void filt_default_params(filt_params* p);
+
+/* Derive src_x0..src_z1 from NX, Nz and source the way main.cpp:109 and
+ * cpu.cpp:346 test the patch: NX/2-source <= I <= NX/2+source,
+ * Nz/2-source <= k <= Nz/2+source.  Leaves explicit bounds alone when the
+ * ini gave them (src_x0 >= 0). */
+void filt_params_finalize(filt_params* p);
+
+/* Parse a defines.ini file into *p (main.cpp:601-683): '#' lines are
+ * skipped, a key runs to '=', a value to the first space, the last
+ * occurrence wins, unknown keys are ignored.  Reads the 16 keys the reference
+ * reads plus the keys defines.ini documents but the reference ignores
+ * (K[i], LAMBDA[i], S_WR[i], M[i], P_D[i], MU1, MU2, RO_W, RO_N, S2_GR,
+ * P_ATM, G_CONST) and this backend's extensions (NEWTON_ITERATIONS,
+ * SRC_X0, SRC_X1, SRC_Z0, SRC_Z1, N_MEDIA).  Keys absent from the file keep
+ * their value in *p.  Returns FILT_ERR_IO when the file cannot be opened
+ * (the reference prints and continues, main.cpp:620-624). */
+int filt_read_defines(const char* path, filt_params* p);
+
+/* Stability guard of the explicit scheme: the amplification number
+ * r = 4*dt*D*(1/h1^2+1/h2^2+1/h3^2) with the water diffusivity
+ * D = K/(mu1*phi*beta1), maximised over the media table and, when given,
+ * over the per-cell fields (perm, poro, n_cells long; either may be null).
+ * The scheme is stable for r <= 2. */
+double filt_stability_number(const filt_params* p, const double* perm,
+                             const double* poro, size_t n_cells);
+
+/* Seeded synthetic property fields (host arrays).  Draws come from a
+ * counter-based generator (splitmix64 of seed, stream and cell index), so a
+ * field does not depend on thread count.
+ * filt_gen_uniform: out[i] = lo + (hi-lo)*U(0,1].
+ * filt_gen_lognormal: K = exp(ln(k_mean) + sigma*z) on an nx*ny*nz grid
+ * (x fastest), z = Gaussian white noise convolved with a separable 3D
+ * Gaussian of std corr_cells/sqrt(2) cells (periodic wrap; corr_cells <= 0:
+ * no filtering), renormalised to zero mean and unit variance and clipped to
+ * [-clip_sigma, clip_sigma] when clip_sigma > 0.  threads <= 0: all cores
+ * (at most 16). */
+int filt_gen_uniform(double* out, size_t n, uint64_t seed, double lo, double hi);
+int filt_gen_lognormal(double* out, int nx, int ny, int nz, uint64_t seed, double k_mean,
+                       double sigma, double corr_cells, double clip_sigma, int threads);
+
+/* ---------------------------------------------------- driver helpers -- */
+
+/* i_to_I (main.cpp:145-158): local x index -> global x index. */
+int filt_i_to_I(int i, int rank, int size, const filt_params* p);
+/* N_to_local (main.cpp:166-189): local extents of rank's x slab. */
+void filt_N_to_local(int* localNx, int* localNy, int size, int rank, const filt_params* p);
+/* is_active_point (main.cpp:192-198): 0 for halo cells of inner ranks. */
+int filt_is_active_point(int i, int localNx, int rank, int size);
+/* initial_data (main.cpp:97-138) on host arrays: S2 = S2_gr on the source
+ * patch of j = 0, else 0; P1 = P_atm + j*ro01*g*h2; x, y, z, media filled
+ * when non-null. */
+int filt_initial_data(const filt_fields* host, const filt_params* p, int localNx, int rank,
+                      int size);
+
+/* ---------------------------------------------------- device management -- */
+
+/* Device_Initialize (defines.h:82, gpu.cu:565-610): select device
+ * rank % device_count, report the launch-block counts the reference would use
+ * (localNx/16, Ny/4, Nz/4 rounded up) and check that the grid fits. */
+int filt_device_initialize(int rank, int localNx, const filt_params* p,
+                           int* blocksX, int* blocksY, int* blocksZ);
+
+/* device_memory_alloc (defines.h:68, gpu.cu:504-531): 20 FP64 arrays and one
+ * int32 array of nX*Ny*Nz cells, plus a 2*Ny*Nz exchange buffer.  perm and
+ * poro are set to null.  One allocation backs all arrays; each array starts
+ * on a 256-byte boundary. */
+int filt_device_memory_alloc(filt_fields* dev, double** dev_buffer, int nX,
+                             const filt_params* p);
+
+/* device_memory_free (defines.h:71, gpu.cu:534-560). */
+int filt_device_memory_free(const filt_fields* dev, double* dev_buffer);
+
+/* load_data_to_host / load_data_to_device / load_data_to_device_int
+ * (defines.h:78-80, gpu.cu:483-501): synchronous copies of localNx*Ny*Nz
+ * elements.  They wait for the library stream first. */
+int filt_load_data_to_host(double* host, const double* dev, int localNx,
+                           const filt_params* p);
+int filt_load_data_to_device(const double* host, double* dev, int localNx,
+                             const filt_params* p);
+int filt_load_data_to_device_int(const int* host, int* dev, int localNx,
+                                 const filt_params* p);
+
+/* Raw synchronous copies of n bytes (direction 0: host->device,
+ * 1: device->host, 2: device->device). */
+int filt_memcpy(void* dst, const void* src, size_t n_bytes, int direction);
+
+/* Allocate / free device memory for per-cell property fields and the like. */
+int filt_device_malloc(void** ptr, size_t n_bytes);
+int filt_device_free_ptr(void* ptr);
+
+/* ---------------------------------------------------------- the stages -- */
+/* Each stage enqueues its kernels and returns.  localNx is the local x
+ * extent, rank/size the x-slab decomposition of main.cpp:145-198 (size 1 on
+ * a single GPU). */
+
+/* ro_P2_Xi_calculation (defines.h:106; cpu.cpp:3-15, 57-74): P2, Xi1, Xi2,
+ * ro1, ro2 from P1, S2 and the media, in every active cell. */
+int filt_ro_P2_Xi_calculation(const filt_fields* dev, const filt_params* p,
+                              int localNx, int rank, int size);
+
+/* u_calculation (defines.h:108; cpu.cpp:17-24, 77-160): Darcy velocities
+ * u1x..u2z, one-sided at the faces, central inside. */
+int filt_u_calculation(const filt_fields* dev, const filt_params* p,
+                       int localNx, int rank, int size);
+
+/* roS_calculation (defines.h:110; cpu.cpp:26-33, 218-291): explicit upwind
+ * update of ro*S in interior cells; writes roS1, roS2, roS1_old, roS2_old.
+ * t is accepted and unused, as in the reference. */
+int filt_roS_calculation(const filt_fields* dev, const filt_params* p,
+                         double t, int localNx, int rank, int size);
+
+/* P1_S2_calculation (defines.h:111; cpu.cpp:35-42, 293-320): per-cell
+ * Newton solve (newton_iterations iterations) for P1 and S2 from roS1, roS2,
+ * in interior cells. */
+int filt_P1_S2_calculation(const filt_fields* dev, const filt_params* p,
+                           int localNx, int rank, int size);
+
+/* boundary_conditions (defines.h:112; cpu.cpp:44-54, 323-403): the
+ * reference's in-place i->j->k sweep over the shell cells, computed in
+ * closed form by two kernels; grids with an extent below 3 use a sequential
+ * kernel that replays the sweep. */
+int filt_boundary_conditions(const filt_fields* dev, const filt_params* p,
+                             int localNx, int rank, int size);
+
+/* One whole time step in the order of time_step_function (main.cpp:83-94);
+ * the exchanges are no-ops on one GPU (no_communication.cpp:3-14).
+ * with_u = 0 skips u_calculation, which nothing on the P/S path reads. */
+int filt_time_step(const filt_fields* dev, const filt_params* p, double t,
+                   int localNx, int rank, int size, int with_u);
+
+/* load_exchange_data / save_exchange_data (defines.h:115-116,
+ * gpu.cu:613-653): pack x-planes i=1 and i=localNx-2 of dev_array into
+ * dev_buffer and copy them to host_buffer (2*Ny*Nz doubles); unpack
+ * host_buffer into planes i=0 (rank != 0) and i=localNx-1 (rank != size-1).
+ * Synchronous.  Only a multi-rank driver calls them. */
+int filt_load_exchange_data(const double* dev_array, double* dev_buffer,
+                            double* host_buffer, int localNx,
+                            const filt_params* p);
+int filt_save_exchange_data(double* dev_array, double* dev_buffer,
+                            const double* host_buffer, int localNx, int rank,
+                            int size, const filt_params* p);
+
+/* ------------------------------------------------------------- runtime -- */
+
+/* Wait for all work the library enqueued. */
+int filt_synchronize(void);
+
+/* Use the given hipStream_t for all later work (null: the library's own
+ * stream).  The caller keeps ownership of the stream. */
+int filt_set_stream(void* hip_stream);
+
+/* The hipStream_t the library currently enqueues on. */
+void* filt_get_stream(void);
+
+/* Text of the last error on the calling thread ("" when none). */
+const char* filt_last_error(void);
+
+/* Version / build string. */
+const char* filt_build_info(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FILTRATION_HIP_H */

@@ -1,0 +1,98 @@
+// ref_harness.cpp -- drives the reference's own CPU path from Python.
+//
+// TEST INFRASTRUCTURE ONLY.  oracle/Makefile compiles this file together
+// with the unmodified reference sources main.cpp (its main renamed by
+// -Dmain=reference_main), cpu.cpp and no_communication.cpp, found under the
+// reference tree given by REF, into oracle/_ref/libref.so.  The functions
+// below only marshal arguments: every number comes from the reference's code.
+// Nothing here is shipped.
+#include "defines.h"  // the reference's contract (defines.h:41-116), via -I$(REF)
+
+#include <unistd.h>
+
+extern double* HostBuffer;  // main.cpp:6
+
+extern "C" {
+
+size_t ref_sizeof_consts(void) { return sizeof(consts); }
+size_t ref_sizeof_ptr_arrays(void) { return sizeof(ptr_Arrays); }
+
+// Reference compile-time constants (defines.h:8-15), for the Python side to
+// check that it mirrors them.
+void ref_constants(double* out)
+{
+    out[0] = mu1;
+    out[1] = mu2;
+    out[2] = ro01;
+    out[3] = ro02;
+    out[4] = S2_gr;
+    out[5] = P_atm;
+    out[6] = g_const;
+    out[7] = NEWTON_ITERATIONS;
+}
+
+// One stage of time_step_function (main.cpp:83-94) on host arrays:
+// 1 ro_P2_Xi_calculation, 2 u_calculation, 3 roS_calculation,
+// 4 P1_S2_calculation, 5 boundary_conditions.
+int ref_stage(int stage, const ptr_Arrays* host, const consts* def, int localNx, int rank,
+              int size, double t)
+{
+    ptr_Arrays h = *host;
+    ptr_Arrays d = *host;  // the CPU backend ignores the device bundle
+    switch (stage) {
+    case 1: ro_P2_Xi_calculation(h, d, *def, localNx, rank, size, 0, 0, 0); return 0;
+    case 2: u_calculation(h, d, localNx, rank, size, 0, 0, 0, *def); return 0;
+    case 3: roS_calculation(h, d, *def, t, localNx, rank, size, 0, 0, 0); return 0;
+    case 4: P1_S2_calculation(h, d, *def, localNx, rank, size, 0, 0, 0); return 0;
+    case 5: boundary_conditions(h, d, localNx, rank, size, 0, 0, 0, *def); return 0;
+    }
+    return 1;
+}
+
+// time_step_function (main.cpp:83-94), nsteps times, t = s*dt as main.cpp:43.
+void ref_run(const ptr_Arrays* host, const consts* def, int localNx, int localNy, int rank,
+             int size, int step0, int nsteps)
+{
+    ptr_Arrays h = *host;
+    for (int s = step0 + 1; s <= step0 + nsteps; s++)
+        time_step_function(h, h, HostBuffer, *def, s * def->dt, localNx, localNy, rank, size,
+                           0, 0, 0);
+}
+
+// initial_data (main.cpp:97-138).
+void ref_initial_data(const ptr_Arrays* host, const consts* def, int localNx, int localNy,
+                      int rank, int size)
+{
+    int t = 0;
+    initial_data(*host, &t, localNx, localNy, rank, size, *def);
+}
+
+int ref_i_to_I(int i, int rank, int size, const consts* def) { return i_to_I(i, rank, size, *def); }
+
+void ref_N_to_local(int* localNx, int* localNy, int size, int rank, const consts* def)
+{
+    N_to_local(localNx, localNy, size, rank, *def);
+}
+
+int ref_is_active_point(int i, int localNx, int rank, int size)
+{
+    return is_active_point(i, localNx, rank, size);
+}
+
+// read_defines (main.cpp:601-683).  The reference opens the fixed relative
+// path DEFINES_FILE (defines.h:5); the caller lays the file out under
+// workdir and this changes into workdir for the call.
+int ref_read_defines(const char* workdir, consts* out)
+{
+    char old[4096];
+    if (!getcwd(old, sizeof old)) return 1;
+    if (chdir(workdir) != 0) return 2;
+    char arg0[] = "ref";
+    char* argv[] = {arg0, nullptr};
+    read_defines(1, argv, out);
+    return chdir(old) != 0 ? 3 : 0;
+}
+
+const char* ref_defines_file(void) { return DEFINES_FILE; }
+
+}  // extern "C"

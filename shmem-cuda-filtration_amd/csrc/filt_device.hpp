@@ -1,0 +1,136 @@
+// filt_device.hpp -- per-cell arithmetic of the filtration step, device side.
+//
+// Every expression keeps the operation order of the reference's CPU path
+// (cpu.cpp:57-74, 77-160, 218-291, 293-320).  The library is compiled with
+// -ffp-contract=off, so apart from pow() every FP64 result here rounds
+// exactly as the CPU build does (AMDGPU FP64 division is IEEE-correct).
+// pow() is ROCm's, except for the exact shortcuts in pow_spec().
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "../../include/filtration_hip.h"
+
+namespace filt {
+
+// Flat index i + j*nx + k*nx*ny (defines.h:41-47 layout), 64-bit.
+__host__ __device__ inline long long idx3(int i, int j, int k, int nx, int ny)
+{
+    return (long long)i + (long long)j * nx + (long long)k * nx * (long long)ny;
+}
+
+// pow(x, y) with exact shortcuts for the exponents the configurations use.
+// y == 2: x*x is the correctly rounded square.  y == 4: x^4 from the exact
+// square (fma error term) squared again, rounded once, i.e. within a hair of
+// the correctly rounded value, which glibc's pow also returns.  Any other
+// case (and non-finite or tiny x) goes to the library pow.
+__device__ inline double pow_spec(double x, double y)
+{
+    if (y == 2.0)
+        return x * x;
+    if (y == 4.0 && fabs(x) > 1e-60 && fabs(x) < 1e60) {
+        const double p = x * x;
+        const double e = fma(x, x, -p);          // x*x = p + e exactly
+        const double hi = p * p;
+        const double lo = fma(p, p, -hi);        // p*p = hi + lo exactly
+        return hi + (lo + 2.0 * p * e);          // (p+e)^2 less e^2
+    }
+    return pow(x, y);
+}
+
+// Stage 1 for one cell: cpu.cpp:57-74 (assign_P2_Xi1_Xi2, assign_ro1_ro2).
+struct Stage1 {
+    double P2, Xi1, Xi2, ro1, ro2;
+};
+
+__device__ inline Stage1 stage1_cell(double P1, double S2, double K, double lam, double swr,
+                                     const filt_params& p)
+{
+    Stage1 r;
+    const double S_e = (1. - S2 - swr) / (1. - swr);
+    const double k1 = pow_spec(S_e, (2. + 3. * lam) / lam);
+    const double k2 = (1. - S_e) * (1. - S_e) * (1 - pow_spec(S_e, (2. + lam) / lam));
+    const double P_k = 0;  // capillary pressure disabled, cpu.cpp:63
+    r.P2 = P1 + P_k;
+    r.Xi1 = -1. * K * k1 / p.mu1;
+    r.Xi2 = -1. * K * k2 / p.mu2;
+    r.ro1 = p.ro01 * (1. + p.beta1 * (P1 - p.P_atm));
+    r.ro2 = p.ro02 * (1. + p.beta2 * (r.P2 - p.P_atm));
+    return r;
+}
+
+// Upwind flux difference along one axis (cpu.cpp:242-244 and siblings).
+// a2, a1: forces on the + and - faces; X*, r*: Xi and ro at centre, -, +.
+__device__ inline double upwind(double a2, double a1, double Xc, double rc, double Xm, double rm,
+                                double Xp, double rp, double h)
+{
+    return (((a2 + fabs(a2)) / 2.0 - (a1 - fabs(a1)) / 2.0) * (-1) * Xc * rc -
+            (a1 + fabs(a1)) / 2.0 * (-1) * Xm * rm +
+            (a2 - fabs(a2)) / 2.0 * (-1) * Xp * rp) / h;
+}
+
+// Newton iterations of cpu.cpp:293-320 on one cell, in registers.
+// With P_d == 0 the capillary terms P_k = P_d*S_e^(-1/lam) and
+// PkS ~ P_d*S_e^(-1/lam-1) are exactly zero whenever those powers are finite,
+// which holds for S_e >= 2^-10 and lam >= 0.1 (|exponent| <= 11); the pows
+// are then skipped.  Elsewhere they are evaluated as the reference does, so
+// inf*0 = NaN propagates identically.
+__device__ inline void newton_cell(double& P, double& S, double roS1, double roS2, double lam,
+                                   double swr, double pd, const filt_params& p)
+{
+    const double ro01 = p.ro01, ro02 = p.ro02, b1 = p.beta1, b2 = p.beta2, Pa = p.P_atm;
+    const bool no_cap = (pd == 0.0) && (lam >= 0.1);
+    for (int w = 1; w <= p.newton_iterations; w++) {
+        const double S_e = (1 - S - swr) / (1 - swr);
+        double P_k, AAA;
+        if (no_cap && S_e >= 0.0009765625) {
+            P_k = 0.0;
+            AAA = 0.0;  // finite; only AAA * pd (= 0) is used
+        } else {
+            P_k = pd * pow(S_e, (-1.) / lam);
+            AAA = pow(S_e, ((-1.) / lam) - 1.);
+        }
+        const double F1 = ro01 * (1. + b1 * (P - Pa)) * (1. - S) - roS1;
+        const double F2 = ro02 * (1. + b2 * (P + P_k - Pa)) * S - roS2;
+        const double PkS = AAA * pd / (lam * (1 - swr));
+        const double F1P = ro01 * b1 * (1 - S);
+        const double F2P = ro02 * b2 * S;
+        const double F1S = (-1) * ro01 * (1 + b1 * (P - Pa));
+        const double F2S = ro02 * (1 + b2 * (P + P_k - Pa + (S * PkS)));
+        const double det = F1P * F2S - F1S * F2P;
+        const double Pn = P - (1 / det) * (F2S * F1 - F1S * F2);
+        const double Sn = S - (1 / det) * (F1P * F2 - F2P * F1);
+        P = Pn;
+        S = Sn;
+    }
+}
+
+// Source patch test of main.cpp:109 / cpu.cpp:346, generalised to bounds.
+__host__ __device__ inline bool in_patch(int I, int k, const filt_params& p)
+{
+    return I >= p.src_x0 && I <= p.src_x1 && k >= p.src_z0 && k <= p.src_z1;
+}
+
+// main.cpp:145-158
+__host__ __device__ inline int i_to_I(int i, int rank, int size, const filt_params& p)
+{
+    const int base = p.NX / size, rem = p.NX % size;
+    if (rank <= rem)
+        return rank == 0 ? i : (base + 1) * rank + i - 1;
+    return (base + 1) * rank - (rank - rem) + i - 1;
+}
+
+// main.cpp:192-198
+__host__ __device__ inline bool is_active(int i, int nx, int rank, int size)
+{
+    return !((rank != 0 && i == 0) || (rank != size - 1 && i == nx - 1));
+}
+
+// Interior test of cpu.cpp:220 / 295.
+__host__ __device__ inline bool is_interior(int i, int j, int k, int nx, int ny, int nz)
+{
+    return i != 0 && i != nx - 1 && j != 0 && j != ny - 1 && ((k != 0 && k != nz - 1) || nz < 2);
+}
+
+}  // namespace filt

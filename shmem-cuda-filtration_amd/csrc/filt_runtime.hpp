@@ -1,0 +1,37 @@
+// filt_runtime.hpp -- internal runtime helpers shared by the .hip sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/filtration_hip.h"
+
+namespace filt {
+
+// The stream all work is enqueued on (created on first use).
+hipStream_t stream();
+
+// Record an error message for filt_last_error() and return code.
+int set_error(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+
+// Map a failed HIP call to FILT_ERR_HIP with its text.
+int hip_fail(hipError_t e, const char* where);
+
+// Clear the calling thread's error text.
+void clear_error();
+
+// Validate a grid's shape against params (non-null, positive extents).
+int check_shape(const filt_params* p, int localNx);
+
+}  // namespace filt
+
+#define FILT_HIP_TRY(call)                                   \
+    do {                                                     \
+        hipError_t e_ = (call);                              \
+        if (e_ != hipSuccess) return filt::hip_fail(e_, #call); \
+    } while (0)
+
+#define FILT_LAUNCH_CHECK(name)                              \
+    do {                                                     \
+        hipError_t e_ = hipGetLastError();                   \
+        if (e_ != hipSuccess) return filt::hip_fail(e_, name); \
+    } while (0)

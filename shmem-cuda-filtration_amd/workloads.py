@@ -1,0 +1,116 @@
+"""The benchmark configurations as parameter sets and seeded initial states.
+
+Shared deviations from the configuration text, applied identically to every
+path and to the oracle:
+  * compressibility beta1 = 1e-4, beta2 = 1e-8 (the reference's defines.ini
+    values) instead of 4.4e-10: with 4.4e-10 the explicit scheme's
+    amplification number is ~435 at 32^3 (stable needs <= 2) and the state
+    reaches NaN by step 6;
+  * 256^3 and larger grids use the defines.ini node spacing
+    (h1 = 0.02045 m, h2 = h3 = 0.01969 m) instead of a 1 m cube;
+  * log-normal ln K is clipped at +-2.5 sigma for sigma >= 1.5.
+filt_stability_number reports the guard value for each set.
+"""
+import ctypes
+
+import numpy as np
+
+from . import abi
+from .grid import HostGrid
+
+K_MEAN = 6.64e-11
+INI_H = (0.02045, 0.01969, 0.01969)
+STABLE_BETA = (1.0e-4, 1.0e-8)
+
+
+def base_params(nx, ny, nz, h=None, dt=1e-4, beta=STABLE_BETA, sn=0.5, patch=4):
+    """Fluids of the configurations: K = 6.64e-11, phi = 0.4, water
+    ro0 = 1000 / mu = 1e-3, NAPL ro0 = 1460 / mu = 0.9e-3, Brooks-Corey
+    lambda = 2, S_wr = 0.1, P_d = 0 (one medium); Sn on a patch x patch
+    source centred on the top face."""
+    p = abi.default_params_py()
+    if h is None:
+        h = (1.0 / (nx - 1), 1.0 / (ny - 1), 1.0 / (nz - 1))
+    p.h1, p.h2, p.h3 = h
+    p.dt = dt
+    p.beta1, p.beta2 = beta
+    p.NX, p.Ny, p.Nz = nx, ny, nz
+    for q in range(abi.FILT_MAX_MEDIA):
+        p.K[q], p.lambda_[q], p.S_wr[q], p.m[q], p.P_d[q] = K_MEAN, 2.0, 0.1, 0.4, 0.0
+    p.n_media = 1
+    p.S2_gr = sn
+    set_patch(p, nx // 2 - patch // 2, nz // 2 - patch // 2, patch)
+    p.save_plots, p.print_screen = 100000, 1000
+    return p
+
+
+def set_patch(p, x0, z0, size=4):
+    p.src_x0, p.src_x1 = x0, x0 + size - 1
+    p.src_z0, p.src_z1 = z0, z0 + size - 1
+    return p
+
+
+def c1_params():
+    """32^3 over a 1 m cube, homogeneous, 500 steps."""
+    return base_params(32, 32, 32)
+
+
+def c2_params(n=256):
+    """256^3, per-cell log-normal K (sigma 1, correlation 8 cells) and
+    porosity U[0.35, 0.45], 200 steps."""
+    return base_params(n, n, n, h=INI_H)
+
+
+def c3_params(nx=512, ny=512, nz=256):
+    """One rank's share: 512 x 512 x 256, sigma 1.5, correlation 16 cells."""
+    return base_params(nx, ny, nz, h=INI_H)
+
+
+def gen_fields(p, seed, sigma, corr, clip=0.0, poro=(0.35, 0.45), threads=0):
+    """Per-cell permeability and porosity (x-fastest, shape (Nz, Ny, NX))."""
+    lib = abi.load_library()
+    shape = (p.Nz, p.Ny, p.NX)
+    perm = np.empty(shape, np.float64)
+    phi = np.empty(shape, np.float64)
+    abi.check(lib, lib.filt_gen_lognormal(perm.ctypes.data, p.NX, p.Ny, p.Nz, seed, K_MEAN,
+                                          sigma, corr, clip, threads), "filt_gen_lognormal")
+    abi.check(lib, lib.filt_gen_uniform(phi.ctypes.data, phi.size, seed ^ 0x5F3759DF,
+                                        poro[0], poro[1]), "filt_gen_uniform")
+    return perm, phi
+
+
+def initial_grid(p, perm=None, poro=None, coords=False):
+    """HostGrid with the initial state of main.cpp:97-138 (filt_initial_data)
+    and the property fields attached."""
+    lib = abi.load_library()
+    g = HostGrid(p, with_perm=perm is not None, with_poro=poro is not None)
+    if perm is not None:
+        g["perm"] = perm
+    if poro is not None:
+        g["poro"] = poro
+    f = g.fields()
+    if not coords:
+        f.x = f.y = f.z = None
+    abi.check(lib, lib.filt_initial_data(ctypes.byref(f), ctypes.byref(p), g.nx, 0, 1),
+              "filt_initial_data")
+    return g
+
+
+def stability(p, perm=None, poro=None):
+    lib = abi.load_library()
+    n = 0 if perm is None else perm.size
+    return lib.filt_stability_number(ctypes.byref(p),
+                                     None if perm is None else perm.ctypes.data,
+                                     None if poro is None else poro.ctypes.data, n)
+
+
+def c2_workload(n=256, seed=2024):
+    p = c2_params(n)
+    perm, poro = gen_fields(p, seed, sigma=1.0, corr=8.0)
+    return p, perm, poro
+
+
+def c3_workload(nx=512, ny=512, nz=256, seed=3033):
+    p = c3_params(nx, ny, nz)
+    perm, poro = gen_fields(p, seed, sigma=1.5, corr=16.0, clip=2.5)
+    return p, perm, poro
