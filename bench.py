@@ -140,6 +140,33 @@ def run_plain(args):
     return p, wall, stage_ms, r, finite
 
 
+def copy_calibration(n_bytes=1 << 30, reps=5):
+    """The library's streaming copy kernel over n_bytes: the achievable HBM
+    ceiling, and the reference point that scales the PMC byte counters."""
+    hiprt = importlib.import_module(PKG.__name__ + ".hiprt")
+    PKG.device.Backend()
+    lib = PKG.abi.load_library()
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    PKG.abi.check(lib, lib.filt_device_malloc(ctypes.byref(src), n_bytes), "malloc")
+    PKG.abi.check(lib, lib.filt_device_malloc(ctypes.byref(dst), n_bytes), "malloc")
+    stream = lib.filt_get_stream()
+    PKG.abi.check(lib, lib.filt_device_copy(dst, src, n_bytes), "copy")
+    times = []
+    for _ in range(reps):
+        a, b = hiprt.Event(), hiprt.Event()
+        a.record(stream)
+        PKG.abi.check(lib, lib.filt_device_copy(dst, src, n_bytes), "copy")
+        b.record(stream)
+        b.sync()
+        times.append(a.elapsed_ms(b))
+    lib.filt_device_free_ptr(src)
+    lib.filt_device_free_ptr(dst)
+    best = min(times)
+    return {"copy_bytes_moved": 2 * n_bytes, "copy_ms": best,
+            "copy_GBps": 2 * n_bytes / (best * 1e-3) / 1e9,
+            "copy_frac_of_peak": 2 * n_bytes / (best * 1e-3) / HBM_PEAK}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -147,11 +174,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=256, help="grid edge")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy", action="store_true",
+                    help="only run the copy-kernel calibration (1 GiB) and print it")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                                    "(multi-GPU sharding is described, not built)"}))
         raise SystemExit(2)
+    if args.copy:
+        print(json.dumps(copy_calibration()), flush=True)
+        return
 
     p, wall, stage_ms, r, finite = run_plain(args)
     cells = p.NX * p.Ny * p.Nz

@@ -185,6 +185,11 @@ int filt_load_data_to_device_int(const int* host, int* dev, int localNx,
  * 1: device->host, 2: device->device). */
 int filt_memcpy(void* dst, const void* src, size_t n_bytes, int direction);
 
+/* Device-to-device copy of n_bytes (a multiple of 16) by the library's own
+ * streaming kernel, enqueued on the library stream.  Serves as the
+ * reference point of the HBM roofline (its bytes moved are known exactly). */
+int filt_device_copy(void* dst, const void* src, size_t n_bytes);
+
 /* Allocate / free device memory for per-cell property fields and the like. */
 int filt_device_malloc(void** ptr, size_t n_bytes);
 int filt_device_free_ptr(void* ptr);

@@ -242,6 +242,34 @@ int filt_load_data_to_device_int(const int* host, int* dev, int localNx, const f
 
 }  // extern "C"
 
+// ------------------------------------------------------------ copy kernel --
+// 16-byte vector loads and stores, grid-stride; one pass over src and dst.
+__global__ __launch_bounds__(256) void k_copy16(const uint4* __restrict__ src,
+                                                uint4* __restrict__ dst, size_t n16)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride)
+        dst[i] = src[i];
+}
+
+extern "C" int filt_device_copy(void* dst, const void* src, size_t n_bytes)
+{
+    if ((!dst || !src) && n_bytes)
+        return set_error(FILT_ERR_INVALID, "null pointer");
+    if (n_bytes % 16)
+        return set_error(FILT_ERR_INVALID, "n_bytes must be a multiple of 16");
+    const size_t n16 = n_bytes / 16;
+    if (!n16)
+        return FILT_OK;
+    size_t blocks = (n16 + 255) / 256;
+    if (blocks > 256 * 64)
+        blocks = 256 * 64;  // 64 workgroups per CU, grid-stride beyond that
+    hipLaunchKernelGGL(k_copy16, dim3((unsigned)blocks), dim3(256), 0, stream(),
+                       (const uint4*)src, (uint4*)dst, n16);
+    FILT_LAUNCH_CHECK("k_copy16");
+    return FILT_OK;
+}
+
 // ----------------------------------------------------------- halo planes --
 // gpu.cu:613-653: x-planes i = 1 and i = localNx-2 packed as buffer[j + Ny*k]
 // and buffer[Ny*Nz + j + Ny*k]; unpacked into i = 0 and i = localNx-1.

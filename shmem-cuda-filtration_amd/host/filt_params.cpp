@@ -178,7 +178,12 @@ double filt_stability_number(const filt_params* p, const double* perm, const dou
         }
         kphi = best;
     }
-    return 4.0 * p->dt * kphi * factor * inv_h2;
+    // Fluxes carry ro(P) while the storage term carries d(ro)/dP = ro01*beta1:
+    // D grows with the density at the bottom of the initial hydrostatic column,
+    // ro/ro01 = 1 + beta1*ro01*g*(Ny-1)*h2.
+    const double depth = (p->Ny > 1 ? p->Ny - 1 : 0) * p->h2;
+    const double rho_factor = fmax(1.0, 1.0 + p->beta1 * p->ro01 * p->g_const * depth);
+    return 4.0 * p->dt * kphi * factor * rho_factor * inv_h2;
 }
 
 }  // extern "C"

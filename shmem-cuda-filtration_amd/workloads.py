@@ -8,7 +8,12 @@ path and to the oracle:
     reaches NaN by step 6;
   * 256^3 and larger grids use the defines.ini node spacing
     (h1 = 0.02045 m, h2 = h3 = 0.01969 m) instead of a 1 m cube;
-  * log-normal ln K is clipped at +-2.5 sigma for sigma >= 1.5.
+  * log-normal ln K is clipped at +-2.5 sigma: on the 5 m deep 256-cell
+    column the artificial compressibility makes water ~6x denser at the
+    bottom than at the top, which multiplies the diffusivity, and the
+    unclipped tail (K up to ~e^4.7 K_mean) then drives the local
+    amplification number to ~3.7 (NaN within 100 steps, oracle and GPU
+    alike); clipped, the guard reads ~0.4.
 filt_stability_number reports the guard value for each set.
 """
 import ctypes
@@ -106,7 +111,7 @@ def stability(p, perm=None, poro=None):
 
 def c2_workload(n=256, seed=2024):
     p = c2_params(n)
-    perm, poro = gen_fields(p, seed, sigma=1.0, corr=8.0)
+    perm, poro = gen_fields(p, seed, sigma=1.0, corr=8.0, clip=2.5)
     return p, perm, poro
 
 
