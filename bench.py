@@ -39,7 +39,7 @@ PROFILES = os.path.join(ROOT, "profiles")
 #   Newton:   S2, P1, roS1, roS2 in (32); P1, S2 out (16)               = 48
 PLAIN_BYTES = {"ro_P2_Xi": 64, "u": 96, "roS": 96, "newton": 48, "boundary": 0}
 PLAIN_STAGES = ("ro_P2_Xi", "u", "roS", "newton", "boundary")
-KERNEL_NAMES = {"ro_P2_Xi": "k_ro_P2_Xi", "u": "k_u", "roS": "k_roS", "newton": "k_newton",
+KERNEL_NAMES = {"ro_P2_Xi": "k_ro_P2_Xi", "u": "k_u_zm", "roS": "k_roS_zm", "newton": "k_newton",
                 "boundary": "k_bc_a+k_bc_b"}
 
 
@@ -113,6 +113,19 @@ def run_plain(args):
         lambda t: lib.filt_P1_S2_calculation(fref, pref, nx, 0, 1),
         lambda t: lib.filt_boundary_conditions(fref, pref, nx, 0, 1),
     )
+    tuned = None
+    if not args.no_autotune:
+        # stage 1 fills the inputs the stencil kernels read, then the
+        # library's autotuner picks tile shapes for this grid
+        PKG.abi.check(lib, calls[0](0.0), "stage 1")
+        best = (ctypes.c_double * 2)()
+        PKG.abi.check(lib, lib.filt_autotune_stencil(fref, pref, nx, 5, best), "autotune")
+        tuned = {}
+        for kern, name in ((0, "roS"), (1, "u")):
+            v = [ctypes.c_int() for _ in range(4)]
+            lib.filt_get_tile_config(kern, nx, p.Ny, p.Nz, *[ctypes.byref(x) for x in v])
+            tuned[name] = {"tile": [v[0].value, v[1].value], "kz": v[2].value,
+                           "order": v[3].value, "ms": best[kern]}
     for s in range(1, args.warmup + 1):
         for c in calls:
             PKG.abi.check(lib, c(s * p.dt), "warmup")
@@ -137,7 +150,7 @@ def run_plain(args):
     d.free()
     import numpy as np
     finite = bool(np.isfinite(out["P1"]).all() and np.isfinite(out["S2"]).all())
-    return p, wall, stage_ms, r, finite
+    return p, wall, stage_ms, r, finite, tuned
 
 
 def copy_calibration(n_bytes=1 << 30, reps=5):
@@ -174,6 +187,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=256, help="grid edge")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default stencil tile shapes")
     ap.add_argument("--copy", action="store_true",
                     help="only run the copy-kernel calibration (1 GiB) and print it")
     args = ap.parse_args()
@@ -185,7 +200,7 @@ def main():
         print(json.dumps(copy_calibration()), flush=True)
         return
 
-    p, wall, stage_ms, r, finite = run_plain(args)
+    p, wall, stage_ms, r, finite, tuned = run_plain(args)
     cells = p.NX * p.Ny * p.Nz
     value = cells * args.steps / wall
     dom = max((s for s in PLAIN_STAGES if PLAIN_BYTES[s]), key=lambda s: stage_ms[s])
@@ -215,6 +230,7 @@ def main():
             "deviations": "beta1=1e-4, beta2=1e-8 (stated 4.4e-10 is unstable); "
                           "defines.ini spacing",
             "stability_r": r,
+            "autotuned_tiles": tuned,
             "state_finite": finite,
         },
         "roofline": {

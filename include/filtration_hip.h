@@ -246,6 +246,25 @@ int filt_save_exchange_data(double* dev_array, double* dev_buffer,
                             const double* host_buffer, int localNx, int rank,
                             int size, const filt_params* p);
 
+/* Tile shape of the z-marching stencil kernels of u_calculation (kernel 1)
+ * and roS_calculation (kernel 0): tx*ty threads per block (64x4, 32x8,
+ * 16x16 or 64x8), kz planes per z-chunk, order 0 = XCD-grouped tile
+ * numbering, 1 = hardware order.  Used by grids with every extent >= 3.
+ * The setting is process-wide; results do not depend on it. */
+int filt_set_tile_config(int kernel, int tx, int ty, int kz, int order);
+/* The configuration used for an nx*ny*nz grid (tuned or default). */
+int filt_get_tile_config(int kernel, int nx, int ny, int nz, int* tx, int* ty, int* kz,
+                         int* order);
+
+/* Autotuner: times every candidate tile configuration of both stencil
+ * kernels on the given device grid (after stage 1 has filled P2, Xi, ro),
+ * reps launches each (median), and keeps the fastest per kernel for grids of
+ * this shape.  Rewrites only roS*, u* with the values one call would give.
+ * best_ms[0] / best_ms[1] receive the roS / u times when non-null.
+ * filt_set_tile_config clears tuned entries. */
+int filt_autotune_stencil(const filt_fields* dev, const filt_params* p, int localNx, int reps,
+                          double* best_ms);
+
 /* ------------------------------------------------------------- runtime -- */
 
 /* Wait for all work the library enqueued. */

@@ -143,6 +143,9 @@ def load_library(path=HIP_LIB):
     _sig(lib, "filt_load_exchange_data", c_int, c_void_p, c_void_p, c_void_p, c_int, P)
     _sig(lib, "filt_save_exchange_data", c_int, c_void_p, c_void_p, c_void_p, c_int,
          c_int, c_int, P)
+    _sig(lib, "filt_set_tile_config", c_int, c_int, c_int, c_int, c_int, c_int)
+    _sig(lib, "filt_get_tile_config", c_int, c_int, c_int, c_int, c_int, pi, pi, pi, pi)
+    _sig(lib, "filt_autotune_stencil", c_int, F, P, c_int, c_int, POINTER(c_double))
     _sig(lib, "filt_synchronize", c_int)
     _sig(lib, "filt_set_stream", c_int, c_void_p)
     _sig(lib, "filt_get_stream", c_void_p)

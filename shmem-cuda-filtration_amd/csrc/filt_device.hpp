@@ -83,17 +83,20 @@ __device__ inline void newton_cell(double& P, double& S, double roS1, double roS
     const bool no_cap = (pd == 0.0) && (lam >= 0.1);
     for (int w = 1; w <= p.newton_iterations; w++) {
         const double S_e = (1 - S - swr) / (1 - swr);
-        double P_k, AAA;
+        double P_k, PkS;
         if (no_cap && S_e >= 0.0009765625) {
+            // P_d*S_e^(-1/lam) and S_e^(-1/lam-1)*P_d/(lam*(1-swr)) with finite
+            // powers and P_d = 0: both are zero, and a zero PkS enters F2S
+            // only as S*PkS added to a nonzero sum.
             P_k = 0.0;
-            AAA = 0.0;  // finite; only AAA * pd (= 0) is used
+            PkS = 0.0;
         } else {
             P_k = pd * pow(S_e, (-1.) / lam);
-            AAA = pow(S_e, ((-1.) / lam) - 1.);
+            const double AAA = pow(S_e, ((-1.) / lam) - 1.);
+            PkS = AAA * pd / (lam * (1 - swr));
         }
         const double F1 = ro01 * (1. + b1 * (P - Pa)) * (1. - S) - roS1;
         const double F2 = ro02 * (1. + b2 * (P + P_k - Pa)) * S - roS2;
-        const double PkS = AAA * pd / (lam * (1 - swr));
         const double F1P = ro01 * b1 * (1 - S);
         const double F2P = ro02 * b2 * S;
         const double F1S = (-1) * ro01 * (1 + b1 * (P - Pa));

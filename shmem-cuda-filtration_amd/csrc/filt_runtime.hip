@@ -243,13 +243,25 @@ int filt_load_data_to_device_int(const int* host, int* dev, int localNx, const f
 }  // extern "C"
 
 // ------------------------------------------------------------ copy kernel --
-// 16-byte vector loads and stores, grid-stride; one pass over src and dst.
+// 16-byte vector accesses; each thread issues its four loads before its four
+// stores (a block moves 16 KiB); one pass over src and dst.
 __global__ __launch_bounds__(256) void k_copy16(const uint4* __restrict__ src,
                                                 uint4* __restrict__ dst, size_t n16)
 {
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride)
-        dst[i] = src[i];
+    const size_t base = (size_t)blockIdx.x * 1024 + threadIdx.x;
+    uint4 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const size_t i = base + (size_t)u * 256;
+        if (i < n16)
+            v[u] = src[i];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const size_t i = base + (size_t)u * 256;
+        if (i < n16)
+            dst[i] = v[u];
+    }
 }
 
 extern "C" int filt_device_copy(void* dst, const void* src, size_t n_bytes)
@@ -261,9 +273,9 @@ extern "C" int filt_device_copy(void* dst, const void* src, size_t n_bytes)
     const size_t n16 = n_bytes / 16;
     if (!n16)
         return FILT_OK;
-    size_t blocks = (n16 + 255) / 256;
-    if (blocks > 256 * 64)
-        blocks = 256 * 64;  // 64 workgroups per CU, grid-stride beyond that
+    const size_t blocks = (n16 + 1023) / 1024;
+    if (blocks > 0x7fffffffULL)
+        return set_error(FILT_ERR_INVALID, "copy too large");
     hipLaunchKernelGGL(k_copy16, dim3((unsigned)blocks), dim3(256), 0, stream(),
                        (const uint4*)src, (uint4*)dst, n16);
     FILT_LAUNCH_CHECK("k_copy16");

@@ -22,6 +22,12 @@ void clear_error();
 // Validate a grid's shape against params (non-null, positive extents).
 int check_shape(const filt_params* p, int localNx);
 
+// z-marching stencil launchers (filt_stencil.hip), for extents >= 3 and
+// arrays below 4 GiB (tiled_fits).
+bool tiled_fits(int nx, int ny, int nz);
+int launch_roS_zm(const filt_fields& f, const filt_params& p, int nx);
+int launch_u_zm(const filt_fields& f, const filt_params& p, int nx, int rank, int size);
+
 }  // namespace filt
 
 #define FILT_HIP_TRY(call)                                   \

@@ -419,6 +419,13 @@ dim3 cell_grid(const filt_params* p, int nx)
     return dim3((nx + TX - 1) / TX, (p->Ny + TY - 1) / TY, p->Nz);
 }
 
+// The z-marching stencil kernels (filt_stencil.hip) need every extent >= 3;
+// smaller grids take the one-cell-per-thread kernels above.
+bool use_tiled(const filt_params* p, int nx)
+{
+    return nx >= 3 && p->Ny >= 3 && p->Nz >= 3 && tiled_fits(nx, p->Ny, p->Nz);
+}
+
 }  // namespace
 
 extern "C" {
@@ -439,6 +446,8 @@ int filt_u_calculation(const filt_fields* dev, const filt_params* p, int localNx
 {
     int rc = launch_check(dev, p, localNx);
     if (rc) return rc;
+    if (use_tiled(p, localNx) && p->NX > 2)
+        return launch_u_zm(*dev, *p, localNx, rank, size);
     hipLaunchKernelGGL(k_u, cell_grid(p, localNx), dim3(TX, TY), 0, stream(), *dev, *p, localNx,
                        rank, size);
     FILT_LAUNCH_CHECK("k_u");
@@ -451,6 +460,8 @@ int filt_roS_calculation(const filt_fields* dev, const filt_params* p, double t,
     (void)t;  // unused by assign_rS_nr as well (cpu.cpp:218)
     int rc = launch_check(dev, p, localNx);
     if (rc) return rc;
+    if (use_tiled(p, localNx))
+        return launch_roS_zm(*dev, *p, localNx);
     hipLaunchKernelGGL(k_roS, cell_grid(p, localNx), dim3(TX, TY), 0, stream(), *dev, *p, localNx,
                        rank, size);
     FILT_LAUNCH_CHECK("k_roS");

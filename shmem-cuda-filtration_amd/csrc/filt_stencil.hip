@@ -1,0 +1,451 @@
+// filt_stencil.hip -- z-marching LDS-tiled kernels of the plain path's two
+// stencil stages, u_calculation (cpu.cpp:77-160) and roS_calculation
+// (cpu.cpp:218-291), for grids with NX, Ny, Nz >= 3 whose arrays are below
+// 4 GiB each.
+//
+// Each block owns a TX x TY tile of columns and a chunk of kz planes.  Per
+// plane it writes its columns' values into an LDS tile with a one-cell halo
+// (halo cells read from global memory, where neighbouring tiles on the same
+// XCD have just loaded them into L2), synchronises, and evaluates the cell
+// from LDS (x, y neighbours) and registers (its own column's k-1, k, k+1).
+// Every HBM element of the stencil arrays is thus read once per chunk.
+// Addresses are 32-bit byte offsets from the (uniform) array bases, which
+// keeps them in one VGPR each.  The arithmetic is expression for expression
+// the one of filt_stages.hip, so both paths give identical bits.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "../../include/filtration_hip.h"
+#include "filt_device.hpp"
+#include "filt_runtime.hpp"
+#include "filt_tiles.hpp"
+
+namespace filt {
+
+namespace {
+
+template <int TX, int TY>
+struct Lds {
+    static constexpr int SX = TX + 2;
+    static constexpr int SN = SX * (TY + 2);
+    __device__ static inline int at(int a, int y, int x) { return a * SN + y * SX + x; }
+};
+
+__device__ __forceinline__ double ld(const double* base, unsigned off)
+{
+    return *(const double*)((const char*)base + off);
+}
+
+__device__ __forceinline__ void st(double* base, unsigned off, double v)
+{
+    *(double*)((char*)base + off) = v;
+}
+
+__device__ __forceinline__ int ldi(const int* base, unsigned off4)
+{
+    return *(const int*)((const char*)base + off4);
+}
+
+// ---------------------------------------------------------------- roS ----
+// cpu.cpp:218-291 for interior cells; arrays P1, P2, Xi1, Xi2, ro1, ro2 are
+// the stencil operands (index 0..5 below).
+template <int TX, int TY>
+__global__ __launch_bounds__(TX* TY) void k_roS_zm(filt_fields f, filt_params p, int nx,
+                                                   TileMap map, int kz)
+{
+    using L = Lds<TX, TY>;
+    __shared__ double sh[6 * L::SN];
+    int bx, by, bz;
+    map.decode(blockIdx.x, bx, by, bz);
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int i = bx * TX + tx, j = by * TY + ty;
+    const int ny = p.Ny, nz = p.Nz;
+    const bool valid = i < nx && j < ny;
+    const bool inner = i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2;
+    const unsigned sy = 8u * nx, sz = 8u * nx * ny;  // byte strides
+    const int kb = max(bz * kz, 1), ke = min(bz * kz + kz, nz - 1);
+    if (kb >= ke)
+        return;  // uniform over the block
+    const double* const A[6] = {f.P1, f.P2, f.Xi1, f.Xi2, f.ro1, f.ro2};
+    const unsigned col = 8u * i + sy * j;
+    double zm[6], zc[6], zp[6];
+    if (valid) {
+#pragma unroll
+        for (int a = 0; a < 6; a++) {
+            zm[a] = ld(A[a], col + (kb - 1) * sz);
+            zc[a] = ld(A[a], col + kb * sz);
+        }
+    }
+    const int cx = tx + 1, cy = ty + 1;
+    const double g = p.g_const;
+    for (int k = kb; k < ke; k++) {
+        const unsigned c = col + k * sz;
+        if (valid) {
+#pragma unroll
+            for (int a = 0; a < 6; a++) {
+                zp[a] = ld(A[a], c + sz);
+                sh[L::at(a, cy, cx)] = zc[a];
+            }
+            if (ty == 0 && j >= 1)
+#pragma unroll
+                for (int a = 0; a < 6; a++) sh[L::at(a, 0, cx)] = ld(A[a], c - sy);
+            if (ty == TY - 1 && j + 1 < ny)
+#pragma unroll
+                for (int a = 0; a < 6; a++) sh[L::at(a, TY + 1, cx)] = ld(A[a], c + sy);
+            if (tx == 0 && i >= 1)
+#pragma unroll
+                for (int a = 0; a < 6; a++) sh[L::at(a, cy, 0)] = ld(A[a], c - 8u);
+            if (tx == TX - 1 && i + 1 < nx)
+#pragma unroll
+                for (int a = 0; a < 6; a++) sh[L::at(a, cy, TX + 1)] = ld(A[a], c + 8u);
+        }
+        __syncthreads();
+        if (valid && inner) {
+            const double p1 = zc[0], p2 = zc[1];
+            const double X1c = zc[2], X2c = zc[3], r1c = zc[4], r2c = zc[5];
+            const double S = ld(f.S2, c);
+            const double roS1 = r1c * (1 - S);
+            const double roS2 = r2c * S;
+            double a1, a2;
+            a2 = -(zp[0] - p1) / p.h3;
+            a1 = -(p1 - zm[0]) / p.h3;
+            const double f3 = upwind(a2, a1, X1c, r1c, zm[2], zm[4], zp[2], zp[4], p.h3);
+            a2 = -(zp[1] - p2) / p.h3;
+            a1 = -(p2 - zm[1]) / p.h3;
+            const double g3 = upwind(a2, a1, X2c, r2c, zm[3], zm[5], zp[3], zp[5], p.h3);
+
+            const int xp = cx + 1, xm = cx - 1, yp = cy + 1, ym = cy - 1;
+            a2 = -(sh[L::at(0, cy, xp)] - p1) / p.h1;
+            a1 = -(p1 - sh[L::at(0, cy, xm)]) / p.h1;
+            const double f1 = upwind(a2, a1, X1c, r1c, sh[L::at(2, cy, xm)], sh[L::at(4, cy, xm)],
+                                     sh[L::at(2, cy, xp)], sh[L::at(4, cy, xp)], p.h1);
+            a2 = -(sh[L::at(0, yp, cx)] - p1) / p.h2 + r1c * g;
+            a1 = -(p1 - sh[L::at(0, ym, cx)]) / p.h2 + r1c * g;
+            const double f2 = upwind(a2, a1, X1c, r1c, sh[L::at(2, ym, cx)], sh[L::at(4, ym, cx)],
+                                     sh[L::at(2, yp, cx)], sh[L::at(4, yp, cx)], p.h2);
+
+            a2 = -(sh[L::at(1, cy, xp)] - p2) / p.h1;
+            a1 = -(p2 - sh[L::at(1, cy, xm)]) / p.h1;
+            const double g1 = upwind(a2, a1, X2c, r2c, sh[L::at(3, cy, xm)], sh[L::at(5, cy, xm)],
+                                     sh[L::at(3, cy, xp)], sh[L::at(5, cy, xp)], p.h1);
+            a2 = -(sh[L::at(1, yp, cx)] - p2) / p.h2 + r2c * g;
+            a1 = -(p2 - sh[L::at(1, ym, cx)]) / p.h2 + r2c * g;
+            const double g2 = upwind(a2, a1, X2c, r2c, sh[L::at(3, ym, cx)], sh[L::at(5, ym, cx)],
+                                     sh[L::at(3, yp, cx)], sh[L::at(5, yp, cx)], p.h2);
+
+            const int md = (f.media && p.n_media > 1) ? ldi(f.media, c >> 1) : 0;
+            const double phi = f.poro ? ld(f.poro, c) : p.m[md];
+            st(f.roS1_old, c, roS1);
+            st(f.roS2_old, c, roS2);
+            st(f.roS1, c, roS1 - (p.dt / phi) * (f1 + f2 + f3));
+            st(f.roS2, c, roS2 - (p.dt / phi) * (g1 + g2 + g3));
+        }
+        __syncthreads();
+#pragma unroll
+        for (int a = 0; a < 6; a++) {
+            zm[a] = zc[a];
+            zc[a] = zp[a];
+        }
+    }
+}
+
+// ------------------------------------------------------------------ u ----
+// cpu.cpp:77-160 for every active cell; stencil operands P1, P2 only.
+template <int TX, int TY>
+__global__ __launch_bounds__(TX* TY) void k_u_zm(filt_fields f, filt_params p, int nx, int rank,
+                                                 int size, TileMap map, int kz)
+{
+    using L = Lds<TX, TY>;
+    __shared__ double sh[2 * L::SN];
+    int bx, by, bz;
+    map.decode(blockIdx.x, bx, by, bz);
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int i = bx * TX + tx, j = by * TY + ty;
+    const int ny = p.Ny, nz = p.Nz;
+    const bool valid = i < nx && j < ny;
+    const bool active = valid && is_active(i, nx, rank, size);
+    const unsigned sy = 8u * nx, sz = 8u * nx * ny;
+    const int kb = bz * kz, ke = min(bz * kz + kz, nz);
+    if (kb >= ke)
+        return;
+    const unsigned col = 8u * i + sy * j;
+    double zm0 = 0, zm1 = 0, zc0 = 0, zc1 = 0, zp0 = 0, zp1 = 0;
+    if (valid) {
+        if (kb >= 1) {
+            zm0 = ld(f.P1, col + (kb - 1) * sz);
+            zm1 = ld(f.P2, col + (kb - 1) * sz);
+        }
+        zc0 = ld(f.P1, col + kb * sz);
+        zc1 = ld(f.P2, col + kb * sz);
+    }
+    const int cx = tx + 1, cy = ty + 1;
+    const double g = p.g_const;
+    for (int k = kb; k < ke; k++) {
+        const unsigned c = col + k * sz;
+        if (valid) {
+            if (k + 1 < nz) {
+                zp0 = ld(f.P1, c + sz);
+                zp1 = ld(f.P2, c + sz);
+            }
+            sh[L::at(0, cy, cx)] = zc0;
+            sh[L::at(1, cy, cx)] = zc1;
+            if (ty == 0 && j >= 1) {
+                sh[L::at(0, 0, cx)] = ld(f.P1, c - sy);
+                sh[L::at(1, 0, cx)] = ld(f.P2, c - sy);
+            }
+            if (ty == TY - 1 && j + 1 < ny) {
+                sh[L::at(0, TY + 1, cx)] = ld(f.P1, c + sy);
+                sh[L::at(1, TY + 1, cx)] = ld(f.P2, c + sy);
+            }
+            if (tx == 0 && i >= 1) {
+                sh[L::at(0, cy, 0)] = ld(f.P1, c - 8u);
+                sh[L::at(1, cy, 0)] = ld(f.P2, c - 8u);
+            }
+            if (tx == TX - 1 && i + 1 < nx) {
+                sh[L::at(0, cy, TX + 1)] = ld(f.P1, c + 8u);
+                sh[L::at(1, cy, TX + 1)] = ld(f.P2, c + 8u);
+            }
+        }
+        __syncthreads();
+        if (active) {
+            const double X1 = ld(f.Xi1, c), X2 = ld(f.Xi2, c);
+            const double P1c = zc0, P2c = zc1;
+            double u1, u2;
+            // x (p.NX > 2 holds: this kernel runs for NX >= 3 only)
+            if (i == 0) {
+                u1 = X1 * (sh[L::at(0, cy, cx + 1)] - P1c) / p.h1;
+                u2 = X2 * (sh[L::at(1, cy, cx + 1)] - P2c) / p.h1;
+            } else if (i == nx - 1) {
+                u1 = X1 * (P1c - sh[L::at(0, cy, cx - 1)]) / p.h1;
+                u2 = X2 * (P2c - sh[L::at(1, cy, cx - 1)]) / p.h1;
+            } else {
+                u1 = X1 * ((sh[L::at(0, cy, cx + 1)] - sh[L::at(0, cy, cx - 1)]) / (2. * p.h1));
+                u2 = X2 * ((sh[L::at(1, cy, cx + 1)] - sh[L::at(1, cy, cx - 1)]) / (2. * p.h1));
+            }
+            st(f.u1x, c, u1);
+            st(f.u2x, c, u2);
+            // y
+            const double r1 = ld(f.ro1, c), r2 = ld(f.ro2, c);
+            if (j == 0) {
+                u1 = X1 * ((sh[L::at(0, cy + 1, cx)] - P1c) / p.h2 - r1 * g);
+                u2 = X2 * ((sh[L::at(1, cy + 1, cx)] - P2c) / p.h2 - r2 * g);
+            } else if (j == ny - 1) {
+                u1 = X1 * ((P1c - sh[L::at(0, cy - 1, cx)]) / p.h2 - r1 * g);
+                u2 = X2 * ((P2c - sh[L::at(1, cy - 1, cx)]) / p.h2 - r2 * g);
+            } else {
+                u1 = X1 * ((sh[L::at(0, cy + 1, cx)] - sh[L::at(0, cy - 1, cx)]) / (2. * p.h2) - r1 * g);
+                u2 = X2 * ((sh[L::at(1, cy + 1, cx)] - sh[L::at(1, cy - 1, cx)]) / (2. * p.h2) - r2 * g);
+            }
+            st(f.u1y, c, u1);
+            st(f.u2y, c, u2);
+            // z
+            if (k == 0) {
+                u1 = X1 * (zp0 - P1c) / p.h3;
+                u2 = X2 * (zp1 - P2c) / p.h3;
+            } else if (k == nz - 1) {
+                u1 = X1 * (P1c - zm0) / p.h3;
+                u2 = X2 * (P2c - zm1) / p.h3;
+            } else {
+                u1 = X1 * ((zp0 - zm0) / (2. * p.h3));
+                u2 = X2 * ((zp1 - zm1) / (2. * p.h3));
+            }
+            st(f.u1z, c, u1);
+            st(f.u2z, c, u2);
+        }
+        __syncthreads();
+        zm0 = zc0;
+        zm1 = zc1;
+        zc0 = zp0;
+        zc1 = zp1;
+    }
+}
+
+// Kernel variants by tile shape.
+using RoSFn = void (*)(filt_fields, filt_params, int, TileMap, int);
+using UFn = void (*)(filt_fields, filt_params, int, int, int, TileMap, int);
+
+RoSFn pick_roS(int tx, int ty)
+{
+    if (tx == 64 && ty == 4) return k_roS_zm<64, 4>;
+    if (tx == 32 && ty == 8) return k_roS_zm<32, 8>;
+    if (tx == 16 && ty == 16) return k_roS_zm<16, 16>;
+    if (tx == 64 && ty == 8) return k_roS_zm<64, 8>;
+    return nullptr;
+}
+
+UFn pick_u(int tx, int ty)
+{
+    if (tx == 64 && ty == 4) return k_u_zm<64, 4>;
+    if (tx == 32 && ty == 8) return k_u_zm<32, 8>;
+    if (tx == 16 && ty == 16) return k_u_zm<16, 16>;
+    if (tx == 64 && ty == 8) return k_u_zm<64, 8>;
+    return nullptr;
+}
+
+TileMap make_map(int nx, int ny, int nz, const TileConfig& t)
+{
+    TileMap m;
+    m.nbx = (nx + t.tx - 1) / t.tx;
+    m.nby = (ny + t.ty - 1) / t.ty;
+    m.nbz = (nz + t.kz - 1) / t.kz;
+    m.order = t.order;
+    return m;
+}
+
+TileConfig g_roS_cfg = {64, 4, 32, 0};
+TileConfig g_u_cfg = {64, 4, 32, 0};
+
+// Autotuned configurations by grid shape (filt_autotune_stencil).
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int>, std::pair<TileConfig, TileConfig>> g_tuned;
+
+bool valid_cfg(const TileConfig& t, bool for_roS)
+{
+    return t.kz >= 1 && t.kz <= 4096 && (t.order == 0 || t.order == 1) &&
+           (for_roS ? pick_roS(t.tx, t.ty) != nullptr : pick_u(t.tx, t.ty) != nullptr);
+}
+
+TileConfig cfg_for(bool roS, int nx, int ny, int nz)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find(std::make_tuple(nx, ny, nz));
+    if (it != g_tuned.end())
+        return roS ? it->second.first : it->second.second;
+    return roS ? g_roS_cfg : g_u_cfg;
+}
+
+}  // namespace
+
+// The tiled kernels address each array with 32-bit byte offsets.
+bool tiled_fits(int nx, int ny, int nz)
+{
+    return (double)nx * ny * nz * 8.0 < 4294967296.0 - 65536.0;
+}
+
+// Launchers used by filt_stages.hip (grids with NX, Ny, Nz >= 3).
+int launch_roS_zm(const filt_fields& f, const filt_params& p, int nx)
+{
+    const TileConfig t = cfg_for(true, nx, p.Ny, p.Nz);
+    const TileMap m = make_map(nx, p.Ny, p.Nz, t);
+    hipLaunchKernelGGL(pick_roS(t.tx, t.ty), dim3((unsigned)m.count()), dim3(t.tx, t.ty), 0,
+                       stream(), f, p, nx, m, t.kz);
+    FILT_LAUNCH_CHECK("k_roS_zm");
+    return FILT_OK;
+}
+
+int launch_u_zm(const filt_fields& f, const filt_params& p, int nx, int rank, int size)
+{
+    const TileConfig t = cfg_for(false, nx, p.Ny, p.Nz);
+    const TileMap m = make_map(nx, p.Ny, p.Nz, t);
+    hipLaunchKernelGGL(pick_u(t.tx, t.ty), dim3((unsigned)m.count()), dim3(t.tx, t.ty), 0,
+                       stream(), f, p, nx, rank, size, m, t.kz);
+    FILT_LAUNCH_CHECK("k_u_zm");
+    return FILT_OK;
+}
+
+}  // namespace filt
+
+extern "C" int filt_set_tile_config(int kernel, int tx, int ty, int kz, int order)
+{
+    filt::TileConfig t = {tx, ty, kz, order};
+    if (kernel != 0 && kernel != 1)
+        return filt::set_error(FILT_ERR_INVALID, "kernel must be 0 (roS) or 1 (u)");
+    if (!filt::valid_cfg(t, kernel == 0))
+        return filt::set_error(FILT_ERR_INVALID, "unsupported tile %dx%d kz=%d order=%d", tx, ty,
+                               kz, order);
+    std::lock_guard<std::mutex> lk(filt::g_tune_mu);
+    (kernel == 0 ? filt::g_roS_cfg : filt::g_u_cfg) = t;
+    filt::g_tuned.clear();
+    return FILT_OK;
+}
+
+// Time every candidate tile configuration of the two stencil kernels on the
+// given grid (reps launches each, median of the per-launch times) and keep
+// the fastest per kernel for grids of this shape.  The kernels read the
+// stage-1 outputs and rewrite only their own outputs (roS*, u*), which do
+// not depend on earlier contents, so tuning on a live grid leaves the same
+// values a single call would.
+extern "C" int filt_autotune_stencil(const filt_fields* dev, const filt_params* p, int localNx,
+                                     int reps, double* best_ms)
+{
+    using namespace filt;
+    int rc = check_shape(p, localNx);
+    if (rc)
+        return rc;
+    if (!dev || !dev->P1)
+        return set_error(FILT_ERR_INVALID, "null device fields");
+    const int nx = localNx, ny = p->Ny, nz = p->Nz;
+    if (nx < 3 || ny < 3 || nz < 3 || !tiled_fits(nx, ny, nz))
+        return set_error(FILT_ERR_UNSUPPORTED, "grid does not use the tiled kernels");
+    if (reps < 1)
+        reps = 3;
+    static const int shapes[4][2] = {{64, 4}, {32, 8}, {16, 16}, {64, 8}};
+    std::vector<int> kzs = {8, 16, 32, 64};
+    if (nz > 64)
+        kzs.push_back(nz);
+    hipEvent_t e0, e1;
+    FILT_HIP_TRY(hipEventCreate(&e0));
+    FILT_HIP_TRY(hipEventCreate(&e1));
+    TileConfig best[2];
+    double best_t[2] = {1e30, 1e30};
+    for (int kern = 0; kern < 2; kern++)
+        for (auto& s : shapes)
+            for (int kz : kzs)
+                for (int order = 0; order < 2; order++) {
+                    const TileConfig t = {s[0], s[1], kz, order};
+                    const TileMap m = make_map(nx, ny, nz, t);
+                    std::vector<float> ms;
+                    for (int r = 0; r <= reps; r++) {
+                        FILT_HIP_TRY(hipEventRecord(e0, stream()));
+                        if (kern == 0)
+                            hipLaunchKernelGGL(pick_roS(t.tx, t.ty), dim3((unsigned)m.count()),
+                                               dim3(t.tx, t.ty), 0, stream(), *dev, *p, nx, m, t.kz);
+                        else
+                            hipLaunchKernelGGL(pick_u(t.tx, t.ty), dim3((unsigned)m.count()),
+                                               dim3(t.tx, t.ty), 0, stream(), *dev, *p, nx, 0, 1,
+                                               m, t.kz);
+                        FILT_LAUNCH_CHECK("autotune launch");
+                        FILT_HIP_TRY(hipEventRecord(e1, stream()));
+                        FILT_HIP_TRY(hipEventSynchronize(e1));
+                        float v = 0;
+                        FILT_HIP_TRY(hipEventElapsedTime(&v, e0, e1));
+                        if (r > 0)  // the first launch warms caches and code
+                            ms.push_back(v);
+                    }
+                    std::sort(ms.begin(), ms.end());
+                    const double med = ms[ms.size() / 2];
+                    if (med < best_t[kern]) {
+                        best_t[kern] = med;
+                        best[kern] = t;
+                    }
+                }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tuned[std::make_tuple(nx, ny, nz)] = std::make_pair(best[0], best[1]);
+    }
+    if (best_ms) {
+        best_ms[0] = best_t[0];
+        best_ms[1] = best_t[1];
+    }
+    return FILT_OK;
+}
+
+extern "C" int filt_get_tile_config(int kernel, int nx, int ny, int nz, int* tx, int* ty, int* kz,
+                                    int* order)
+{
+    if (kernel != 0 && kernel != 1)
+        return filt::set_error(FILT_ERR_INVALID, "kernel must be 0 (roS) or 1 (u)");
+    const filt::TileConfig t = filt::cfg_for(kernel == 0, nx, ny, nz);
+    if (tx) *tx = t.tx;
+    if (ty) *ty = t.ty;
+    if (kz) *kz = t.kz;
+    if (order) *order = t.order;
+    return FILT_OK;
+}

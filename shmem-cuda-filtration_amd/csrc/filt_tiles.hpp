@@ -1,0 +1,47 @@
+// filt_tiles.hpp -- launch geometry of the z-marching stencil kernels.
+//
+// A block owns an x-y tile of TX x TY columns and marches along z over a
+// chunk of kz planes, keeping its own column's k-1, k, k+1 values in
+// registers and staging the plane's x/y neighbours through LDS.
+//
+// MI355X dispatches workgroup b to XCD b % 8 and each XCD has its own L2.
+// TileMap renumbers blocks so that the tiles an XCD runs at the same time
+// are neighbours: with order 0 the logical tile index L = xcd*C + b/8
+// (C = tiles per XCD) runs x-tiles fastest, then y-tiles, then z-chunks, so
+// an XCD works through whole z-chunks and the x/y halos its tiles read come
+// from tiles on the same XCD.  order 1 keeps the hardware numbering.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace filt {
+
+struct TileMap {
+    int nbx, nby, nbz;  // tiles along x, y and z-chunks along z
+    int order;          // 0: XCD-grouped, 1: identity
+
+    __host__ __device__ long long count() const { return (long long)nbx * nby * nbz; }
+
+    __device__ inline void decode(long long b, int& bx, int& by, int& bz) const
+    {
+        const long long T = count();
+        long long L = b;
+        if (order == 0 && T % 8 == 0) {
+            const long long C = T / 8;
+            L = (b % 8) * C + b / 8;
+        }
+        bx = (int)(L % nbx);
+        const long long r = L / nbx;
+        by = (int)(r % nby);
+        bz = (int)(r / nby);
+    }
+};
+
+// Tile configuration of one stencil kernel.
+struct TileConfig {
+    int tx, ty;  // threads per block in x and y (tx * ty = 256)
+    int kz;      // planes per z-chunk
+    int order;   // TileMap order
+};
+
+}  // namespace filt

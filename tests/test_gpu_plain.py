@@ -142,3 +142,64 @@ def test_trajectory_degenerate_grids(hip, orc, dims):
     out, want = trajectory(hip, orc, p, 30)
     for n in ("P1", "S2", "u1x", "u2y", "u2z"):
         assert max_rel(out[n], want[n]) <= 1e-10, (n, dims)
+
+
+TILES = [(64, 4), (32, 8), (16, 16), (64, 8)]
+
+
+@pytest.mark.parametrize("tile", TILES)
+@pytest.mark.parametrize("kz,order", [(1, 0), (5, 1), (16, 0), (64, 1)])
+def test_tiled_stencils_every_config_bitwise(hip, orc, tile, kz, order):
+    """Every tile shape, z-chunk and block order of the z-marching u / roS
+    kernels gives the oracle's bits (partial tiles in x and y, chunk ends
+    not aligned with the grid)."""
+    import ctypes
+    lib = hip.lib
+    p = bl_params(37, 19, 23)
+    g = random_state(p, np.random.default_rng(kz * 100 + tile[0] + order))
+    try:
+        for kern in (0, 1):
+            assert lib.filt_set_tile_config(kern, tile[0], tile[1], kz, order) == 0
+        for stage in (2, 3):
+            out, want = run_stage_pair(hip, orc, p, g, stage)
+            for n in STAGE_OUT[stage]:
+                assert np.array_equal(out[n], want[n]), (n, tile, kz, order)
+    finally:
+        for kern in (0, 1):
+            lib.filt_set_tile_config(kern, 64, 4, 32, 0)
+    assert ctypes is not None
+
+
+def test_autotuner_picks_a_config_and_keeps_results(hip, orc):
+    import ctypes
+    lib = hip.lib
+    p = bl_params(48, 40, 36)
+    g = random_state(p, np.random.default_rng(77), media=False)
+    d = hip.grid(p)
+    try:
+        d.upload(g)
+        best = (ctypes.c_double * 2)()
+        assert lib.filt_autotune_stencil(ctypes.byref(d.f), ctypes.byref(d.params), d.nx, 2,
+                                         best) == 0
+        assert best[0] > 0 and best[1] > 0
+        v = [ctypes.c_int() for _ in range(4)]
+        assert lib.filt_get_tile_config(0, 48, 40, 36, *[ctypes.byref(x) for x in v]) == 0
+        assert (v[0].value, v[1].value) in TILES
+        d.sync()
+        out = g.copy()
+        d.download(out)
+        # tuning ran u and roS many times; their outputs equal one oracle call
+        want = g.copy()
+        orc.stage(2, want, p)
+        orc.stage(3, want, p)
+        for n in STAGE_OUT[2] + STAGE_OUT[3]:
+            assert np.array_equal(out[n], want[n]), n
+        # the tuned config is used by later calls on this shape
+        d.stage(3)
+        d.sync()
+        out2 = g.copy()
+        d.download(out2)
+        assert np.array_equal(out2["roS1"], want["roS1"])
+    finally:
+        d.free()
+        lib.filt_set_tile_config(0, 64, 4, 32, 0)
