@@ -265,6 +265,69 @@ int filt_get_tile_config(int kernel, int nx, int ny, int nz, int* tx, int* ty, i
 int filt_autotune_stencil(const filt_fields* dev, const filt_params* p, int localNx, int reps,
                           double* best_ms);
 
+/* -------------------------------------- fused / batched / mixed forms -- */
+/*
+ * No reference counterpart: these advance P1 and S2 by the same time step as
+ * time_step_function (main.cpp:83-94) without materialising P2, ro, Xi, u,
+ * roS, in one kernel launch per step for any number of grids.
+ *
+ * An ensemble holds member grids of any shapes (every extent >= 3, one rank),
+ * each with its own parameters, optional per-cell permeability / porosity /
+ * media and its state, all resident in HBM.  Results of a member do not
+ * depend on the other members.  FILT_PREC_FP64 computes what the stage path
+ * computes (same expressions); FILT_PREC_MIXED evaluates the stencil fluxes
+ * and the ro*S increment in FP32 (FP32 property fields), keeping pressure
+ * differences, the ro*S base, the Newton solve and mass sums in FP64.
+ * Use an ensemble from one host thread at a time.
+ */
+#define FILT_PREC_FP64 0
+#define FILT_PREC_MIXED 1
+
+typedef struct filt_ensemble filt_ensemble;
+
+int filt_ensemble_create(filt_ensemble** out, int precision);
+int filt_ensemble_destroy(filt_ensemble* e);
+
+/* Add a member: copies the host arrays (NX*Ny*Nz cells each, x fastest) to
+ * the device.  perm, poro, media may be null (media table entry 0 / the
+ * media array).  *id receives the member's handle. */
+int filt_ensemble_add(filt_ensemble* e, const filt_params* p, const double* P1, const double* S2,
+                      const double* perm, const double* poro, const int* media, int* id);
+int filt_ensemble_remove(filt_ensemble* e, int id);
+int filt_ensemble_count(const filt_ensemble* e);
+long long filt_ensemble_cells(const filt_ensemble* e);
+/* Steps a member has advanced since it was added (-1: unknown id). */
+long long filt_ensemble_steps(const filt_ensemble* e, int id);
+/* Device bytes one member with these parameters occupies. */
+size_t filt_ensemble_member_bytes(const filt_params* p, int precision, int with_perm,
+                                  int with_poro, int with_media);
+/* Device bytes held by all members. */
+size_t filt_ensemble_device_bytes(const filt_ensemble* e);
+
+/* Advance every member nsteps steps (one launch per step, enqueued). */
+int filt_ensemble_step(filt_ensemble* e, int nsteps);
+/* Record nsteps steps as a hipGraph; filt_ensemble_replay launches it.
+ * Adding / removing members or changing the tiling drops the graph. */
+int filt_ensemble_capture(filt_ensemble* e, int nsteps);
+int filt_ensemble_replay(filt_ensemble* e);
+
+/* Current state of a member to host arrays (either may be null). */
+int filt_ensemble_get(filt_ensemble* e, int id, double* P1, double* S2);
+/* Total water / NAPL mass over a member's interior cells,
+ * sum phi*ro(P)*S*h1*h2*h3, FP64, fixed summation order. */
+int filt_ensemble_mass(filt_ensemble* e, int id, double* out2);
+
+/* Tiling of the fused kernel: kz planes per z-chunk (tiles are 16x16
+ * columns), order 0 = XCD-grouped, 1 = hardware block order. */
+int filt_ensemble_set_tiling(filt_ensemble* e, int kz, int order);
+/* Autotuner: times each (kz, order) candidate over reps steps of the current
+ * membership (the members advance by the steps timed) and keeps the fastest. */
+int filt_ensemble_autotune(filt_ensemble* e, int reps, int* best_kz, int* best_order,
+                           double* best_ms_per_step);
+
+/* Free / total device memory (admission control). */
+int filt_device_free_bytes(size_t* free_bytes, size_t* total_bytes);
+
 /* ------------------------------------------------------------- runtime -- */
 
 /* Wait for all work the library enqueued. */

@@ -146,6 +146,25 @@ def load_library(path=HIP_LIB):
     _sig(lib, "filt_set_tile_config", c_int, c_int, c_int, c_int, c_int, c_int)
     _sig(lib, "filt_get_tile_config", c_int, c_int, c_int, c_int, c_int, pi, pi, pi, pi)
     _sig(lib, "filt_autotune_stencil", c_int, F, P, c_int, c_int, POINTER(c_double))
+    E = c_void_p
+    _sig(lib, "filt_ensemble_create", c_int, POINTER(c_void_p), c_int)
+    _sig(lib, "filt_ensemble_destroy", c_int, E)
+    _sig(lib, "filt_ensemble_add", c_int, E, P, c_void_p, c_void_p, c_void_p, c_void_p,
+         c_void_p, pi)
+    _sig(lib, "filt_ensemble_remove", c_int, E, c_int)
+    _sig(lib, "filt_ensemble_count", c_int, E)
+    _sig(lib, "filt_ensemble_cells", ctypes.c_longlong, E)
+    _sig(lib, "filt_ensemble_steps", ctypes.c_longlong, E, c_int)
+    _sig(lib, "filt_ensemble_member_bytes", c_size_t, P, c_int, c_int, c_int, c_int)
+    _sig(lib, "filt_ensemble_device_bytes", c_size_t, E)
+    _sig(lib, "filt_ensemble_step", c_int, E, c_int)
+    _sig(lib, "filt_ensemble_capture", c_int, E, c_int)
+    _sig(lib, "filt_ensemble_replay", c_int, E)
+    _sig(lib, "filt_ensemble_get", c_int, E, c_int, c_void_p, c_void_p)
+    _sig(lib, "filt_ensemble_mass", c_int, E, c_int, POINTER(c_double))
+    _sig(lib, "filt_ensemble_set_tiling", c_int, E, c_int, c_int)
+    _sig(lib, "filt_ensemble_autotune", c_int, E, c_int, pi, pi, POINTER(c_double))
+    _sig(lib, "filt_device_free_bytes", c_int, POINTER(c_size_t), POINTER(c_size_t))
     _sig(lib, "filt_synchronize", c_int)
     _sig(lib, "filt_set_stream", c_int, c_void_p)
     _sig(lib, "filt_get_stream", c_void_p)

@@ -1,0 +1,522 @@
+// filt_fused.hip -- one launch per time step for a batch of grids.
+//
+// The fused step computes what the five defines.h stages compute for P1 and
+// S2 (cpu.cpp:3-54 in the order of main.cpp:83-94) without materialising the
+// intermediate arrays: state lives in ping-pong buffers (P, S in / out), the
+// static fields (permeability, porosity, media) are read once per cell, and
+// stage 1 (density, mobility) is evaluated on the fly for the 7-point
+// stencil.  Per cell and step it moves 16 B of state in, 16 B out and the
+// statics (16 B FP64, 8 B FP32): 48 / 40 B against the 304 B of the stage
+// path.  u_calculation is not on the P/S path; it runs on demand.
+//
+// Boundary conditions (cpu.cpp:44-54, 323-403) need no second launch: every
+// shell cell's post-sweep value is either a copy of a value from the
+// previous state, the source patch or P_atm (the "shell pass" blocks), or a
+// copy of the new value of one interior cell, which the thread computing
+// that cell writes as well:
+//   interior (i,j,k) with value n also writes
+//     (0,j,k)              if i == 1          [i = 0 face, interior source]
+//     (N-1,j,k)            if i == N-2        [i = N-1 face]
+//   and for column c in {i} and, if i == N-2, also {N-1}:
+//     (c,M-1,k) = (n_S, n_P + ro1(i,1,0)*g*h2)   if j == M-2, plus k = 0 / L-1
+//                                                 when k == 1 / L-2
+//     (c,j,0) = n if k == 1,  (c,j,L-1) = n if k == L-2
+//   shell pass: (0,j,k) = old (1,j,k) where (1,j,k) is not interior;
+//     (i,0,k), i >= 1: P_atm and S = src(i',min(k,L-2)) for k >= 1,
+//     old S(i',0,1) for k = 0, with i' = min(i, N-2)
+// which is the closed form of the reference's i -> j -> k sweep.
+//
+// Launch: interior tiles of TX x TY columns march over z-chunks (stage-1
+// values of the tile plane and its one-cell ring in LDS, the own column's
+// k-1 and k+1 in registers), shell-pass blocks follow each grid's tiles.  A
+// table maps each block to its grid, so grids of different shapes share one
+// launch; blocks are regrouped per XCD as in filt_tiles.hpp.
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "../../include/filtration_hip.h"
+#include "filt_device.hpp"
+#include "filt_fused.hpp"
+#include "filt_runtime.hpp"
+
+namespace filt {
+
+namespace {
+
+constexpr int FTX = 16;
+constexpr int FTY = 16;
+constexpr int FT = FTX * FTY;
+constexpr int RING = 2 * FTX + 2 * FTY;
+constexpr int SX = FTX + 2;
+constexpr int SN = SX * (FTY + 2);
+
+__device__ __forceinline__ int slot(int x, int y) { return (y + 1) * SX + (x + 1); }
+
+// Stage-1 values of one cell as the flux needs them (P2 = P1 + 0 = P1).
+struct C1 {
+    double P, X1, X2, r1, r2;
+};
+
+__device__ __forceinline__ int media_at(const FusedGrid& G, long long c)
+{
+    return (G.media && G.p.n_media > 1) ? G.media[c] : 0;
+}
+
+// ------------------------------------------------------------ FP64 form --
+
+struct Fp64 {
+    __device__ static inline C1 cell(const FusedGrid& G, const double* Pin, const double* Sin,
+                                     long long c)
+    {
+        const int md = media_at(G, c);
+        const double K = G.perm ? G.perm[c] : G.p.K[md];
+        const double P = Pin[c];
+        const Stage1 r = stage1_cell(P, Sin[c], K, G.p.lambda[md], G.p.S_wr[md], G.p);
+        return C1{P, r.Xi1, r.Xi2, r.ro1, r.ro2};
+    }
+
+    // roS_calculation (cpu.cpp:218-291) then Newton (cpu.cpp:293-320) for an
+    // interior cell; same expressions as filt_stages.hip.
+    __device__ static inline void update(const FusedGrid& G, long long c, const C1& cc,
+                                         const C1& xm, const C1& xp, const C1& ym, const C1& yp,
+                                         const C1& zm, const C1& zp, double S, double& nP,
+                                         double& nS)
+    {
+        const filt_params& p = G.p;
+        const double roS1 = cc.r1 * (1 - S);
+        const double roS2 = cc.r2 * S;
+        const double p1 = cc.P, g = p.g_const;
+        double a1, a2;
+        a2 = -(zp.P - p1) / p.h3;
+        a1 = -(p1 - zm.P) / p.h3;
+        const double f3 = upwind(a2, a1, cc.X1, cc.r1, zm.X1, zm.r1, zp.X1, zp.r1, p.h3);
+        const double g3 = upwind(a2, a1, cc.X2, cc.r2, zm.X2, zm.r2, zp.X2, zp.r2, p.h3);
+        a2 = -(xp.P - p1) / p.h1;
+        a1 = -(p1 - xm.P) / p.h1;
+        const double f1 = upwind(a2, a1, cc.X1, cc.r1, xm.X1, xm.r1, xp.X1, xp.r1, p.h1);
+        const double g1 = upwind(a2, a1, cc.X2, cc.r2, xm.X2, xm.r2, xp.X2, xp.r2, p.h1);
+        const double dy2 = -(yp.P - p1) / p.h2, dy1 = -(p1 - ym.P) / p.h2;
+        const double f2 = upwind(dy2 + cc.r1 * g, dy1 + cc.r1 * g, cc.X1, cc.r1, ym.X1, ym.r1,
+                                 yp.X1, yp.r1, p.h2);
+        const double g2 = upwind(dy2 + cc.r2 * g, dy1 + cc.r2 * g, cc.X2, cc.r2, ym.X2, ym.r2,
+                                 yp.X2, yp.r2, p.h2);
+        const int md = media_at(G, c);
+        const double phi = G.poro ? G.poro[c] : p.m[md];
+        const double A1 = roS1 - (p.dt / phi) * (f1 + f2 + f3);
+        const double A2 = roS2 - (p.dt / phi) * (g1 + g2 + g3);
+        nP = cc.P;
+        nS = S;
+        newton_cell(nP, nS, A1, A2, p.lambda[md], p.S_wr[md], p.P_d[md], p);
+    }
+};
+
+// ----------------------------------------------------------- mixed form --
+// Stencil fluxes and the ro*S increment in FP32; pressure differences (the
+// y face force -dP/h + ro*g cancels toward hydrostatic balance), the
+// ro(P)*S base and the Newton in FP64.  Divisions by the uniform spacings
+// and 1/(1-S_wr) become multiplications by reciprocals; 1/det is an FP64
+// reciprocal refined by one Newton-Raphson step.  Statics are FP32.
+
+struct Mixed {
+    __device__ static inline C1 cell(const FusedGrid& G, const double* Pin, const double* Sin,
+                                     long long c)
+    {
+        const int md = media_at(G, c);
+        const float K = G.perm32 ? G.perm32[c] : (float)G.p.K[md];
+        const double P = Pin[c];
+        const float S = (float)Sin[c];
+        const float swr = (float)G.p.S_wr[md];
+        const float se = (1.f - S - swr) * (float)G.inv_1mswr[md];
+        float k1, k2;
+        if (G.p.lambda[md] == 2.0) {  // Brooks-Corey exponents 4 and 2
+            const float s2 = se * se;
+            k1 = s2 * s2;
+            k2 = (1.f - se) * (1.f - se) * (1.f - s2);
+        } else {
+            const float lam = (float)G.p.lambda[md];
+            k1 = powf(se, (2.f + 3.f * lam) / lam);
+            k2 = (1.f - se) * (1.f - se) * (1.f - powf(se, (2.f + lam) / lam));
+        }
+        C1 r;
+        r.P = P;
+        r.X1 = (double)(-K * k1 * (float)G.inv_mu1);
+        r.X2 = (double)(-K * k2 * (float)G.inv_mu2);
+        r.r1 = G.p.ro01 * (1. + G.p.beta1 * (P - G.p.P_atm));
+        r.r2 = G.p.ro02 * (1. + G.p.beta2 * (P - G.p.P_atm));
+        return r;
+    }
+
+    // flux with L = -Xi*ro >= 0 at centre, - and + neighbours
+    __device__ static inline float flux(float a2, float a1, float Lc, float Lm, float Lp)
+    {
+        return (fmaxf(a2, 0.f) - fminf(a1, 0.f)) * Lc - fmaxf(a1, 0.f) * Lm + fminf(a2, 0.f) * Lp;
+    }
+
+    __device__ static inline void update(const FusedGrid& G, long long c, const C1& cc,
+                                         const C1& xm, const C1& xp, const C1& ym, const C1& yp,
+                                         const C1& zm, const C1& zp, double S, double& nP,
+                                         double& nS)
+    {
+        const filt_params& p = G.p;
+        const double p1 = cc.P;
+        // face forces in FP64, then FP32
+        const float ax2 = (float)(-(xp.P - p1) * G.inv_h[0]);
+        const float ax1 = (float)(-(p1 - xm.P) * G.inv_h[0]);
+        const double dy2 = -(yp.P - p1) * G.inv_h[1], dy1 = -(p1 - ym.P) * G.inv_h[1];
+        const float az2 = (float)(-(zp.P - p1) * G.inv_h[2]);
+        const float az1 = (float)(-(p1 - zm.P) * G.inv_h[2]);
+        const double g = p.g_const;
+        const float ay2w = (float)(dy2 + cc.r1 * g), ay1w = (float)(dy1 + cc.r1 * g);
+        const float ay2n = (float)(dy2 + cc.r2 * g), ay1n = (float)(dy1 + cc.r2 * g);
+        auto L1 = [](const C1& v) { return -(float)v.X1 * (float)v.r1; };
+        auto L2 = [](const C1& v) { return -(float)v.X2 * (float)v.r2; };
+        const float ihx = (float)G.inv_h[0], ihy = (float)G.inv_h[1], ihz = (float)G.inv_h[2];
+        const float F1 = flux(ax2, ax1, L1(cc), L1(xm), L1(xp)) * ihx +
+                         flux(ay2w, ay1w, L1(cc), L1(ym), L1(yp)) * ihy +
+                         flux(az2, az1, L1(cc), L1(zm), L1(zp)) * ihz;
+        const float F2 = flux(ax2, ax1, L2(cc), L2(xm), L2(xp)) * ihx +
+                         flux(ay2n, ay1n, L2(cc), L2(ym), L2(yp)) * ihy +
+                         flux(az2, az1, L2(cc), L2(zm), L2(zp)) * ihz;
+        const int md = media_at(G, c);
+        const float dtphi = (float)p.dt / (G.poro32 ? G.poro32[c] : (float)p.m[md]);
+        const double A1 = cc.r1 * (1 - S) - (double)(dtphi * F1);
+        const double A2 = cc.r2 * S - (double)(dtphi * F2);
+        // Newton (cpu.cpp:293-320) in FP64 with reciprocals
+        const double ro01 = p.ro01, ro02 = p.ro02, b1 = p.beta1, b2 = p.beta2, Pa = p.P_atm;
+        const double swr = p.S_wr[md], pd = p.P_d[md], lam = p.lambda[md];
+        const double inv1 = G.inv_1mswr[md];
+        const bool no_cap = (pd == 0.0) && (lam >= 0.1);
+        double P = cc.P, Sv = S;
+        for (int w = 1; w <= p.newton_iterations; w++) {
+            const double se = (1 - Sv - swr) * inv1;
+            double P_k = 0.0, PkS = 0.0;
+            if (!(no_cap && se >= 0.0009765625)) {
+                P_k = pd * pow(se, (-1.) / lam);
+                PkS = pow(se, ((-1.) / lam) - 1.) * pd / (lam * (1 - swr));
+            }
+            const double F1v = ro01 * (1. + b1 * (P - Pa)) * (1. - Sv) - A1;
+            const double F2v = ro02 * (1. + b2 * (P + P_k - Pa)) * Sv - A2;
+            const double F1P = ro01 * b1 * (1 - Sv);
+            const double F2P = ro02 * b2 * Sv;
+            const double F1S = -ro01 * (1 + b1 * (P - Pa));
+            const double F2S = ro02 * (1 + b2 * (P + P_k - Pa + Sv * PkS));
+            const double det = F1P * F2S - F1S * F2P;
+            double r = __builtin_amdgcn_rcp(det);
+            r = fma(fma(-det, r, 1.0), r, r);
+            const double Pn = P - r * (F2S * F1v - F1S * F2v);
+            const double Sn = Sv - r * (F1P * F2v - F2P * F1v);
+            P = Pn;
+            Sv = Sn;
+        }
+        nP = P;
+        nS = Sv;
+    }
+};
+
+// -------------------------------------------------------------- writing --
+
+// Output of interior cell (i,j,k) and the shell cells that copy it.
+__device__ inline void write_cell(const FusedGrid& G, double* Po, double* So, const double* Pin,
+                                  int i, int j, int k, double nP, double nS)
+{
+    const int nx = G.nx, ny = G.ny, nz = G.nz;
+    const long long sy = nx, sz = (long long)nx * ny;
+    const long long c = i + j * sy + k * sz;
+    Po[c] = nP;
+    So[c] = nS;
+    if (i == 1) {
+        Po[c - 1] = nP;
+        So[c - 1] = nS;
+    }
+    const bool lastcol = (i == nx - 2);
+    if (lastcol) {
+        Po[c + 1] = nP;
+        So[c + 1] = nS;
+    }
+    const bool top = (j == ny - 2), kl = (k == 1), kh = (k == nz - 2);
+    if (!(top || kl || kh))
+        return;
+    double Pt = 0;
+    if (top) {
+        const double P10 = Pin[i + sy];  // (i, 1, 0)
+        const double ro1 = G.p.ro01 * (1. + G.p.beta1 * (P10 - G.p.P_atm));
+        Pt = nP + ro1 * G.p.g_const * G.p.h2;
+    }
+    for (int q = 0; q < (lastcol ? 2 : 1); q++) {
+        const long long col = (i + q) + j * sy;  // column i, then N-1
+        if (top) {
+            Po[col + sy + k * sz] = Pt;
+            So[col + sy + k * sz] = nS;
+            if (kl) {
+                Po[col + sy] = Pt;
+                So[col + sy] = nS;
+            }
+            if (kh) {
+                Po[col + sy + (nz - 1) * sz] = Pt;
+                So[col + sy + (nz - 1) * sz] = nS;
+            }
+        }
+        if (kl) {
+            Po[col] = nP;
+            So[col] = nS;
+        }
+        if (kh) {
+            Po[col + (nz - 1) * sz] = nP;
+            So[col + (nz - 1) * sz] = nS;
+        }
+    }
+}
+
+// Shell cells that do not depend on this step's interior values.
+__device__ inline void shell_cell(const FusedGrid& G, const double* Pin, const double* Sin,
+                                  double* Po, double* So, long long q)
+{
+    const int nx = G.nx, ny = G.ny, nz = G.nz;
+    const long long sy = nx, sz = (long long)nx * ny;
+    const long long ringA = 2LL * ny + 2LL * (nz - 2);
+    if (q < ringA) {  // i = 0 face where (1, j, k) is a shell cell
+        int j, k;
+        if (q < ny) { j = (int)q; k = 0; }
+        else if (q < 2LL * ny) { j = (int)(q - ny); k = nz - 1; }
+        else if (q < 2LL * ny + (nz - 2)) { j = 0; k = 1 + (int)(q - 2LL * ny); }
+        else { j = ny - 1; k = 1 + (int)(q - 2LL * ny - (nz - 2)); }
+        const long long c = j * sy + k * sz;
+        Po[c] = Pin[c + 1];
+        So[c] = Sin[c + 1];
+        return;
+    }
+    q -= ringA;
+    if (q >= (long long)(nx - 1) * nz)
+        return;
+    const int i = 1 + (int)(q % (nx - 1));
+    const int k = (int)(q / (nx - 1));
+    const int ii = i < nx - 1 ? i : nx - 2;
+    const long long c = i + k * sz;
+    Po[c] = G.p.P_atm;
+    So[c] = (k == 0) ? Sin[ii + sz] : (in_patch(ii, k < nz - 1 ? k : nz - 2, G.p) ? G.p.S2_gr : 0.0);
+}
+
+// ------------------------------------------------------------- the kernel --
+
+template <class Pol>
+__device__ void interior_tile(const FusedGrid& G, long long lb, int parity, double* lds)
+{
+    const double* Pin = G.P[parity];
+    const double* Sin = G.S[parity];
+    double* Po = G.P[parity ^ 1];
+    double* So = G.S[parity ^ 1];
+    const int nx = G.nx, ny = G.ny, nz = G.nz;
+    const int bx = (int)(lb % G.nbx);
+    const long long r = lb / G.nbx;
+    const int by = (int)(r % G.nby);
+    const int bz = (int)(r / G.nby);
+    const int i0 = 1 + bx * FTX, j0 = 1 + by * FTY;
+    const int kb = 1 + bz * G.kz, ke = min(kb + G.kz, nz - 1);
+    const int tx = threadIdx.x, ty = threadIdx.y, t = ty * FTX + tx;
+    const int i = i0 + tx, j = j0 + ty;
+    const bool in_grid = i < nx && j < ny;
+    const bool inner = i <= nx - 2 && j <= ny - 2;
+    const long long sy = nx, sz = (long long)nx * ny;
+    double* sP = lds;
+    double* sX1 = lds + SN;
+    double* sX2 = lds + 2 * SN;
+    double* sR1 = lds + 3 * SN;
+    double* sR2 = lds + 4 * SN;
+    // ring cell of this thread (t < RING)
+    int rx = 0, ry = 0;
+    if (t < FTX) { rx = t; ry = -1; }
+    else if (t < 2 * FTX) { rx = t - FTX; ry = FTY; }
+    else if (t < 2 * FTX + FTY) { rx = -1; ry = t - 2 * FTX; }
+    else if (t < RING) { rx = FTX; ry = t - 2 * FTX - FTY; }
+    const bool ring = t < RING && i0 + rx < nx && j0 + ry < ny;
+    const long long rcol = (i0 + rx) + (j0 + ry) * sy;
+    const long long col = i + j * sy;
+
+    C1 zm{}, zc{}, zp{};
+    if (in_grid) {
+        zm = Pol::cell(G, Pin, Sin, col + (kb - 1) * sz);
+        zc = Pol::cell(G, Pin, Sin, col + kb * sz);
+    }
+    for (int k = kb; k < ke; k++) {
+        if (in_grid) {
+            zp = Pol::cell(G, Pin, Sin, col + (k + 1) * sz);
+            const int s = slot(tx, ty);
+            sP[s] = zc.P;
+            sX1[s] = zc.X1;
+            sX2[s] = zc.X2;
+            sR1[s] = zc.r1;
+            sR2[s] = zc.r2;
+        }
+        if (ring) {
+            const C1 v = Pol::cell(G, Pin, Sin, rcol + k * sz);
+            const int s = slot(rx, ry);
+            sP[s] = v.P;
+            sX1[s] = v.X1;
+            sX2[s] = v.X2;
+            sR1[s] = v.r1;
+            sR2[s] = v.r2;
+        }
+        __syncthreads();
+        if (in_grid && inner) {
+            auto at = [&](int x, int y) {
+                const int s = slot(x, y);
+                return C1{sP[s], sX1[s], sX2[s], sR1[s], sR2[s]};
+            };
+            const long long c = col + k * sz;
+            double nP, nS;
+            Pol::update(G, c, zc, at(tx - 1, ty), at(tx + 1, ty), at(tx, ty - 1), at(tx, ty + 1),
+                        zm, zp, Sin[c], nP, nS);
+            write_cell(G, Po, So, Pin, i, j, k, nP, nS);
+        }
+        __syncthreads();
+        zm = zc;
+        zc = zp;
+    }
+}
+
+template <class Pol>
+__global__ __launch_bounds__(FT) void k_fused(const FusedGrid* __restrict__ grids,
+                                              const int* __restrict__ block_grid,
+                                              long long total_blocks, int parity, int order)
+{
+    __shared__ double lds[5 * SN];
+    long long b = blockIdx.x;
+    if (order == 0 && total_blocks % 8 == 0)
+        b = (b % 8) * (total_blocks / 8) + b / 8;
+    const FusedGrid& G = grids[block_grid[b]];
+    const long long lb = b - G.first_block;
+    if (lb < G.n_interior_blocks) {
+        interior_tile<Pol>(G, lb, parity, lds);
+        return;
+    }
+    const long long q = (lb - G.n_interior_blocks) * FT + threadIdx.y * FTX + threadIdx.x;
+    shell_cell(G, G.P[parity], G.S[parity], G.P[parity ^ 1], G.S[parity ^ 1], q);
+}
+
+// ------------------------------------------------------------------ mass --
+// Total water / NAPL mass over interior cells, deterministic: each block sums
+// a fixed range of 4096 cells (fixed tree), a second pass sums the block
+// partials in order.  Independent of any tiling.
+constexpr int MASS_CHUNK = 4096;
+
+__global__ __launch_bounds__(256) void k_mass_partial(const FusedGrid* __restrict__ Gp,
+                                                      int parity, double* __restrict__ partial)
+{
+    const FusedGrid& G = *Gp;
+    __shared__ double s1[256], s2[256];
+    const long long n = (long long)G.nx * G.ny * G.nz;
+    const long long base = (long long)blockIdx.x * MASS_CHUNK;
+    const double V = G.p.h1 * G.p.h2 * G.p.h3;
+    double a1 = 0, a2 = 0;
+    for (int u = 0; u < MASS_CHUNK / 256; u++) {
+        const long long c = base + u * 256 + threadIdx.x;
+        if (c >= n)
+            break;
+        const int i = (int)(c % G.nx);
+        const long long r = c / G.nx;
+        const int j = (int)(r % G.ny), k = (int)(r / G.ny);
+        if (!is_interior(i, j, k, G.nx, G.ny, G.nz))
+            continue;
+        const int md = media_at(G, c);
+        const double phi = G.poro ? G.poro[c] : (G.poro32 ? (double)G.poro32[c] : G.p.m[md]);
+        const double P = G.P[parity][c], S = G.S[parity][c];
+        const double r1 = G.p.ro01 * (1. + G.p.beta1 * (P - G.p.P_atm));
+        const double r2 = G.p.ro02 * (1. + G.p.beta2 * (P - G.p.P_atm));
+        a1 += phi * r1 * (1. - S) * V;
+        a2 += phi * r2 * S * V;
+    }
+    s1[threadIdx.x] = a1;
+    s2[threadIdx.x] = a2;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            s1[threadIdx.x] += s1[threadIdx.x + w];
+            s2[threadIdx.x] += s2[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = s1[0];
+        partial[2 * blockIdx.x + 1] = s2[0];
+    }
+}
+
+__global__ void k_mass_final(const double* __restrict__ partial, int nblocks,
+                             double* __restrict__ out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0)
+        return;
+    double a = 0, b = 0;
+    for (int q = 0; q < nblocks; q++) {
+        a += partial[2 * q];
+        b += partial[2 * q + 1];
+    }
+    out[0] = a;
+    out[1] = b;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ host side --
+
+void fused_prepare(FusedGrid& G)
+{
+    const filt_params& p = G.p;
+    G.inv_h[0] = 1.0 / p.h1;
+    G.inv_h[1] = 1.0 / p.h2;
+    G.inv_h[2] = 1.0 / p.h3;
+    G.inv_mu1 = 1.0 / p.mu1;
+    G.inv_mu2 = 1.0 / p.mu2;
+    for (int q = 0; q < FILT_MAX_MEDIA; q++)
+        G.inv_1mswr[q] = 1.0 / (1.0 - p.S_wr[q]);
+    if (G.kz < 1)
+        G.kz = 32;
+    G.nbx = (G.nx - 2 + FTX - 1) / FTX;
+    G.nby = (G.ny - 2 + FTY - 1) / FTY;
+    G.nbz = (G.nz - 2 + G.kz - 1) / G.kz;
+    G.n_interior_blocks = (long long)G.nbx * G.nby * G.nbz;
+    const long long shell = 2LL * G.ny + 2LL * (G.nz - 2) + (long long)(G.nx - 1) * G.nz;
+    G.n_shell_blocks = (shell + FT - 1) / FT;
+}
+
+int fused_launch(const FusedGrid* dev_grids, const int* dev_block_grid, long long total_blocks,
+                 int parity, int precision, int order, hipStream_t s)
+{
+    if (total_blocks <= 0)
+        return FILT_OK;
+    if (total_blocks > 0x7fffffffLL)
+        return set_error(FILT_ERR_INVALID, "batch too large for one launch");
+    if (precision == FILT_PREC_MIXED)
+        hipLaunchKernelGGL(k_fused<Mixed>, dim3((unsigned)total_blocks), dim3(FTX, FTY), 0, s,
+                           dev_grids, dev_block_grid, total_blocks, parity, order);
+    else
+        hipLaunchKernelGGL(k_fused<Fp64>, dim3((unsigned)total_blocks), dim3(FTX, FTY), 0, s,
+                           dev_grids, dev_block_grid, total_blocks, parity, order);
+    FILT_LAUNCH_CHECK("k_fused");
+    return FILT_OK;
+}
+
+int fused_mass(const FusedGrid* dev_grid, const FusedGrid& G, int parity, double* dev_partial,
+               double* dev_out, hipStream_t s)
+{
+    const long long n = (long long)G.nx * G.ny * G.nz;
+    const long long nb = (n + MASS_CHUNK - 1) / MASS_CHUNK;
+    hipLaunchKernelGGL(k_mass_partial, dim3((unsigned)nb), dim3(256), 0, s, dev_grid, parity,
+                       dev_partial);
+    FILT_LAUNCH_CHECK("k_mass_partial");
+    hipLaunchKernelGGL(k_mass_final, dim3(1), dim3(64), 0, s, dev_partial, (int)nb, dev_out);
+    FILT_LAUNCH_CHECK("k_mass_final");
+    return FILT_OK;
+}
+
+long long fused_mass_blocks(const FusedGrid& G)
+{
+    const long long n = (long long)G.nx * G.ny * G.nz;
+    return (n + MASS_CHUNK - 1) / MASS_CHUNK;
+}
+
+}  // namespace filt

@@ -105,6 +105,98 @@ class DeviceGrid:
             pass
 
 
+PREC_FP64 = 0
+PREC_MIXED = 1
+
+
+class Ensemble:
+    """Fused one-launch-per-step batch of grids (filt_ensemble_* in the C ABI)."""
+
+    def __init__(self, backend, precision=PREC_FP64):
+        self.lib = backend.lib
+        self.h = ctypes.c_void_p()
+        check(self.lib, self.lib.filt_ensemble_create(ctypes.byref(self.h), precision),
+              "filt_ensemble_create")
+        self.shapes = {}
+
+    @staticmethod
+    def _p(a):
+        if a is None:
+            return None
+        assert a.flags.c_contiguous
+        return a.ctypes.data
+
+    def add(self, params, P1, S2, perm=None, poro=None, media=None):
+        mid = ctypes.c_int()
+        P1 = np.ascontiguousarray(P1, np.float64)
+        S2 = np.ascontiguousarray(S2, np.float64)
+        perm = None if perm is None else np.ascontiguousarray(perm, np.float64)
+        poro = None if poro is None else np.ascontiguousarray(poro, np.float64)
+        media = None if media is None else np.ascontiguousarray(media, np.int32)
+        check(self.lib, self.lib.filt_ensemble_add(self.h, ctypes.byref(params), self._p(P1),
+                                                   self._p(S2), self._p(perm), self._p(poro),
+                                                   self._p(media), ctypes.byref(mid)),
+              "filt_ensemble_add")
+        self.shapes[mid.value] = (params.Nz, params.Ny, params.NX)
+        return mid.value
+
+    def remove(self, mid):
+        check(self.lib, self.lib.filt_ensemble_remove(self.h, mid), "filt_ensemble_remove")
+        self.shapes.pop(mid, None)
+
+    def step(self, n=1):
+        check(self.lib, self.lib.filt_ensemble_step(self.h, n), "filt_ensemble_step")
+
+    def capture(self, n):
+        check(self.lib, self.lib.filt_ensemble_capture(self.h, n), "filt_ensemble_capture")
+
+    def replay(self):
+        check(self.lib, self.lib.filt_ensemble_replay(self.h), "filt_ensemble_replay")
+
+    def get(self, mid):
+        shape = self.shapes[mid]
+        P = np.empty(shape, np.float64)
+        S = np.empty(shape, np.float64)
+        check(self.lib, self.lib.filt_ensemble_get(self.h, mid, P.ctypes.data, S.ctypes.data),
+              "filt_ensemble_get")
+        return P, S
+
+    def mass(self, mid):
+        out = (ctypes.c_double * 2)()
+        check(self.lib, self.lib.filt_ensemble_mass(self.h, mid, out), "filt_ensemble_mass")
+        return out[0], out[1]
+
+    def steps(self, mid):
+        return self.lib.filt_ensemble_steps(self.h, mid)
+
+    def set_tiling(self, kz, order=0):
+        check(self.lib, self.lib.filt_ensemble_set_tiling(self.h, kz, order), "set_tiling")
+
+    def autotune(self, reps=3):
+        kz, order, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        check(self.lib, self.lib.filt_ensemble_autotune(self.h, reps, ctypes.byref(kz),
+                                                        ctypes.byref(order), ctypes.byref(ms)),
+              "filt_ensemble_autotune")
+        return kz.value, order.value, ms.value
+
+    def cells(self):
+        return self.lib.filt_ensemble_cells(self.h)
+
+    def device_bytes(self):
+        return self.lib.filt_ensemble_device_bytes(self.h)
+
+    def close(self):
+        if self.h:
+            self.lib.filt_ensemble_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Backend:
     """Entry to the HIP backend.  Raises when the library is missing or no
     GPU is present: there is no CPU fallback."""
@@ -121,6 +213,9 @@ class Backend:
 
     def grid(self, params, localNx=None, perm=False, poro=False):
         return DeviceGrid(self, params, localNx, perm, poro)
+
+    def ensemble(self, precision=PREC_FP64):
+        return Ensemble(self, precision)
 
     def sync(self):
         check(self.lib, self.lib.filt_synchronize(), "filt_synchronize")

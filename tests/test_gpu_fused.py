@@ -1,0 +1,219 @@
+"""The fused one-launch-per-step forms (filt_ensemble_*) on the GPU.
+
+FP64 fused: same bits as the plain stage path on the GPU (same expressions),
+relative L-inf <= 1e-10 on P1 / S2 and 1e-12 on mass against the oracle.
+Mixed (FP32 fluxes, FP64 Newton): errors against the oracle stated per field
+and held to S2 L-inf <= 1e-4, P1 relative L-inf <= 1e-6, mass <= 1e-9.
+Batching: a member's result is the same alone and in a mix, bit for bit."""
+import numpy as np
+import pytest
+
+from cases import HostGrid, c1_params, max_rel, random_state, ref_params
+
+pytestmark = pytest.mark.gpu
+
+
+def plain_run(hip, p, g, steps):
+    d = hip.grid(p, perm=g.a.get("perm") is not None, poro=g.a.get("poro") is not None)
+    try:
+        d.upload(g)
+        d.run(steps, with_u=0)
+        d.sync()
+        return d.get("P1"), d.get("S2")
+    finally:
+        d.free()
+
+
+def fused_run(hip, p, g, steps, precision=0, graph=False, kz=None):
+    e = hip.ensemble(precision)
+    try:
+        mid = e.add(p, g["P1"], g["S2"], g.a.get("perm"), g.a.get("poro"),
+                    g.a["media"] if p.n_media > 1 else None)
+        if kz:
+            e.set_tiling(*kz)
+        if graph:
+            e.capture(steps)
+            e.replay()
+        else:
+            e.step(steps)
+        P, S = e.get(mid)
+        return P, S, e.mass(mid)
+    finally:
+        e.close()
+
+
+def c2_like(n, seed=5, sigma=1.0):
+    import importlib
+    W = importlib.import_module("shmem-cuda-filtration_amd").workloads
+    p = W.c2_params(n)
+    perm, poro = W.gen_fields(p, seed, sigma=sigma, corr=4.0, clip=2.5)
+    return p, W.initial_grid(p, perm, poro)
+
+
+@pytest.mark.parametrize("dims", [(32, 32, 32), (37, 19, 23), (3, 3, 3), (18, 5, 40)])
+def test_fused_fp64_equals_plain_bitwise(hip, dims):
+    p = c1_params()
+    p.NX, p.Ny, p.Nz = dims
+    p.src_x0, p.src_x1 = max(0, dims[0] // 2 - 2), dims[0] // 2 + 1
+    p.src_z0, p.src_z1 = max(0, dims[2] // 2 - 2), dims[2] // 2 + 1
+    g = HostGrid(p)
+    import oracle
+    oracle.Oracle().initial_data(g, p)
+    Pp, Sp = plain_run(hip, p, g, 25)
+    Pf, Sf, _ = fused_run(hip, p, g, 25)
+    assert np.array_equal(Pf, Pp)
+    assert np.array_equal(Sf, Sp)
+
+
+def test_fused_fp64_equals_plain_per_cell_fields(hip):
+    p, g = c2_like(40)
+    Pp, Sp = plain_run(hip, p, g, 30)
+    for kz in ((1, 0), (7, 1), (64, 0)):
+        Pf, Sf, _ = fused_run(hip, p, g, 30, kz=kz)
+        assert np.array_equal(Pf, Pp), kz
+        assert np.array_equal(Sf, Sp), kz
+
+
+def test_fused_fp64_one_step_random_state_reference_media(hip, orc):
+    """Random state, two media with P_d != 0 (generic pow), one step, against
+    the oracle's full step (u skipped: not on the P/S path)."""
+    p = ref_params(21, 17, 13, source=2)
+    g = random_state(p, np.random.default_rng(12))
+    Pf, Sf, _ = fused_run(hip, p, g, 1)
+    want = g.copy()
+    orc.run(want, p, 1, with_u=0)
+    assert max_rel(Pf, want["P1"]) <= 1e-12
+    assert max_rel(Sf, want["S2"]) <= 1e-12
+
+
+def test_fused_fp64_c1_trajectory_vs_oracle(hip, orc):
+    p = c1_params()
+    g = HostGrid(p)
+    orc.initial_data(g, p)
+    Pf, Sf, mf = fused_run(hip, p, g, 500, graph=True)
+    orc.run(g, p, 500, with_u=0)
+    assert max_rel(Pf, g["P1"]) <= 1e-10
+    assert max_rel(Sf, g["S2"]) <= 1e-10
+    mo = orc.mass(g, p)
+    assert abs(mf[0] - mo[0]) <= 1e-12 * mo[0]
+    assert abs(mf[1] - mo[1]) <= 1e-12 * mo[1]
+
+
+def test_mixed_c1_trajectory_error(hip, orc):
+    """Mixed form against the FP64 oracle on configuration 1, 500 steps."""
+    p = c1_params()
+    g = HostGrid(p)
+    orc.initial_data(g, p)
+    Pm, Sm, mm = fused_run(hip, p, g, 500, precision=1)
+    orc.run(g, p, 500, with_u=0)
+    eS = float(np.max(np.abs(Sm - g["S2"])))
+    eP = max_rel(Pm, g["P1"])
+    mo = orc.mass(g, p)
+    e1, e2 = abs(mm[0] - mo[0]) / mo[0], abs(mm[1] - mo[1]) / mo[1]
+    print(f"mixed C1 500 steps: S2 Linf {eS:.3e}, P1 rel Linf {eP:.3e}, "
+          f"water mass {e1:.3e}, NAPL mass {e2:.3e}")
+    assert eS <= 1e-4
+    assert eP <= 1e-6
+    assert e1 <= 1e-9 and e2 <= 1e-9
+
+
+def test_mixed_heterogeneous_error(hip, orc):
+    p, g = c2_like(48, seed=9, sigma=1.5)
+    Pm, Sm, mm = fused_run(hip, p, g, 100, precision=1)
+    orc.set_threads(8)
+    orc.run(g, p, 100, with_u=0)
+    orc.set_threads(1)
+    eS = float(np.max(np.abs(Sm - g["S2"])))
+    mo = orc.mass(g, p)
+    e2 = abs(mm[1] - mo[1]) / mo[1]
+    print(f"mixed 48^3 sigma 1.5, 100 steps: S2 Linf {eS:.3e}, P1 rel {max_rel(Pm, g['P1']):.3e},"
+          f" NAPL mass {e2:.3e}")
+    assert eS <= 1e-4 and e2 <= 1e-9
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_member_same_alone_and_in_mix(hip, precision):
+    import importlib
+    W = importlib.import_module("shmem-cuda-filtration_amd").workloads
+    rng = np.random.default_rng(4)
+    members = []
+    for q, (nx, ny, nz) in enumerate([(24, 16, 20), (17, 33, 9), (40, 8, 12), (9, 9, 9)]):
+        p = W.base_params(nx, ny, nz, h=W.INI_H)
+        W.set_patch(p, int(rng.integers(0, nx - 4)), int(rng.integers(0, nz - 4)))
+        perm, poro = W.gen_fields(p, 100 + q, sigma=float(rng.uniform(0.5, 2)), corr=4.0,
+                                  clip=2.5)
+        members.append((p, W.initial_grid(p, perm, poro)))
+    alone = [fused_run(hip, p, g, 40, precision=precision) for p, g in members]
+    e = hip.ensemble(precision)
+    try:
+        ids = [e.add(p, g["P1"], g["S2"], g["perm"], g["poro"]) for p, g in members]
+        e.capture(20)
+        e.replay()
+        e.replay()
+        for mid, (Pa, Sa, ma) in zip(ids, alone):
+            P, S = e.get(mid)
+            assert np.array_equal(P, Pa) and np.array_equal(S, Sa)
+            assert e.mass(mid) == ma
+            assert e.steps(mid) == 40
+    finally:
+        e.close()
+
+
+def test_continuous_batching_add_remove_between_steps(hip):
+    """Members admitted and retired while others run keep their own results."""
+    import importlib
+    W = importlib.import_module("shmem-cuda-filtration_amd").workloads
+    grids = []
+    for q, n in enumerate((20, 28, 12)):
+        p = W.base_params(n, n + 3, n - 2, h=W.INI_H)
+        perm, poro = W.gen_fields(p, 300 + q, sigma=1.0, corr=4.0, clip=2.5)
+        grids.append((p, W.initial_grid(p, perm, poro)))
+    alone = [fused_run(hip, p, g, 15, precision=1) for p, g in grids]
+    e = hip.ensemble(1)
+    try:
+        a = e.add(grids[0][0], grids[0][1]["P1"], grids[0][1]["S2"], grids[0][1]["perm"],
+                  grids[0][1]["poro"])
+        e.step(5)
+        b = e.add(grids[1][0], grids[1][1]["P1"], grids[1][1]["S2"], grids[1][1]["perm"],
+                  grids[1][1]["poro"])
+        e.step(10)
+        Pa, Sa = e.get(a)
+        e.remove(a)
+        c = e.add(grids[2][0], grids[2][1]["P1"], grids[2][1]["S2"], grids[2][1]["perm"],
+                  grids[2][1]["poro"])
+        e.step(5)
+        Pb, Sb = e.get(b)
+        e.step(10)
+        Pc, Sc = e.get(c)
+        assert np.array_equal(Pa, alone[0][0]) and np.array_equal(Sa, alone[0][1])
+        assert np.array_equal(Pb, alone[1][0]) and np.array_equal(Sb, alone[1][1])
+        assert np.array_equal(Pc, alone[2][0]) and np.array_equal(Sc, alone[2][1])
+    finally:
+        e.close()
+
+
+def test_graph_replays_match_eager_steps(hip):
+    p, g = c2_like(26)
+    Pe, Se, _ = fused_run(hip, p, g, 12)
+    e = hip.ensemble(0)
+    try:
+        mid = e.add(p, g["P1"], g["S2"], g["perm"], g["poro"])
+        e.capture(3)
+        for r in range(4):
+            e.replay()
+            e.mass(mid)  # eager work and a sync between replays
+        P, S = e.get(mid)
+        assert np.array_equal(P, Pe) and np.array_equal(S, Se)
+    finally:
+        e.close()
+
+
+def test_ensemble_rejects_degenerate_grid(hip):
+    p = ref_params(9, 8, 2)
+    g = HostGrid(p)
+    e = hip.ensemble(0)
+    try:
+        with pytest.raises(Exception):
+            e.add(p, g["P1"], g["S2"])
+    finally:
+        e.close()
