@@ -46,8 +46,10 @@ struct filt_ensemble {
     size_t d_partial_cap = 0;
     double* d_mass = nullptr;
     FusedGrid* d_one = nullptr;  // single-grid descriptor for the mass kernel
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    // one graph per starting buffer parity (the kernels' parity argument is
+    // baked into the captured launches)
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
     int graph_steps = 0;
 };
 
@@ -55,10 +57,12 @@ namespace {
 
 void drop_graph(filt_ensemble* e)
 {
-    if (e->exec) (void)hipGraphExecDestroy(e->exec);
-    if (e->graph) (void)hipGraphDestroy(e->graph);
-    e->exec = nullptr;
-    e->graph = nullptr;
+    for (int q = 0; q < 2; q++) {
+        if (e->exec[q]) (void)hipGraphExecDestroy(e->exec[q]);
+        if (e->graph[q]) (void)hipGraphDestroy(e->graph[q]);
+        e->exec[q] = nullptr;
+        e->graph[q] = nullptr;
+    }
     e->graph_steps = 0;
 }
 
@@ -314,36 +318,41 @@ int filt_ensemble_capture(filt_ensemble* e, int nsteps)
         return rc;
     drop_graph(e);
     FILT_HIP_TRY(hipStreamSynchronize(stream()));
-    FILT_HIP_TRY(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
-    int par = e->parity;
-    for (int s = 0; s < nsteps && rc == FILT_OK; s++) {
-        rc = fused_launch(e->d_grids, e->d_block_grid, e->total_blocks, par, e->precision, e->order,
-                          stream());
-        par ^= 1;
+    for (int start = 0; start < 2; start++) {
+        FILT_HIP_TRY(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
+        int par = start;
+        for (int s = 0; s < nsteps && rc == FILT_OK; s++) {
+            rc = fused_launch(e->d_grids, e->d_block_grid, e->total_blocks, par, e->precision,
+                              e->order, stream());
+            par ^= 1;
+        }
+        hipGraph_t g = nullptr;
+        hipError_t r = hipStreamEndCapture(stream(), &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            drop_graph(e);
+            return rc;
+        }
+        if (r != hipSuccess) {
+            drop_graph(e);
+            return hip_fail(r, "hipStreamEndCapture");
+        }
+        r = hipGraphInstantiate(&e->exec[start], g, nullptr, nullptr, 0);
+        e->graph[start] = g;
+        if (r != hipSuccess) {
+            drop_graph(e);
+            return hip_fail(r, "hipGraphInstantiate");
+        }
     }
-    hipGraph_t g = nullptr;
-    hipError_t r = hipStreamEndCapture(stream(), &g);
-    if (rc) {
-        if (g) (void)hipGraphDestroy(g);
-        return rc;
-    }
-    if (r != hipSuccess)
-        return hip_fail(r, "hipStreamEndCapture");
-    r = hipGraphInstantiate(&e->exec, g, nullptr, nullptr, 0);
-    if (r != hipSuccess) {
-        (void)hipGraphDestroy(g);
-        return hip_fail(r, "hipGraphInstantiate");
-    }
-    e->graph = g;
     e->graph_steps = nsteps;
     return FILT_OK;
 }
 
 int filt_ensemble_replay(filt_ensemble* e)
 {
-    if (!e || !e->exec)
+    if (!e || !e->exec[e->parity])
         return set_error(FILT_ERR_INVALID, "no captured graph (membership changed?)");
-    FILT_HIP_TRY(hipGraphLaunch(e->exec, stream()));
+    FILT_HIP_TRY(hipGraphLaunch(e->exec[e->parity], stream()));
     if (e->graph_steps & 1)
         e->parity ^= 1;
     for (auto& m : e->members)

@@ -128,18 +128,20 @@ struct Mixed {
         const int md = media_at(G, c);
         const float K = G.perm32 ? G.perm32[c] : (float)G.p.K[md];
         const double P = Pin[c];
-        const float S = (float)Sin[c];
-        const float swr = (float)G.p.S_wr[md];
-        const float se = (1.f - S - swr) * (float)G.inv_1mswr[md];
+        // 1 - S_e = S/(1-S_wr) formed directly: at small NAPL saturation the
+        // FP32 difference 1 - S_e would cancel to a few significant bits.
+        const float oms = (float)(Sin[c] * G.inv_1mswr[md]);
+        const float se = 1.f - oms;
         float k1, k2;
         if (G.p.lambda[md] == 2.0) {  // Brooks-Corey exponents 4 and 2
             const float s2 = se * se;
             k1 = s2 * s2;
-            k2 = (1.f - se) * (1.f - se) * (1.f - s2);
+            k2 = oms * oms * (oms * (2.f - oms));  // (1-S_e)^2 (1-S_e^2)
         } else {
             const float lam = (float)G.p.lambda[md];
             k1 = powf(se, (2.f + 3.f * lam) / lam);
-            k2 = (1.f - se) * (1.f - se) * (1.f - powf(se, (2.f + lam) / lam));
+            // 1 - S_e^e2 = -expm1(e2*log1p(-oms)), accurate for S_e near 1
+            k2 = oms * oms * -expm1f((2.f + lam) / lam * log1pf(-oms));
         }
         C1 r;
         r.P = P;
@@ -182,9 +184,13 @@ struct Mixed {
                          flux(ay2n, ay1n, L2(cc), L2(ym), L2(yp)) * ihy +
                          flux(az2, az1, L2(cc), L2(zm), L2(zp)) * ihz;
         const int md = media_at(G, c);
-        const float dtphi = (float)p.dt / (G.poro32 ? G.poro32[c] : (float)p.m[md]);
-        const double A1 = cc.r1 * (1 - S) - (double)(dtphi * F1);
-        const double A2 = cc.r2 * S - (double)(dtphi * F2);
+        // dt/phi in FP64: an FP32 factor would bias every increment alike
+        const double phi = G.poro32 ? (double)G.poro32[c] : p.m[md];
+        double rphi = __builtin_amdgcn_rcp(phi);
+        rphi = fma(fma(-phi, rphi, 1.0), rphi, rphi);
+        const double dtphi = p.dt * rphi;
+        const double A1 = cc.r1 * (1 - S) - dtphi * (double)F1;
+        const double A2 = cc.r2 * S - dtphi * (double)F2;
         // Newton (cpu.cpp:293-320) in FP64 with reciprocals
         const double ro01 = p.ro01, ro02 = p.ro02, b1 = p.beta1, b2 = p.beta2, Pa = p.P_atm;
         const double swr = p.S_wr[md], pd = p.P_d[md], lam = p.lambda[md];
