@@ -91,11 +91,10 @@ def load_traffic(workload_tag, kernel):
         return json.load(fh).get("bytes_per_launch")
 
 
-def run_plain(args):
+def run_plain(args, p, perm, poro):
     from importlib import import_module
     hiprt = import_module(PKG.__name__ + ".hiprt")
     be = PKG.device.Backend()
-    p, perm, poro = PKG.workloads.c2_workload(args.n)
     r = PKG.workloads.stability(p, perm, poro)
     if r > 2.0:
         raise SystemExit(f"stability guard: r = {r:.3g} > 2 for this workload")
@@ -153,6 +152,55 @@ def run_plain(args):
     return p, wall, stage_ms, r, finite, tuned
 
 
+FUSED_BYTES = {0: 48, 1: 40}  # P, S in + out (32) + perm, poro (FP64 16 / FP32 8)
+
+
+def time_ensemble(e, steps, hiprt, lib):
+    """Capture `steps` fused steps as one graph, replay it once untimed and
+    once timed (HIP events on the library stream).  Returns ms per step."""
+    e.capture(steps)
+    e.replay()
+    stream = lib.filt_get_stream()
+    a, b = hiprt.Event(), hiprt.Event()
+    hiprt.device_synchronize()
+    t0 = time.perf_counter()
+    a.record(stream)
+    e.replay()
+    b.record(stream)
+    b.sync()
+    wall = time.perf_counter() - t0
+    return a.elapsed_ms(b) / steps, wall
+
+
+def run_fused(args, p, perm, poro, precision):
+    """The same workload through the fused one-launch-per-step form."""
+    hiprt = importlib.import_module(PKG.__name__ + ".hiprt")
+    be = PKG.device.Backend()
+    g = PKG.workloads.initial_grid(p, perm, poro)
+    e = be.ensemble(precision)
+    try:
+        e.add(p, g["P1"], g["S2"], perm, poro)
+        kz, order, _ = e.autotune(3)
+        ms, wall = time_ensemble(e, args.steps, hiprt, be.lib)
+    finally:
+        e.close()
+    # accuracy run: exactly args.steps steps from the initial state
+    e = be.ensemble(precision)
+    try:
+        mid = e.add(p, g["P1"], g["S2"], perm, poro)
+        e.set_tiling(kz, order)
+        e.step(args.steps)
+        P, S = e.get(mid)
+        m = e.mass(mid)
+    finally:
+        e.close()
+    cells = p.NX * p.Ny * p.Nz
+    return {"value": cells / (ms * 1e-3), "ms_per_step": ms, "kz": kz, "order": order,
+            "bytes_per_cell_update": FUSED_BYTES[precision],
+            "hbm_GBps": FUSED_BYTES[precision] * cells / (ms * 1e-3) / 1e9,
+            "wall_s": wall}, (P, S, m)
+
+
 def copy_calibration(n_bytes=1 << 30, reps=5):
     """The library's streaming copy kernel over n_bytes: the achievable HBM
     ceiling, and the reference point that scales the PMC byte counters."""
@@ -187,6 +235,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=256, help="grid edge")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--plain-only", action="store_true",
+                    help="skip the fused FP64 / mixed forms")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default stencil tile shapes")
     ap.add_argument("--copy", action="store_true",
@@ -200,7 +250,22 @@ def main():
         print(json.dumps(copy_calibration()), flush=True)
         return
 
-    p, wall, stage_ms, r, finite, tuned = run_plain(args)
+    p, perm, poro = PKG.workloads.c2_workload(args.n)
+    p, wall, stage_ms, r, finite, tuned = run_plain(args, p, perm, poro)
+    forms = {}
+    if not args.plain_only:
+        import numpy as np
+        f64, (P64, S64, m64) = run_fused(args, p, perm, poro, 0)
+        mix, (Pm, Sm, mm) = run_fused(args, p, perm, poro, 1)
+        mix["error_vs_fp64"] = {
+            "S2_linf": float(np.max(np.abs(Sm - S64))),
+            "P1_rel_linf": float(np.max(np.abs(Pm - P64)) / np.max(np.abs(P64))),
+            "water_mass_rel": abs(mm[0] - m64[0]) / m64[0],
+            "napl_mass_rel": abs(mm[1] - m64[1]) / m64[1],
+            "total_mass_rel": abs(mm[0] + mm[1] - m64[0] - m64[1]) / (m64[0] + m64[1]),
+            "reference": "fused FP64 (bit-identical to the plain path) after the same steps",
+        }
+        forms = {"fused_fp64": f64, "mixed": mix}
     cells = p.NX * p.Ny * p.Nz
     value = cells * args.steps / wall
     dom = max((s for s in PLAIN_STAGES if PLAIN_BYTES[s]), key=lambda s: stage_ms[s])
@@ -249,6 +314,7 @@ def main():
             "achieved_GBps": step_bytes * args.steps / wall / 1e9,
             "frac": step_bytes * args.steps / wall / HBM_PEAK,
         },
+        "forms": forms,
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args)

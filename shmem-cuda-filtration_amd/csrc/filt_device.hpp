@@ -104,8 +104,15 @@ __device__ inline void newton_cell(double& P, double& S, double roS1, double roS
         const double det = F1P * F2S - F1S * F2P;
         const double Pn = P - (1 / det) * (F2S * F1 - F1S * F2);
         const double Sn = S - (1 / det) * (F1P * F2 - F2P * F1);
+        // An iteration is a function of (P, S) alone: once it returns its
+        // input, the remaining iterations would too, so stopping there gives
+        // the reference's result bit for bit (NaN never compares equal and
+        // runs all iterations, as the reference does).
+        const bool fixed = (Pn == P) && (Sn == S);
         P = Pn;
         S = Sn;
+        if (fixed)
+            break;
     }
 }
 

@@ -36,6 +36,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/filtration_hip.h"
@@ -56,172 +57,199 @@ constexpr int SN = SX * (FTY + 2);
 
 __device__ __forceinline__ int slot(int x, int y) { return (y + 1) * SX + (x + 1); }
 
-// Stage-1 values of one cell as the flux needs them (P2 = P1 + 0 = P1).
-struct C1 {
-    double P, X1, X2, r1, r2;
-};
-
 __device__ __forceinline__ int media_at(const FusedGrid& G, long long c)
 {
     return (G.media && G.p.n_media > 1) ? G.media[c] : 0;
 }
 
 // ------------------------------------------------------------ FP64 form --
+// Stage-1 values of one cell as the flux needs them (P2 = P1 + 0 = P1).
+struct C1 {
+    double P, X1, X2, r1, r2;
+};
 
-struct Fp64 {
-    __device__ static inline C1 cell(const FusedGrid& G, const double* Pin, const double* Sin,
-                                     long long c)
-    {
-        const int md = media_at(G, c);
-        const double K = G.perm ? G.perm[c] : G.p.K[md];
-        const double P = Pin[c];
-        const Stage1 r = stage1_cell(P, Sin[c], K, G.p.lambda[md], G.p.S_wr[md], G.p);
-        return C1{P, r.Xi1, r.Xi2, r.ro1, r.ro2};
-    }
+__device__ inline C1 fp64_cell(const FusedGrid& G, const double* Pin, const double* Sin,
+                               long long c)
+{
+    const int md = media_at(G, c);
+    const double K = G.perm ? G.perm[c] : G.p.K[md];
+    const double P = Pin[c];
+    const Stage1 r = stage1_cell(P, Sin[c], K, G.p.lambda[md], G.p.S_wr[md], G.p);
+    return C1{P, r.Xi1, r.Xi2, r.ro1, r.ro2};
+}
 
-    // roS_calculation (cpu.cpp:218-291) then Newton (cpu.cpp:293-320) for an
-    // interior cell; same expressions as filt_stages.hip.
-    __device__ static inline void update(const FusedGrid& G, long long c, const C1& cc,
-                                         const C1& xm, const C1& xp, const C1& ym, const C1& yp,
-                                         const C1& zm, const C1& zp, double S, double& nP,
-                                         double& nS)
+// LDS planes of the FP64 form: P, Xi1, ro1, Xi2, ro2.
+struct Fp64Lds {
+    double* P;
+    double* X1;
+    double* R1;
+    double* X2;
+    double* R2;
+    __device__ void put(int s, const C1& v) const
     {
-        const filt_params& p = G.p;
-        const double roS1 = cc.r1 * (1 - S);
-        const double roS2 = cc.r2 * S;
-        const double p1 = cc.P, g = p.g_const;
-        double a1, a2;
-        a2 = -(zp.P - p1) / p.h3;
-        a1 = -(p1 - zm.P) / p.h3;
-        const double f3 = upwind(a2, a1, cc.X1, cc.r1, zm.X1, zm.r1, zp.X1, zp.r1, p.h3);
-        const double g3 = upwind(a2, a1, cc.X2, cc.r2, zm.X2, zm.r2, zp.X2, zp.r2, p.h3);
-        a2 = -(xp.P - p1) / p.h1;
-        a1 = -(p1 - xm.P) / p.h1;
-        const double f1 = upwind(a2, a1, cc.X1, cc.r1, xm.X1, xm.r1, xp.X1, xp.r1, p.h1);
-        const double g1 = upwind(a2, a1, cc.X2, cc.r2, xm.X2, xm.r2, xp.X2, xp.r2, p.h1);
-        const double dy2 = -(yp.P - p1) / p.h2, dy1 = -(p1 - ym.P) / p.h2;
-        const double f2 = upwind(dy2 + cc.r1 * g, dy1 + cc.r1 * g, cc.X1, cc.r1, ym.X1, ym.r1,
-                                 yp.X1, yp.r1, p.h2);
-        const double g2 = upwind(dy2 + cc.r2 * g, dy1 + cc.r2 * g, cc.X2, cc.r2, ym.X2, ym.r2,
-                                 yp.X2, yp.r2, p.h2);
-        const int md = media_at(G, c);
-        const double phi = G.poro ? G.poro[c] : p.m[md];
-        const double A1 = roS1 - (p.dt / phi) * (f1 + f2 + f3);
-        const double A2 = roS2 - (p.dt / phi) * (g1 + g2 + g3);
-        nP = cc.P;
-        nS = S;
-        newton_cell(nP, nS, A1, A2, p.lambda[md], p.S_wr[md], p.P_d[md], p);
+        P[s] = v.P;
+        X1[s] = v.X1;
+        R1[s] = v.r1;
+        X2[s] = v.X2;
+        R2[s] = v.r2;
     }
 };
+
+// roS_calculation (cpu.cpp:218-291) then Newton (cpu.cpp:293-320) for an
+// interior cell, expression for expression as in filt_stages.hip; neighbours
+// in x / y from LDS slots, in z from the registers zm / zp.
+__device__ inline void fp64_update(const FusedGrid& G, long long c, const C1& cc, const C1& zm,
+                                   const C1& zp, const Fp64Lds& L, int s, double S, double& nP,
+                                   double& nS)
+{
+    const filt_params& p = G.p;
+    const int xm = s - 1, xp = s + 1, ym = s - SX, yp = s + SX;
+    const double p1 = cc.P, g = p.g_const;
+    // face forces: P2 = P1 here, so both phases share them (y adds ro_c*g)
+    const double az2 = -(zp.P - p1) / p.h3, az1 = -(p1 - zm.P) / p.h3;
+    const double ax2 = -(L.P[xp] - p1) / p.h1, ax1 = -(p1 - L.P[xm]) / p.h1;
+    const double dy2 = -(L.P[yp] - p1) / p.h2, dy1 = -(p1 - L.P[ym]) / p.h2;
+    const int md = media_at(G, c);
+    const double phi = G.poro ? G.poro[c] : p.m[md];
+    const double f3 = upwind(az2, az1, cc.X1, cc.r1, zm.X1, zm.r1, zp.X1, zp.r1, p.h3);
+    const double f1 = upwind(ax2, ax1, cc.X1, cc.r1, L.X1[xm], L.R1[xm], L.X1[xp], L.R1[xp], p.h1);
+    const double f2 = upwind(dy2 + cc.r1 * g, dy1 + cc.r1 * g, cc.X1, cc.r1, L.X1[ym], L.R1[ym],
+                             L.X1[yp], L.R1[yp], p.h2);
+    const double A1 = cc.r1 * (1 - S) - (p.dt / phi) * (f1 + f2 + f3);
+    const double g3 = upwind(az2, az1, cc.X2, cc.r2, zm.X2, zm.r2, zp.X2, zp.r2, p.h3);
+    const double g1 = upwind(ax2, ax1, cc.X2, cc.r2, L.X2[xm], L.R2[xm], L.X2[xp], L.R2[xp], p.h1);
+    const double g2 = upwind(dy2 + cc.r2 * g, dy1 + cc.r2 * g, cc.X2, cc.r2, L.X2[ym], L.R2[ym],
+                             L.X2[yp], L.R2[yp], p.h2);
+    const double A2 = cc.r2 * S - (p.dt / phi) * (g1 + g2 + g3);
+    nP = cc.P;
+    nS = S;
+    newton_cell(nP, nS, A1, A2, p.lambda[md], p.S_wr[md], p.P_d[md], p);
+}
 
 // ----------------------------------------------------------- mixed form --
-// Stencil fluxes and the ro*S increment in FP32; pressure differences (the
-// y face force -dP/h + ro*g cancels toward hydrostatic balance), the
-// ro(P)*S base and the Newton in FP64.  Divisions by the uniform spacings
-// and 1/(1-S_wr) become multiplications by reciprocals; 1/det is an FP64
-// reciprocal refined by one Newton-Raphson step.  Statics are FP32.
+// Stencil fluxes in FP32 on FP32 mobility-density products L = -Xi*ro >= 0;
+// pressure differences (the y face force -dP/h + ro*g cancels toward
+// hydrostatic balance), the ro(P)*S base, dt/phi times the flux, the Newton
+// and the mass sums in FP64.  Divisions by the uniform spacings, the
+// viscosities and 1 - S_wr are multiplications by reciprocals; 1/det is an
+// FP64 reciprocal refined by one Newton-Raphson step.  Statics are FP32.
+struct M1 {
+    double P;
+    float L1, L2;
+};
 
-struct Mixed {
-    __device__ static inline C1 cell(const FusedGrid& G, const double* Pin, const double* Sin,
-                                     long long c)
-    {
-        const int md = media_at(G, c);
-        const float K = G.perm32 ? G.perm32[c] : (float)G.p.K[md];
-        const double P = Pin[c];
-        // 1 - S_e = S/(1-S_wr) formed directly: at small NAPL saturation the
-        // FP32 difference 1 - S_e would cancel to a few significant bits.
-        const float oms = (float)(Sin[c] * G.inv_1mswr[md]);
-        const float se = 1.f - oms;
-        float k1, k2;
-        if (G.p.lambda[md] == 2.0) {  // Brooks-Corey exponents 4 and 2
-            const float s2 = se * se;
-            k1 = s2 * s2;
-            k2 = oms * oms * (oms * (2.f - oms));  // (1-S_e)^2 (1-S_e^2)
-        } else {
-            const float lam = (float)G.p.lambda[md];
-            k1 = powf(se, (2.f + 3.f * lam) / lam);
-            // 1 - S_e^e2 = -expm1(e2*log1p(-oms)), accurate for S_e near 1
-            k2 = oms * oms * -expm1f((2.f + lam) / lam * log1pf(-oms));
-        }
-        C1 r;
-        r.P = P;
-        r.X1 = (double)(-K * k1 * (float)G.inv_mu1);
-        r.X2 = (double)(-K * k2 * (float)G.inv_mu2);
-        r.r1 = G.p.ro01 * (1. + G.p.beta1 * (P - G.p.P_atm));
-        r.r2 = G.p.ro02 * (1. + G.p.beta2 * (P - G.p.P_atm));
-        return r;
+__device__ inline M1 mixed_cell(const FusedGrid& G, const double* Pin, const double* Sin,
+                                long long c)
+{
+    const int md = media_at(G, c);
+    const float K = G.perm32 ? G.perm32[c] : (float)G.p.K[md];
+    const double P = Pin[c];
+    // 1 - S_e = S/(1-S_wr) formed directly: at small NAPL saturation the FP32
+    // difference 1 - S_e would cancel to a few significant bits.
+    const float oms = (float)(Sin[c] * G.inv_1mswr[md]);
+    const float se = 1.f - oms;
+    float k1, k2;
+    if (G.p.lambda[md] == 2.0) {  // Brooks-Corey exponents 4 and 2
+        const float s2 = se * se;
+        k1 = s2 * s2;
+        k2 = oms * oms * (oms * (2.f - oms));  // (1-S_e)^2 (1-S_e^2)
+    } else {
+        const float lam = (float)G.p.lambda[md];
+        k1 = powf(se, (2.f + 3.f * lam) / lam);
+        // 1 - S_e^e2 = -expm1(e2*log1p(-oms)), accurate for S_e near 1
+        k2 = oms * oms * -expm1f((2.f + lam) / lam * log1pf(-oms));
     }
+    const float r1 = (float)(G.p.ro01 * (1. + G.p.beta1 * (P - G.p.P_atm)));
+    const float r2 = (float)(G.p.ro02 * (1. + G.p.beta2 * (P - G.p.P_atm)));
+    M1 v;
+    v.P = P;
+    v.L1 = K * k1 * (float)G.inv_mu1 * r1;
+    v.L2 = K * k2 * (float)G.inv_mu2 * r2;
+    return v;
+}
 
-    // flux with L = -Xi*ro >= 0 at centre, - and + neighbours
-    __device__ static inline float flux(float a2, float a1, float Lc, float Lm, float Lp)
+struct MixedLds {
+    double* P;
+    float* L1;
+    float* L2;
+    __device__ void put(int s, const M1& v) const
     {
-        return (fmaxf(a2, 0.f) - fminf(a1, 0.f)) * Lc - fmaxf(a1, 0.f) * Lm + fminf(a2, 0.f) * Lp;
-    }
-
-    __device__ static inline void update(const FusedGrid& G, long long c, const C1& cc,
-                                         const C1& xm, const C1& xp, const C1& ym, const C1& yp,
-                                         const C1& zm, const C1& zp, double S, double& nP,
-                                         double& nS)
-    {
-        const filt_params& p = G.p;
-        const double p1 = cc.P;
-        // face forces in FP64, then FP32
-        const float ax2 = (float)(-(xp.P - p1) * G.inv_h[0]);
-        const float ax1 = (float)(-(p1 - xm.P) * G.inv_h[0]);
-        const double dy2 = -(yp.P - p1) * G.inv_h[1], dy1 = -(p1 - ym.P) * G.inv_h[1];
-        const float az2 = (float)(-(zp.P - p1) * G.inv_h[2]);
-        const float az1 = (float)(-(p1 - zm.P) * G.inv_h[2]);
-        const double g = p.g_const;
-        const float ay2w = (float)(dy2 + cc.r1 * g), ay1w = (float)(dy1 + cc.r1 * g);
-        const float ay2n = (float)(dy2 + cc.r2 * g), ay1n = (float)(dy1 + cc.r2 * g);
-        auto L1 = [](const C1& v) { return -(float)v.X1 * (float)v.r1; };
-        auto L2 = [](const C1& v) { return -(float)v.X2 * (float)v.r2; };
-        const float ihx = (float)G.inv_h[0], ihy = (float)G.inv_h[1], ihz = (float)G.inv_h[2];
-        const float F1 = flux(ax2, ax1, L1(cc), L1(xm), L1(xp)) * ihx +
-                         flux(ay2w, ay1w, L1(cc), L1(ym), L1(yp)) * ihy +
-                         flux(az2, az1, L1(cc), L1(zm), L1(zp)) * ihz;
-        const float F2 = flux(ax2, ax1, L2(cc), L2(xm), L2(xp)) * ihx +
-                         flux(ay2n, ay1n, L2(cc), L2(ym), L2(yp)) * ihy +
-                         flux(az2, az1, L2(cc), L2(zm), L2(zp)) * ihz;
-        const int md = media_at(G, c);
-        // dt/phi in FP64: an FP32 factor would bias every increment alike
-        const double phi = G.poro32 ? (double)G.poro32[c] : p.m[md];
-        double rphi = __builtin_amdgcn_rcp(phi);
-        rphi = fma(fma(-phi, rphi, 1.0), rphi, rphi);
-        const double dtphi = p.dt * rphi;
-        const double A1 = cc.r1 * (1 - S) - dtphi * (double)F1;
-        const double A2 = cc.r2 * S - dtphi * (double)F2;
-        // Newton (cpu.cpp:293-320) in FP64 with reciprocals
-        const double ro01 = p.ro01, ro02 = p.ro02, b1 = p.beta1, b2 = p.beta2, Pa = p.P_atm;
-        const double swr = p.S_wr[md], pd = p.P_d[md], lam = p.lambda[md];
-        const double inv1 = G.inv_1mswr[md];
-        const bool no_cap = (pd == 0.0) && (lam >= 0.1);
-        double P = cc.P, Sv = S;
-        for (int w = 1; w <= p.newton_iterations; w++) {
-            const double se = (1 - Sv - swr) * inv1;
-            double P_k = 0.0, PkS = 0.0;
-            if (!(no_cap && se >= 0.0009765625)) {
-                P_k = pd * pow(se, (-1.) / lam);
-                PkS = pow(se, ((-1.) / lam) - 1.) * pd / (lam * (1 - swr));
-            }
-            const double F1v = ro01 * (1. + b1 * (P - Pa)) * (1. - Sv) - A1;
-            const double F2v = ro02 * (1. + b2 * (P + P_k - Pa)) * Sv - A2;
-            const double F1P = ro01 * b1 * (1 - Sv);
-            const double F2P = ro02 * b2 * Sv;
-            const double F1S = -ro01 * (1 + b1 * (P - Pa));
-            const double F2S = ro02 * (1 + b2 * (P + P_k - Pa + Sv * PkS));
-            const double det = F1P * F2S - F1S * F2P;
-            double r = __builtin_amdgcn_rcp(det);
-            r = fma(fma(-det, r, 1.0), r, r);
-            const double Pn = P - r * (F2S * F1v - F1S * F2v);
-            const double Sn = Sv - r * (F1P * F2v - F2P * F1v);
-            P = Pn;
-            Sv = Sn;
-        }
-        nP = P;
-        nS = Sv;
+        P[s] = v.P;
+        L1[s] = v.L1;
+        L2[s] = v.L2;
     }
 };
+
+// upwind flux with the face forces a2 (+ face) and a1 (- face)
+__device__ __forceinline__ float uflux(float a2, float a1, float Lc, float Lm, float Lp)
+{
+    return (fmaxf(a2, 0.f) - fminf(a1, 0.f)) * Lc - fmaxf(a1, 0.f) * Lm + fminf(a2, 0.f) * Lp;
+}
+
+__device__ inline void mixed_update(const FusedGrid& G, long long c, const M1& cc, const M1& zm,
+                                    const M1& zp, const MixedLds& L, int s, double S, double& nP,
+                                    double& nS)
+{
+    const filt_params& p = G.p;
+    const int xm = s - 1, xp = s + 1, ym = s - SX, yp = s + SX;
+    const double p1 = cc.P;
+    const double r1c = p.ro01 * (1. + p.beta1 * (p1 - p.P_atm));
+    const double r2c = p.ro02 * (1. + p.beta2 * (p1 - p.P_atm));
+    const double ihx = G.inv_h[0], ihy = G.inv_h[1], ihz = G.inv_h[2];
+    const float ax2 = (float)((p1 - L.P[xp]) * ihx), ax1 = (float)((L.P[xm] - p1) * ihx);
+    const float az2 = (float)((p1 - zp.P) * ihz), az1 = (float)((zm.P - p1) * ihz);
+    const double dy2 = (p1 - L.P[yp]) * ihy, dy1 = (L.P[ym] - p1) * ihy;
+    const double g = p.g_const;
+    const float fx = (float)ihx, fy = (float)ihy, fz = (float)ihz;
+    const float F1 = uflux(ax2, ax1, cc.L1, L.L1[xm], L.L1[xp]) * fx +
+                     uflux((float)(dy2 + r1c * g), (float)(dy1 + r1c * g), cc.L1, L.L1[ym],
+                           L.L1[yp]) * fy +
+                     uflux(az2, az1, cc.L1, zm.L1, zp.L1) * fz;
+    const float F2 = uflux(ax2, ax1, cc.L2, L.L2[xm], L.L2[xp]) * fx +
+                     uflux((float)(dy2 + r2c * g), (float)(dy1 + r2c * g), cc.L2, L.L2[ym],
+                           L.L2[yp]) * fy +
+                     uflux(az2, az1, cc.L2, zm.L2, zp.L2) * fz;
+    const int md = media_at(G, c);
+    // dt/phi in FP64: an FP32 factor would bias every increment alike
+    const double phi = G.poro32 ? (double)G.poro32[c] : p.m[md];
+    double rphi = __builtin_amdgcn_rcp(phi);
+    rphi = fma(fma(-phi, rphi, 1.0), rphi, rphi);
+    const double dtphi = p.dt * rphi;
+    const double A1 = r1c * (1 - S) - dtphi * (double)F1;
+    const double A2 = r2c * S - dtphi * (double)F2;
+    // Newton of cpu.cpp:293-320 in FP64 with reciprocals
+    const double ro01 = p.ro01, ro02 = p.ro02, b1 = p.beta1, b2 = p.beta2, Pa = p.P_atm;
+    const double swr = p.S_wr[md], pd = p.P_d[md], lam = p.lambda[md];
+    const double inv1 = G.inv_1mswr[md];
+    const bool no_cap = (pd == 0.0) && (lam >= 0.1);
+    double P = p1, Sv = S;
+    for (int w = 1; w <= p.newton_iterations; w++) {
+        const double se = (1 - Sv - swr) * inv1;
+        double P_k = 0.0, PkS = 0.0;
+        if (!(no_cap && se >= 0.0009765625)) {
+            P_k = pd * pow(se, (-1.) / lam);
+            PkS = pow(se, ((-1.) / lam) - 1.) * pd / (lam * (1 - swr));
+        }
+        const double F1v = ro01 * (1. + b1 * (P - Pa)) * (1. - Sv) - A1;
+        const double F2v = ro02 * (1. + b2 * (P + P_k - Pa)) * Sv - A2;
+        const double F1P = ro01 * b1 * (1 - Sv);
+        const double F2P = ro02 * b2 * Sv;
+        const double F1S = -ro01 * (1 + b1 * (P - Pa));
+        const double F2S = ro02 * (1 + b2 * (P + P_k - Pa + Sv * PkS));
+        const double det = F1P * F2S - F1S * F2P;
+        double r = __builtin_amdgcn_rcp(det);
+        r = fma(fma(-det, r, 1.0), r, r);
+        const double Pn = P - r * (F2S * F1v - F1S * F2v);
+        const double Sn = Sv - r * (F1P * F2v - F2P * F1v);
+        const bool fixed = (Pn == P) && (Sn == Sv);  // later iterations repeat it
+        P = Pn;
+        Sv = Sn;
+        if (fixed)
+            break;
+    }
+    nP = P;
+    nS = Sv;
+}
 
 // -------------------------------------------------------------- writing --
 
@@ -308,9 +336,15 @@ __device__ inline void shell_cell(const FusedGrid& G, const double* Pin, const d
 
 // ------------------------------------------------------------- the kernel --
 
-template <class Pol>
-__device__ void interior_tile(const FusedGrid& G, long long lb, int parity, double* lds)
+// ------------------------------------------------------------- the tiles --
+// One tile: TX x TY columns, planes [kb, ke) of the interior.  Per plane:
+// every thread evaluates stage 1 of its own column at k+1 (registers keep
+// k-1 and k), writes plane k into the LDS tile; threads t < RING (wave 0)
+// evaluate the one-cell ring; sync; interior cells update; sync.
+template <bool MIXED>
+__device__ void interior_tile(const FusedGrid& G, long long lb, int parity, char* lds)
 {
+    using V = typename std::conditional<MIXED, M1, C1>::type;
     const double* Pin = G.P[parity];
     const double* Sin = G.S[parity];
     double* Po = G.P[parity ^ 1];
@@ -327,11 +361,6 @@ __device__ void interior_tile(const FusedGrid& G, long long lb, int parity, doub
     const bool in_grid = i < nx && j < ny;
     const bool inner = i <= nx - 2 && j <= ny - 2;
     const long long sy = nx, sz = (long long)nx * ny;
-    double* sP = lds;
-    double* sX1 = lds + SN;
-    double* sX2 = lds + 2 * SN;
-    double* sR1 = lds + 3 * SN;
-    double* sR2 = lds + 4 * SN;
     // ring cell of this thread (t < RING)
     int rx = 0, ry = 0;
     if (t < FTX) { rx = t; ry = -1; }
@@ -341,41 +370,48 @@ __device__ void interior_tile(const FusedGrid& G, long long lb, int parity, doub
     const bool ring = t < RING && i0 + rx < nx && j0 + ry < ny;
     const long long rcol = (i0 + rx) + (j0 + ry) * sy;
     const long long col = i + j * sy;
+    const int s = slot(tx, ty), rs = slot(rx, ry);
 
-    C1 zm{}, zc{}, zp{};
+    typename std::conditional<MIXED, MixedLds, Fp64Lds>::type L;
+    if constexpr (MIXED) {
+        L.P = (double*)lds;
+        L.L1 = (float*)(lds + SN * sizeof(double));
+        L.L2 = L.L1 + SN;
+    } else {
+        double* d = (double*)lds;
+        L.P = d;
+        L.X1 = d + SN;
+        L.R1 = d + 2 * SN;
+        L.X2 = d + 3 * SN;
+        L.R2 = d + 4 * SN;
+    }
+    auto cell = [&](long long c) -> V {
+        if constexpr (MIXED)
+            return mixed_cell(G, Pin, Sin, c);
+        else
+            return fp64_cell(G, Pin, Sin, c);
+    };
+
+    V zm{}, zc{}, zp{};
     if (in_grid) {
-        zm = Pol::cell(G, Pin, Sin, col + (kb - 1) * sz);
-        zc = Pol::cell(G, Pin, Sin, col + kb * sz);
+        zm = cell(col + (kb - 1) * sz);
+        zc = cell(col + kb * sz);
     }
     for (int k = kb; k < ke; k++) {
         if (in_grid) {
-            zp = Pol::cell(G, Pin, Sin, col + (k + 1) * sz);
-            const int s = slot(tx, ty);
-            sP[s] = zc.P;
-            sX1[s] = zc.X1;
-            sX2[s] = zc.X2;
-            sR1[s] = zc.r1;
-            sR2[s] = zc.r2;
+            zp = cell(col + (k + 1) * sz);
+            L.put(s, zc);
         }
-        if (ring) {
-            const C1 v = Pol::cell(G, Pin, Sin, rcol + k * sz);
-            const int s = slot(rx, ry);
-            sP[s] = v.P;
-            sX1[s] = v.X1;
-            sX2[s] = v.X2;
-            sR1[s] = v.r1;
-            sR2[s] = v.r2;
-        }
+        if (ring)
+            L.put(rs, cell(rcol + k * sz));
         __syncthreads();
         if (in_grid && inner) {
-            auto at = [&](int x, int y) {
-                const int s = slot(x, y);
-                return C1{sP[s], sX1[s], sX2[s], sR1[s], sR2[s]};
-            };
             const long long c = col + k * sz;
             double nP, nS;
-            Pol::update(G, c, zc, at(tx - 1, ty), at(tx + 1, ty), at(tx, ty - 1), at(tx, ty + 1),
-                        zm, zp, Sin[c], nP, nS);
+            if constexpr (MIXED)
+                mixed_update(G, c, zc, zm, zp, L, s, Sin[c], nP, nS);
+            else
+                fp64_update(G, c, zc, zm, zp, L, s, Sin[c], nP, nS);
             write_cell(G, Po, So, Pin, i, j, k, nP, nS);
         }
         __syncthreads();
@@ -384,19 +420,21 @@ __device__ void interior_tile(const FusedGrid& G, long long lb, int parity, doub
     }
 }
 
-template <class Pol>
+constexpr size_t LDS_BYTES = 5 * SN * sizeof(double);  // the larger (FP64) layout
+
+template <bool MIXED>
 __global__ __launch_bounds__(FT) void k_fused(const FusedGrid* __restrict__ grids,
                                               const int* __restrict__ block_grid,
                                               long long total_blocks, int parity, int order)
 {
-    __shared__ double lds[5 * SN];
+    __shared__ __attribute__((aligned(16))) char lds[MIXED ? SN * 16 : LDS_BYTES];
     long long b = blockIdx.x;
     if (order == 0 && total_blocks % 8 == 0)
         b = (b % 8) * (total_blocks / 8) + b / 8;
     const FusedGrid& G = grids[block_grid[b]];
     const long long lb = b - G.first_block;
     if (lb < G.n_interior_blocks) {
-        interior_tile<Pol>(G, lb, parity, lds);
+        interior_tile<MIXED>(G, lb, parity, lds);
         return;
     }
     const long long q = (lb - G.n_interior_blocks) * FT + threadIdx.y * FTX + threadIdx.x;
@@ -497,10 +535,10 @@ int fused_launch(const FusedGrid* dev_grids, const int* dev_block_grid, long lon
     if (total_blocks > 0x7fffffffLL)
         return set_error(FILT_ERR_INVALID, "batch too large for one launch");
     if (precision == FILT_PREC_MIXED)
-        hipLaunchKernelGGL(k_fused<Mixed>, dim3((unsigned)total_blocks), dim3(FTX, FTY), 0, s,
+        hipLaunchKernelGGL(k_fused<true>, dim3((unsigned)total_blocks), dim3(FTX, FTY), 0, s,
                            dev_grids, dev_block_grid, total_blocks, parity, order);
     else
-        hipLaunchKernelGGL(k_fused<Fp64>, dim3((unsigned)total_blocks), dim3(FTX, FTY), 0, s,
+        hipLaunchKernelGGL(k_fused<false>, dim3((unsigned)total_blocks), dim3(FTX, FTY), 0, s,
                            dev_grids, dev_block_grid, total_blocks, parity, order);
     FILT_LAUNCH_CHECK("k_fused");
     return FILT_OK;
