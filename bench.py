@@ -201,6 +201,94 @@ def run_fused(args, p, perm, poro, precision):
             "wall_s": wall}, (P, S, m)
 
 
+def run_c4(args):
+    """Batched ensemble: args.c4 realizations with uneven grids, args.c4_steps
+    steps.  Plain FP64 path run realization by realization (all 5 stages,
+    all arrays, default tiles) against the batched forms (every member in one
+    launch per step, the whole run one hipGraph)."""
+    import numpy as np
+    hiprt = importlib.import_module(PKG.__name__ + ".hiprt")
+    be = PKG.device.Backend()
+    lib = be.lib
+    stream = lib.filt_get_stream()
+    W = PKG.workloads
+    t0 = time.perf_counter()
+    members = W.c4_members(args.c4)
+    gen_s = time.perf_counter() - t0
+    cells = sum(m["params"].NX * m["params"].Ny * m["params"].Nz for m in members)
+    steps = args.c4_steps
+    out = {"realizations": len(members), "cells": cells, "steps": steps,
+           "field_generation_s": gen_s,
+           "sigma_range": [min(m["sigma"] for m in members), max(m["sigma"] for m in members)],
+           "max_stability_r": max(m["stability_r"] for m in members)}
+    # plain, one realization after another (inputs resident first)
+    grids = []
+    for m in members:
+        g = W.initial_grid(m["params"], m["perm"], m["poro"])
+        d = be.grid(m["params"], perm=True, poro=True)
+        d.upload(g)
+        grids.append(d)
+    be.sync()
+    a, b = hiprt.Event(), hiprt.Event()
+    a.record(stream)
+    for d in grids:
+        for s in range(1, steps + 1):
+            PKG.abi.check(lib, lib.filt_time_step(ctypes.byref(d.f), ctypes.byref(d.params),
+                                                  s * d.params.dt, d.nx, 0, 1, 1), "plain")
+    b.record(stream)
+    b.sync()
+    plain_ms = a.elapsed_ms(b)
+    for d in grids:
+        d.free()
+    out["plain_per_realization"] = {"value": cells * steps / (plain_ms * 1e-3),
+                                    "total_ms": plain_ms}
+    results = {}
+    for prec, name in ((0, "batched_fp64"), (1, "batched_mixed")):
+        e = be.ensemble(prec)
+        ids = []
+        for m in members:
+            g = W.initial_grid(m["params"], m["perm"], m["poro"])
+            ids.append(e.add(m["params"], g["P1"], g["S2"], m["perm"], m["poro"]))
+        kz, order, _ = e.autotune(2)
+        e.close()
+        # fresh ensemble: the timed run starts from the initial states
+        e = be.ensemble(prec)
+        ids = []
+        for m in members:
+            g = W.initial_grid(m["params"], m["perm"], m["poro"])
+            ids.append(e.add(m["params"], g["P1"], g["S2"], m["perm"], m["poro"]))
+        e.set_tiling(kz, order)
+        e.capture(steps)
+        be.sync()
+        a.record(stream)
+        e.replay()
+        b.record(stream)
+        b.sync()
+        ms = a.elapsed_ms(b)
+        out[name] = {"value": cells * steps / (ms * 1e-3), "total_ms": ms, "kz": kz,
+                     "order": order, "bytes_per_cell_update": FUSED_BYTES[prec],
+                     "hbm_GBps": FUSED_BYTES[prec] * cells * steps / (ms * 1e-3) / 1e9}
+        results[prec] = [(e.get(mid), e.mass(mid)) for mid in ids]
+        e.close()
+    errs = {"S2_linf": 0.0, "P1_rel_linf": 0.0, "water_mass_rel": 0.0, "napl_mass_rel": 0.0,
+            "total_mass_rel": 0.0}
+    for ((P64, S64), m64), ((Pm, Sm), mm) in zip(results[0], results[1]):
+        errs["S2_linf"] = max(errs["S2_linf"], float(np.max(np.abs(Sm - S64))))
+        errs["P1_rel_linf"] = max(errs["P1_rel_linf"],
+                                  float(np.max(np.abs(Pm - P64)) / np.max(np.abs(P64))))
+        errs["water_mass_rel"] = max(errs["water_mass_rel"], abs(mm[0] - m64[0]) / m64[0])
+        if m64[1] > 0:
+            errs["napl_mass_rel"] = max(errs["napl_mass_rel"], abs(mm[1] - m64[1]) / m64[1])
+        errs["total_mass_rel"] = max(errs["total_mass_rel"],
+                                     abs(sum(mm) - sum(m64)) / sum(m64))
+    out["batched_mixed"]["error_vs_fp64_max_over_members"] = errs
+    out["speedup_batched_mixed_vs_plain"] = (out["batched_mixed"]["value"] /
+                                             out["plain_per_realization"]["value"])
+    out["speedup_batched_fp64_vs_plain"] = (out["batched_fp64"]["value"] /
+                                            out["plain_per_realization"]["value"])
+    return out
+
+
 def copy_calibration(n_bytes=1 << 30, reps=5):
     """The library's streaming copy kernel over n_bytes: the achievable HBM
     ceiling, and the reference point that scales the PMC byte counters."""
@@ -235,6 +323,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=256, help="grid edge")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--c4", type=int, default=96,
+                    help="realizations of the batched-ensemble leg (0: skip)")
+    ap.add_argument("--c4-steps", type=int, default=300)
     ap.add_argument("--plain-only", action="store_true",
                     help="skip the fused FP64 / mixed forms")
     ap.add_argument("--no-autotune", action="store_true",
@@ -266,6 +357,8 @@ def main():
             "reference": "fused FP64 (bit-identical to the plain path) after the same steps",
         }
         forms = {"fused_fp64": f64, "mixed": mix}
+        if args.c4 > 0:
+            forms["batched_ensemble"] = run_c4(args)
     cells = p.NX * p.Ny * p.Nz
     value = cells * args.steps / wall
     dom = max((s for s in PLAIN_STAGES if PLAIN_BYTES[s]), key=lambda s: stage_ms[s])

@@ -119,3 +119,36 @@ def c3_workload(nx=512, ny=512, nz=256, seed=3033):
     p = c3_params(nx, ny, nz)
     perm, poro = gen_fields(p, seed, sigma=1.5, corr=16.0, clip=2.5)
     return p, perm, poro
+
+
+C4_EDGES = (48, 64, 96, 128, 160)
+
+
+def ensemble_member(rng, edges=C4_EDGES, corr=8.0, seed=0, shape=None):
+    """One realization of the batched-ensemble / serving configurations: each
+    axis drawn from `edges` (or the given shape), sigma ~ U[0.5, 2],
+    correlation `corr` cells, porosity U[0.35, 0.45], a 4x4 source patch at a
+    random position on the top face.  ln K is clipped at +-2 sigma (with
+    sigma = 2 on a 160-cell column the unclipped tail exceeds the stability
+    limit); the clip is tightened further if the guard still exceeds 1.5."""
+    if shape is None:
+        shape = tuple(int(rng.choice(edges)) for _ in range(3))
+    nx, ny, nz = shape
+    p = base_params(nx, ny, nz, h=INI_H)
+    set_patch(p, int(rng.integers(0, nx - 3)), int(rng.integers(0, nz - 3)))
+    sigma = float(rng.uniform(0.5, 2.0))
+    clip = 2.0
+    while True:
+        perm, poro = gen_fields(p, seed, sigma=sigma, corr=corr, clip=clip)
+        r = stability(p, perm, poro)
+        if r <= 1.5 or clip <= 0.5:
+            break
+        clip -= 0.25
+    return {"params": p, "perm": perm, "poro": poro, "sigma": sigma, "clip": clip,
+            "stability_r": r, "shape": shape}
+
+
+def c4_members(n=96, seed=4096):
+    """Configuration 4: n realizations with edges from {48,64,96,128,160}."""
+    rng = np.random.default_rng(seed)
+    return [ensemble_member(rng, seed=seed * 1000 + q) for q in range(n)]
