@@ -18,6 +18,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
 ORACLE_LIB = os.path.join(HERE, "liboracle.so")
 REF_LIB = os.path.join(HERE, "_ref", "libref.so")
+# the reference driver linked against the HIP backend (drop-in check)
+REF_HIP_LIB = os.path.join(HERE, "_ref", "libref_hip.so")
 
 
 def _pkg():
@@ -195,8 +197,22 @@ class RefCPU:
         lib.ref_read_defines.argtypes = [c_char_p, C]
         lib.ref_read_defines.restype = c_int
         lib.ref_defines_file.restype = c_char_p
+        lib.ref_drive.argtypes = [C, c_int, c_void_p, c_void_p, c_size_t]
+        lib.ref_drive.restype = c_int
         assert lib.ref_sizeof_consts() == ctypes.sizeof(RefConsts)
         assert lib.ref_sizeof_ptr_arrays() == ctypes.sizeof(RefArrays)
+
+    def drive(self, p, nsteps):
+        """The reference driver end to end (Initialize, time_step_function x
+        nsteps, load_data_to_host, Finalize) on this library's backend."""
+        import numpy as np
+        shape = (p.Nz, p.Ny, p.NX)
+        P = np.empty(shape)
+        S = np.empty(shape)
+        rc = self.lib.ref_drive(ctypes.byref(to_ref_consts(p)), nsteps, P.ctypes.data,
+                                S.ctypes.data, P.size)
+        assert rc == 0
+        return P, S
 
     def constants(self):
         out = (c_double * 8)()

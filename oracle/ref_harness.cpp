@@ -95,4 +95,34 @@ int ref_read_defines(const char* workdir, consts* out)
 
 const char* ref_defines_file(void) { return DEFINES_FILE; }
 
+extern double* DevBuffer;  // main.cpp:7
+
+// The reference driver end to end with whichever backend this library links:
+// Initialize (main.cpp:204-231: Device_Initialize, memory_alloc,
+// initial_data, load_data_to_device), nsteps calls of time_step_function
+// (main.cpp:41-43), load_data_to_host of P1 and S2 (main.cpp:322-323),
+// Finalize.  Copies localNx*Ny*Nz values of P1 and S2 to the outputs.
+int ref_drive(const consts* def, int nsteps, double* P1_out, double* S2_out, size_t capacity)
+{
+    ptr_Arrays H, D;
+    memset(&H, 0, sizeof H);
+    memset(&D, 0, sizeof D);
+    int j = 0, localNx = 0, localNy = 0, size = 0, rank = 0, bx = 0, by = 0, bz = 0;
+    char arg0[] = "ref";
+    char* argv[] = {arg0, nullptr};
+    Initialize(&H, &D, &j, &localNx, &localNy, &size, &rank, &bx, &by, &bz, 1, argv, *def);
+    for (int s = 1; s <= nsteps; s++)
+        time_step_function(H, D, DevBuffer, *def, s * def->dt, localNx, localNy, rank, size, bx,
+                           by, bz);
+    load_data_to_host(H.P1, D.P1, localNx, *def);
+    load_data_to_host(H.S2, D.S2, localNx, *def);
+    const size_t n = (size_t)localNx * def->Ny * def->Nz;
+    if (n <= capacity) {
+        memcpy(P1_out, H.P1, n * sizeof(double));
+        memcpy(S2_out, H.S2, n * sizeof(double));
+    }
+    Finalize(H, D, DevBuffer);
+    return n <= capacity ? 0 : 1;
+}
+
 }  // extern "C"

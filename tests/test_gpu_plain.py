@@ -6,6 +6,8 @@ bit-exact; stages that call pow (ro_P2_Xi, Newton with P_d != 0) agree to
 1e-13 relative per element (ROCm's pow vs glibc's, ulp level); trajectories
 agree to relative L-inf <= 1e-10 on P1 and S2 and to 1e-12 relative on total
 water / NAPL mass."""
+import os
+
 import numpy as np
 import pytest
 
@@ -203,3 +205,23 @@ def test_autotuner_picks_a_config_and_keeps_results(hip, orc):
     finally:
         d.free()
         lib.filt_set_tile_config(0, 64, 4, 32, 0)
+
+
+@pytest.mark.parametrize("dims,source,steps", [((15, 15, 15), 3, 40), ((24, 20, 16), 2, 60)])
+def test_reference_driver_links_and_runs_on_hip_backend(dims, source, steps):
+    """Drop-in: the reference's own main.cpp driver (Initialize,
+    time_step_function, load_data_to_host, Finalize) and
+    no_communication.cpp, linked against the HIP backend's defines.h shims
+    (compiled against the reference's defines.h) instead of cpu.cpp / gpu.cu,
+    gives the same P1, S2 as the same driver on the reference CPU backend."""
+    import oracle
+    if not (os.path.exists(oracle.REF_HIP_LIB) and oracle.ref_available()):
+        pytest.skip("oracle/_ref not built (reference tree absent at build time)")
+    p = ref_params(*dims, source=source)
+    cpu = oracle.RefCPU(oracle.REF_LIB)
+    gpu = oracle.RefCPU(oracle.REF_HIP_LIB)
+    Pc, Sc = cpu.drive(p, steps)
+    Pg, Sg = gpu.drive(p, steps)
+    assert max_rel(Pg, Pc) <= 1e-10
+    assert max_rel(Sg, Sc) <= 1e-10
+    assert np.any(Sc[:, 1, :] > 0)  # NAPL has entered the domain
