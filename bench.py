@@ -289,6 +289,25 @@ def run_c4(args):
     return out
 
 
+def run_serving(args):
+    """Serving configuration: Poisson arrivals (rate args.serve_rate per s for
+    args.serve_duration s), edges log-uniform in [32, 192], steps U[100, 2000],
+    continuous batching through filt_serve_*.  FP64 (the fused form, whose
+    results equal the plain path's bit for bit) against mixed precision on the
+    same request stream."""
+    serving = importlib.import_module(PKG.__name__ + ".serving")
+    be = PKG.device.Backend()
+    stream = serving.c5_stream(rate=args.serve_rate, duration=args.serve_duration)
+    out = {"rate_per_s": args.serve_rate, "window_s": args.serve_duration,
+           "requests": len(stream),
+           "offered_cell_updates_per_s": sum(s[1][0] * s[1][1] * s[1][2] * s[2] for s in stream)
+           / args.serve_duration}
+    for prec, name in ((0, "fp64"), (1, "mixed")):
+        stats, _ = serving.run(be, prec, stream)
+        out[name] = stats
+    return out
+
+
 def copy_calibration(n_bytes=1 << 30, reps=5):
     """The library's streaming copy kernel over n_bytes: the achievable HBM
     ceiling, and the reference point that scales the PMC byte counters."""
@@ -326,6 +345,9 @@ def main():
     ap.add_argument("--c4", type=int, default=96,
                     help="realizations of the batched-ensemble leg (0: skip)")
     ap.add_argument("--c4-steps", type=int, default=300)
+    ap.add_argument("--serve-duration", type=float, default=20.0,
+                    help="arrival window of the serving leg in seconds (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--plain-only", action="store_true",
                     help="skip the fused FP64 / mixed forms")
     ap.add_argument("--no-autotune", action="store_true",
@@ -359,6 +381,8 @@ def main():
         forms = {"fused_fp64": f64, "mixed": mix}
         if args.c4 > 0:
             forms["batched_ensemble"] = run_c4(args)
+        if args.serve_duration > 0:
+            forms["serving"] = run_serving(args)
     cells = p.NX * p.Ny * p.Nz
     value = cells * args.steps / wall
     dom = max((s for s in PLAIN_STAGES if PLAIN_BYTES[s]), key=lambda s: stage_ms[s])

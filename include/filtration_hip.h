@@ -328,6 +328,35 @@ int filt_ensemble_autotune(filt_ensemble* e, int reps, int* best_kz, int* best_o
 /* Free / total device memory (admission control). */
 int filt_device_free_bytes(size_t* free_bytes, size_t* total_bytes);
 
+/* ------------------------------------------------------ serving loop -- */
+/*
+ * No reference counterpart.  Continuous batching of simulation requests: a
+ * FIFO queue, admission while a slot (max_active) and the member's device
+ * bytes plus reserve_bytes are free, all active requests advanced together
+ * (one fused launch per step, up to max_round_steps steps per round and never
+ * past the next completion), finished requests copied to host and freed.
+ * State stays on the device between admission and retirement.  A request's
+ * result equals the result of the same request run alone.  One host thread
+ * at a time.
+ */
+typedef struct filt_server filt_server;
+
+int filt_serve_create(filt_server** out, int precision, size_t reserve_bytes, int max_active,
+                      int max_round_steps);
+int filt_serve_destroy(filt_server* s);
+/* Queue a request (arrays copied; perm, poro, media may be null). */
+int filt_serve_submit(filt_server* s, const filt_params* p, const double* P1, const double* S2,
+                      const double* perm, const double* poro, const int* media,
+                      long long nsteps, int* id);
+/* One round: admit, step, retire.  Counts of this round (may be null). */
+int filt_serve_round(filt_server* s, int* n_admitted, int* n_retired);
+int filt_serve_status(const filt_server* s, int* queued, int* active, int* done,
+                      long long* active_cells, long long* cell_updates);
+int filt_serve_is_done(const filt_server* s, int id);
+/* Result of a finished request (P1, S2: NX*Ny*Nz each; mass2: water, NAPL);
+ * the server forgets it afterwards. */
+int filt_serve_take(filt_server* s, int id, double* P1, double* S2, double* mass2);
+
 /* ------------------------------------------------------------- runtime -- */
 
 /* Wait for all work the library enqueued. */

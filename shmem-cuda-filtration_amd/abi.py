@@ -165,6 +165,15 @@ def load_library(path=HIP_LIB):
     _sig(lib, "filt_ensemble_set_tiling", c_int, E, c_int, c_int)
     _sig(lib, "filt_ensemble_autotune", c_int, E, c_int, pi, pi, POINTER(c_double))
     _sig(lib, "filt_device_free_bytes", c_int, POINTER(c_size_t), POINTER(c_size_t))
+    _sig(lib, "filt_serve_create", c_int, POINTER(c_void_p), c_int, c_size_t, c_int, c_int)
+    _sig(lib, "filt_serve_destroy", c_int, c_void_p)
+    _sig(lib, "filt_serve_submit", c_int, c_void_p, P, c_void_p, c_void_p, c_void_p, c_void_p,
+         c_void_p, ctypes.c_longlong, pi)
+    _sig(lib, "filt_serve_round", c_int, c_void_p, pi, pi)
+    _sig(lib, "filt_serve_status", c_int, c_void_p, pi, pi, pi, POINTER(ctypes.c_longlong),
+         POINTER(ctypes.c_longlong))
+    _sig(lib, "filt_serve_is_done", c_int, c_void_p, c_int)
+    _sig(lib, "filt_serve_take", c_int, c_void_p, c_int, c_void_p, c_void_p, POINTER(c_double))
     _sig(lib, "filt_synchronize", c_int)
     _sig(lib, "filt_set_stream", c_int, c_void_p)
     _sig(lib, "filt_get_stream", c_void_p)

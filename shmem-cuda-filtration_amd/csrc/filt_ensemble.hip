@@ -88,6 +88,9 @@ int sync_tables(filt_ensemble* e)
         bg.insert(bg.end(), (size_t)nb, (int)q);
     }
     e->total_blocks = first;
+    // launches already enqueued read the current tables: let them finish
+    // before the tables are rewritten or reallocated
+    FILT_HIP_TRY(hipStreamSynchronize(stream()));
     if (gs.size() > e->d_grids_cap) {
         if (e->d_grids) (void)hipFree(e->d_grids);
         e->d_grids = nullptr;

@@ -124,7 +124,7 @@ def c3_workload(nx=512, ny=512, nz=256, seed=3033):
 C4_EDGES = (48, 64, 96, 128, 160)
 
 
-def ensemble_member(rng, edges=C4_EDGES, corr=8.0, seed=0, shape=None):
+def ensemble_member(rng, edges=C4_EDGES, corr=8.0, seed=0, shape=None, threads=0):
     """One realization of the batched-ensemble / serving configurations: each
     axis drawn from `edges` (or the given shape), sigma ~ U[0.5, 2],
     correlation `corr` cells, porosity U[0.35, 0.45], a 4x4 source patch at a
@@ -139,7 +139,7 @@ def ensemble_member(rng, edges=C4_EDGES, corr=8.0, seed=0, shape=None):
     sigma = float(rng.uniform(0.5, 2.0))
     clip = 2.0
     while True:
-        perm, poro = gen_fields(p, seed, sigma=sigma, corr=corr, clip=clip)
+        perm, poro = gen_fields(p, seed, sigma=sigma, corr=corr, clip=clip, threads=threads)
         r = stability(p, perm, poro)
         if r <= 1.5 or clip <= 0.5:
             break
