@@ -250,6 +250,8 @@ int filt_save_exchange_data(double* dev_array, double* dev_buffer,
  * and roS_calculation (kernel 0): tx*ty threads per block (64x4, 32x8,
  * 16x16 or 64x8), kz planes per z-chunk, order 0 = XCD-grouped tile
  * numbering, 1 = hardware order.  Used by grids with every extent >= 3.
+ * tx = 0 restores the built-in default (the fastest configuration on a
+ * 256^3 grid: roS 64x4, kz 16, order 1; u 64x8, kz 64, order 1).
  * The setting is process-wide; results do not depend on it. */
 int filt_set_tile_config(int kernel, int tx, int ty, int kz, int order);
 /* The configuration used for an nx*ny*nz grid (tuned or default). */

@@ -199,8 +199,15 @@ class RefCPU:
         lib.ref_defines_file.restype = c_char_p
         lib.ref_drive.argtypes = [C, c_int, c_void_p, c_void_p, c_size_t]
         lib.ref_drive.restype = c_int
+        lib.ref_exchange.argtypes = [c_int, c_void_p, c_void_p, c_int, c_int, c_int, C]
         assert lib.ref_sizeof_consts() == ctypes.sizeof(RefConsts)
         assert lib.ref_sizeof_ptr_arrays() == ctypes.sizeof(RefArrays)
+
+    def exchange(self, which, array, buffer, localNx, rank, size, p):
+        """The reference backend's load_exchange_data (which = 0) or
+        save_exchange_data (which = 1) on host arrays."""
+        self.lib.ref_exchange(which, array.ctypes.data, buffer.ctypes.data, localNx, rank, size,
+                              ctypes.byref(to_ref_consts(p)))
 
     def drive(self, p, nsteps):
         """The reference driver end to end (Initialize, time_step_function x

@@ -95,6 +95,18 @@ int ref_read_defines(const char* workdir, consts* out)
 
 const char* ref_defines_file(void) { return DEFINES_FILE; }
 
+// load_exchange_data / save_exchange_data of the linked backend on host
+// arrays (cpu.cpp:436-465 for the CPU backend): which = 0 packs array into
+// buffer, which = 1 unpacks buffer into array.
+void ref_exchange(int which, double* array, double* buffer, int localNx, int rank, int size,
+                  const consts* def)
+{
+    if (which == 0)
+        load_exchange_data(array, array, buffer, buffer, localNx, 0, 0, rank, size, *def);
+    else
+        save_exchange_data(array, array, buffer, buffer, localNx, 0, 0, rank, size, *def);
+}
+
 extern double* DevBuffer;  // main.cpp:7
 
 // The reference driver end to end with whichever backend this library links:

@@ -4,14 +4,21 @@
 // 4 GiB each.
 //
 // Each block owns a TX x TY tile of columns and a chunk of kz planes.  Per
-// plane it writes its columns' values into an LDS tile with a one-cell halo
-// (halo cells read from global memory, where neighbouring tiles on the same
-// XCD have just loaded them into L2), synchronises, and evaluates the cell
-// from LDS (x, y neighbours) and registers (its own column's k-1, k, k+1).
-// Every HBM element of the stencil arrays is thus read once per chunk.
-// Addresses are 32-bit byte offsets from the (uniform) array bases, which
-// keeps them in one VGPR each.  The arithmetic is expression for expression
-// the one of filt_stages.hip, so both paths give identical bits.
+// plane it writes its columns' values and a one-cell halo ring (read from
+// global memory, where neighbouring tiles on the same XCD have just loaded
+// them into L2) into one of two LDS tiles, synchronises once, and evaluates
+// the cell from LDS (x, y neighbours) and registers (its own column's k-1,
+// k, k+1).  The loads are software-pipelined: while plane k is evaluated,
+// the halo ring and the point operands of plane k+1 and the own column of
+// plane k+2 are in flight, so no load issued in an iteration is waited for
+// in the same iteration; the two LDS tiles alternate, so one barrier per
+// plane orders both the writes before the reads and the reads of plane k
+// before the overwrite at plane k+2.  Halo cells are spread one per thread
+// (ring cell h to thread h).  Every HBM element of the stencil arrays is
+// read once per chunk.  Addresses are 32-bit byte offsets from the
+// (uniform) array bases, which keeps them in one VGPR each.  The arithmetic
+// is expression for expression the one of filt_stages.hip, so both paths
+// give identical bits.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -51,6 +58,50 @@ __device__ __forceinline__ int ldi(const int* base, unsigned off4)
     return *(const int*)((const char*)base + off4);
 }
 
+// The halo ring of a TX x TY tile without its corners (a cross stencil never
+// reads them): 2*TX + 2*TY cells, cell h at LDS row ly, column lx and at
+// (di, dj) from the tile origin.  Rows y-1 and y+TY first, then columns x-1
+// and x+TX.
+template <int TX, int TY>
+struct Ring {
+    static constexpr int N = 2 * TX + 2 * TY;
+    __device__ static inline void cell(int h, int& ly, int& lx, int& di, int& dj)
+    {
+        if (h < TX) {
+            ly = 0, lx = h + 1, di = h, dj = -1;
+        } else if (h < 2 * TX) {
+            ly = TY + 1, lx = h - TX + 1, di = h - TX, dj = TY;
+        } else if (h < 2 * TX + TY) {
+            ly = h - 2 * TX + 1, lx = 0, di = -1, dj = ly - 1;
+        } else {
+            ly = h - 2 * TX - TY + 1, lx = TX + 1, di = TX, dj = ly - 1;
+        }
+    }
+};
+
+// Thread (tx, ty)'s ring cell, when it has one inside the grid: its LDS
+// slot (array 0) and its column's byte offset.
+template <int TX, int TY>
+__device__ inline bool ring_slot(int tx, int ty, int i0, int j0, int nx, int ny, unsigned sy,
+                                 int& slot, unsigned& hcol)
+{
+    using R = Ring<TX, TY>;
+    static_assert(R::N <= TX * TY, "one ring cell per thread at most");
+    const int h = tx + TX * ty;
+    slot = 0;
+    hcol = 0;
+    if (h >= R::N)
+        return false;
+    int ly, lx, di, dj;
+    R::cell(h, ly, lx, di, dj);
+    const int gi = i0 + di, gj = j0 + dj;
+    if (gi < 0 || gi >= nx || gj < 0 || gj >= ny)
+        return false;
+    slot = Lds<TX, TY>::at(0, ly, lx);
+    hcol = 8u * gi + sy * gj;
+    return true;
+}
+
 // ---------------------------------------------------------------- roS ----
 // cpu.cpp:218-291 for interior cells; arrays P1, P2, Xi1, Xi2, ro1, ro2 are
 // the stencil operands (index 0..5 below).
@@ -59,11 +110,12 @@ __global__ __launch_bounds__(TX* TY) void k_roS_zm(filt_fields f, filt_params p,
                                                    TileMap map, int kz)
 {
     using L = Lds<TX, TY>;
-    __shared__ double sh[6 * L::SN];
+    __shared__ double shb[2][6 * L::SN];
     int bx, by, bz;
     map.decode(blockIdx.x, bx, by, bz);
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const int i = bx * TX + tx, j = by * TY + ty;
+    const int i0 = bx * TX, j0 = by * TY;
+    const int i = i0 + tx, j = j0 + ty;
     const int ny = p.Ny, nz = p.Nz;
     const bool valid = i < nx && j < ny;
     const bool inner = i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2;
@@ -73,42 +125,63 @@ __global__ __launch_bounds__(TX* TY) void k_roS_zm(filt_fields f, filt_params p,
         return;  // uniform over the block
     const double* const A[6] = {f.P1, f.P2, f.Xi1, f.Xi2, f.ro1, f.ro2};
     const unsigned col = 8u * i + sy * j;
-    double zm[6], zc[6], zp[6];
+    int hslot;
+    unsigned hcol;
+    const bool hv = ring_slot<TX, TY>(tx, ty, i0, j0, nx, ny, sy, hslot, hcol);
+    // Porosity per cell, else by media index (one medium: index 0).
+    const bool by_media = !f.poro && f.media && p.n_media > 1;
+    // Pipeline registers: own column at k-1, k, k+1 and (in flight) k+2;
+    // ring cell and point operands of the next plane.
+    double zm[6] = {}, zc[6] = {}, zp[6] = {}, zq[6] = {}, hn[6] = {};
+    double Sc = 0, Sn = 0, phic = 0, phin = 0;
+    int mdc = 0, mdn = 0;
     if (valid) {
+        const unsigned c = col + kb * sz;
 #pragma unroll
         for (int a = 0; a < 6; a++) {
-            zm[a] = ld(A[a], col + (kb - 1) * sz);
-            zc[a] = ld(A[a], col + kb * sz);
+            zm[a] = ld(A[a], c - sz);
+            zc[a] = ld(A[a], c);
+            zp[a] = ld(A[a], c + sz);  // kb + 1 <= ke <= nz - 1
         }
+        Sc = ld(f.S2, c);
+        if (f.poro)
+            phic = ld(f.poro, c);
+        else if (by_media)
+            mdc = ldi(f.media, c >> 1);
     }
+    if (hv)
+#pragma unroll
+        for (int a = 0; a < 6; a++) hn[a] = ld(A[a], hcol + kb * sz);
     const int cx = tx + 1, cy = ty + 1;
     const double g = p.g_const;
     for (int k = kb; k < ke; k++) {
+        double* const sh = shb[(k - kb) & 1];
         const unsigned c = col + k * sz;
-        if (valid) {
+        if (valid)
 #pragma unroll
-            for (int a = 0; a < 6; a++) {
-                zp[a] = ld(A[a], c + sz);
-                sh[L::at(a, cy, cx)] = zc[a];
+            for (int a = 0; a < 6; a++) sh[L::at(a, cy, cx)] = zc[a];
+        if (hv)
+#pragma unroll
+            for (int a = 0; a < 6; a++) sh[hslot + a * L::SN] = hn[a];
+        if (k + 1 < ke) {  // next plane is evaluated: fetch its operands
+            if (valid) {
+#pragma unroll
+                for (int a = 0; a < 6; a++) zq[a] = ld(A[a], c + 2 * sz);  // k+2 <= ke
+                Sn = ld(f.S2, c + sz);
+                if (f.poro)
+                    phin = ld(f.poro, c + sz);
+                else if (by_media)
+                    mdn = ldi(f.media, (c + sz) >> 1);
             }
-            if (ty == 0 && j >= 1)
+            if (hv)
 #pragma unroll
-                for (int a = 0; a < 6; a++) sh[L::at(a, 0, cx)] = ld(A[a], c - sy);
-            if (ty == TY - 1 && j + 1 < ny)
-#pragma unroll
-                for (int a = 0; a < 6; a++) sh[L::at(a, TY + 1, cx)] = ld(A[a], c + sy);
-            if (tx == 0 && i >= 1)
-#pragma unroll
-                for (int a = 0; a < 6; a++) sh[L::at(a, cy, 0)] = ld(A[a], c - 8u);
-            if (tx == TX - 1 && i + 1 < nx)
-#pragma unroll
-                for (int a = 0; a < 6; a++) sh[L::at(a, cy, TX + 1)] = ld(A[a], c + 8u);
+                for (int a = 0; a < 6; a++) hn[a] = ld(A[a], hcol + (k + 1) * sz);
         }
         __syncthreads();
         if (valid && inner) {
             const double p1 = zc[0], p2 = zc[1];
             const double X1c = zc[2], X2c = zc[3], r1c = zc[4], r2c = zc[5];
-            const double S = ld(f.S2, c);
+            const double S = Sc;
             const double roS1 = r1c * (1 - S);
             const double roS2 = r2c * S;
             double a1, a2;
@@ -138,19 +211,21 @@ __global__ __launch_bounds__(TX* TY) void k_roS_zm(filt_fields f, filt_params p,
             const double g2 = upwind(a2, a1, X2c, r2c, sh[L::at(3, ym, cx)], sh[L::at(5, ym, cx)],
                                      sh[L::at(3, yp, cx)], sh[L::at(5, yp, cx)], p.h2);
 
-            const int md = (f.media && p.n_media > 1) ? ldi(f.media, c >> 1) : 0;
-            const double phi = f.poro ? ld(f.poro, c) : p.m[md];
+            const double phi = f.poro ? phic : p.m[mdc];
             st(f.roS1_old, c, roS1);
             st(f.roS2_old, c, roS2);
             st(f.roS1, c, roS1 - (p.dt / phi) * (f1 + f2 + f3));
             st(f.roS2, c, roS2 - (p.dt / phi) * (g1 + g2 + g3));
         }
-        __syncthreads();
 #pragma unroll
         for (int a = 0; a < 6; a++) {
             zm[a] = zc[a];
             zc[a] = zp[a];
+            zp[a] = zq[a];
         }
+        Sc = Sn;
+        phic = phin;
+        mdc = mdn;
     }
 }
 
@@ -161,11 +236,12 @@ __global__ __launch_bounds__(TX* TY) void k_u_zm(filt_fields f, filt_params p, i
                                                  int size, TileMap map, int kz)
 {
     using L = Lds<TX, TY>;
-    __shared__ double sh[2 * L::SN];
+    __shared__ double shb[2][2 * L::SN];
     int bx, by, bz;
     map.decode(blockIdx.x, bx, by, bz);
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const int i = bx * TX + tx, j = by * TY + ty;
+    const int i0 = bx * TX, j0 = by * TY;
+    const int i = i0 + tx, j = j0 + ty;
     const int ny = p.Ny, nz = p.Nz;
     const bool valid = i < nx && j < ny;
     const bool active = valid && is_active(i, nx, rank, size);
@@ -174,46 +250,67 @@ __global__ __launch_bounds__(TX* TY) void k_u_zm(filt_fields f, filt_params p, i
     if (kb >= ke)
         return;
     const unsigned col = 8u * i + sy * j;
-    double zm0 = 0, zm1 = 0, zc0 = 0, zc1 = 0, zp0 = 0, zp1 = 0;
+    int hslot;
+    unsigned hcol;
+    const bool hv = ring_slot<TX, TY>(tx, ty, i0, j0, nx, ny, sy, hslot, hcol);
+    // Pipeline registers: P1, P2 of the own column at k-1, k, k+1 and (in
+    // flight) k+2; the ring cell and the point operands of the next plane.
+    double zm0 = 0, zm1 = 0, zc0 = 0, zc1 = 0, zp0 = 0, zp1 = 0, zq0 = 0, zq1 = 0;
+    double hn0 = 0, hn1 = 0;
+    double X1c = 0, X2c = 0, r1c = 0, r2c = 0, X1n = 0, X2n = 0, r1n = 0, r2n = 0;
     if (valid) {
+        const unsigned c = col + kb * sz;
         if (kb >= 1) {
-            zm0 = ld(f.P1, col + (kb - 1) * sz);
-            zm1 = ld(f.P2, col + (kb - 1) * sz);
+            zm0 = ld(f.P1, c - sz);
+            zm1 = ld(f.P2, c - sz);
         }
-        zc0 = ld(f.P1, col + kb * sz);
-        zc1 = ld(f.P2, col + kb * sz);
+        zc0 = ld(f.P1, c);
+        zc1 = ld(f.P2, c);
+        if (kb + 1 < nz) {
+            zp0 = ld(f.P1, c + sz);
+            zp1 = ld(f.P2, c + sz);
+        }
+        X1c = ld(f.Xi1, c);
+        X2c = ld(f.Xi2, c);
+        r1c = ld(f.ro1, c);
+        r2c = ld(f.ro2, c);
+    }
+    if (hv) {
+        hn0 = ld(f.P1, hcol + kb * sz);
+        hn1 = ld(f.P2, hcol + kb * sz);
     }
     const int cx = tx + 1, cy = ty + 1;
     const double g = p.g_const;
     for (int k = kb; k < ke; k++) {
+        double* const sh = shb[(k - kb) & 1];
         const unsigned c = col + k * sz;
         if (valid) {
-            if (k + 1 < nz) {
-                zp0 = ld(f.P1, c + sz);
-                zp1 = ld(f.P2, c + sz);
-            }
             sh[L::at(0, cy, cx)] = zc0;
             sh[L::at(1, cy, cx)] = zc1;
-            if (ty == 0 && j >= 1) {
-                sh[L::at(0, 0, cx)] = ld(f.P1, c - sy);
-                sh[L::at(1, 0, cx)] = ld(f.P2, c - sy);
+        }
+        if (hv) {
+            sh[hslot] = hn0;
+            sh[hslot + L::SN] = hn1;
+        }
+        if (k + 1 < ke) {  // next plane is evaluated: fetch its operands
+            if (valid) {
+                if (k + 2 < nz) {
+                    zq0 = ld(f.P1, c + 2 * sz);
+                    zq1 = ld(f.P2, c + 2 * sz);
+                }
+                X1n = ld(f.Xi1, c + sz);
+                X2n = ld(f.Xi2, c + sz);
+                r1n = ld(f.ro1, c + sz);
+                r2n = ld(f.ro2, c + sz);
             }
-            if (ty == TY - 1 && j + 1 < ny) {
-                sh[L::at(0, TY + 1, cx)] = ld(f.P1, c + sy);
-                sh[L::at(1, TY + 1, cx)] = ld(f.P2, c + sy);
-            }
-            if (tx == 0 && i >= 1) {
-                sh[L::at(0, cy, 0)] = ld(f.P1, c - 8u);
-                sh[L::at(1, cy, 0)] = ld(f.P2, c - 8u);
-            }
-            if (tx == TX - 1 && i + 1 < nx) {
-                sh[L::at(0, cy, TX + 1)] = ld(f.P1, c + 8u);
-                sh[L::at(1, cy, TX + 1)] = ld(f.P2, c + 8u);
+            if (hv) {
+                hn0 = ld(f.P1, hcol + (k + 1) * sz);
+                hn1 = ld(f.P2, hcol + (k + 1) * sz);
             }
         }
         __syncthreads();
         if (active) {
-            const double X1 = ld(f.Xi1, c), X2 = ld(f.Xi2, c);
+            const double X1 = X1c, X2 = X2c;
             const double P1c = zc0, P2c = zc1;
             double u1, u2;
             // x (p.NX > 2 holds: this kernel runs for NX >= 3 only)
@@ -230,7 +327,7 @@ __global__ __launch_bounds__(TX* TY) void k_u_zm(filt_fields f, filt_params p, i
             st(f.u1x, c, u1);
             st(f.u2x, c, u2);
             // y
-            const double r1 = ld(f.ro1, c), r2 = ld(f.ro2, c);
+            const double r1 = r1c, r2 = r2c;
             if (j == 0) {
                 u1 = X1 * ((sh[L::at(0, cy + 1, cx)] - P1c) / p.h2 - r1 * g);
                 u2 = X2 * ((sh[L::at(1, cy + 1, cx)] - P2c) / p.h2 - r2 * g);
@@ -257,11 +354,16 @@ __global__ __launch_bounds__(TX* TY) void k_u_zm(filt_fields f, filt_params p, i
             st(f.u1z, c, u1);
             st(f.u2z, c, u2);
         }
-        __syncthreads();
         zm0 = zc0;
         zm1 = zc1;
         zc0 = zp0;
         zc1 = zp1;
+        zp0 = zq0;
+        zp1 = zq1;
+        X1c = X1n;
+        X2c = X2n;
+        r1c = r1n;
+        r2c = r2n;
     }
 }
 
@@ -297,8 +399,12 @@ TileMap make_map(int nx, int ny, int nz, const TileConfig& t)
     return m;
 }
 
-TileConfig g_roS_cfg = {64, 4, 32, 0};
-TileConfig g_u_cfg = {64, 4, 32, 0};
+// Defaults: the fastest configurations of a sweep over every shape, kz in
+// {16, 64} and both orders on a 256^3 grid on MI355X.
+constexpr TileConfig kRoSDefault = {64, 4, 16, 1};
+constexpr TileConfig kUDefault = {64, 8, 64, 1};
+TileConfig g_roS_cfg = kRoSDefault;
+TileConfig g_u_cfg = kUDefault;
 
 // Autotuned configurations by grid shape (filt_autotune_stencil).
 std::mutex g_tune_mu;
@@ -355,6 +461,8 @@ extern "C" int filt_set_tile_config(int kernel, int tx, int ty, int kz, int orde
     filt::TileConfig t = {tx, ty, kz, order};
     if (kernel != 0 && kernel != 1)
         return filt::set_error(FILT_ERR_INVALID, "kernel must be 0 (roS) or 1 (u)");
+    if (tx == 0)  // back to the built-in default
+        t = kernel == 0 ? filt::kRoSDefault : filt::kUDefault;
     if (!filt::valid_cfg(t, kernel == 0))
         return filt::set_error(FILT_ERR_INVALID, "unsupported tile %dx%d kz=%d order=%d", tx, ty,
                                kz, order);

@@ -168,7 +168,7 @@ def test_tiled_stencils_every_config_bitwise(hip, orc, tile, kz, order):
                 assert np.array_equal(out[n], want[n]), (n, tile, kz, order)
     finally:
         for kern in (0, 1):
-            lib.filt_set_tile_config(kern, 64, 4, 32, 0)
+            lib.filt_set_tile_config(kern, 0, 0, 0, 0)
     assert ctypes is not None
 
 
@@ -204,7 +204,8 @@ def test_autotuner_picks_a_config_and_keeps_results(hip, orc):
         assert np.array_equal(out2["roS1"], want["roS1"])
     finally:
         d.free()
-        lib.filt_set_tile_config(0, 64, 4, 32, 0)
+        lib.filt_set_tile_config(0, 0, 0, 0, 0)
+        lib.filt_set_tile_config(1, 0, 0, 0, 0)
 
 
 @pytest.mark.parametrize("dims,source,steps", [((15, 15, 15), 3, 40), ((24, 20, 16), 2, 60)])
@@ -225,3 +226,35 @@ def test_reference_driver_links_and_runs_on_hip_backend(dims, source, steps):
     assert max_rel(Pg, Pc) <= 1e-10
     assert max_rel(Sg, Sc) <= 1e-10
     assert np.any(Sc[:, 1, :] > 0)  # NAPL has entered the domain
+
+
+@pytest.mark.parametrize("rank,size", [(0, 2), (1, 3), (2, 3)])
+def test_exchange_pack_unpack_match_reference(hip, ref, rank, size):
+    """load_exchange_data / save_exchange_data (gpu.cu:613-653): the HIP
+    pack / unpack kernels give the reference CPU backend's buffers and halo
+    planes (cpu.cpp:436-465)."""
+    import ctypes
+    lib = hip.lib
+    p = ref_params(19, 7, 6)
+    lnx, _ = ref.N_to_local(size, rank, p)
+    rng = np.random.default_rng(rank * 10 + size)
+    a = rng.uniform(0, 1, (p.Nz, p.Ny, lnx))
+    buf_ref = np.zeros(2 * p.Ny * p.Nz)
+    ref.exchange(0, a.copy(), buf_ref, lnx, rank, size, p)
+    d = hip.grid(p, lnx)
+    try:
+        h = np.ascontiguousarray(a)
+        assert lib.filt_memcpy(d.f.P1, h.ctypes.data, h.nbytes, 0) == 0
+        hb = np.zeros(2 * p.Ny * p.Nz)
+        assert lib.filt_load_exchange_data(d.f.P1, d.buf, hb.ctypes.data, lnx,
+                                           ctypes.byref(d.params)) == 0
+        assert np.array_equal(hb, buf_ref)
+        incoming = rng.uniform(5, 6, 2 * p.Ny * p.Nz)
+        want = a.copy()
+        ref.exchange(1, want, incoming.copy(), lnx, rank, size, p)
+        assert lib.filt_save_exchange_data(d.f.P1, d.buf, incoming.ctypes.data, lnx, rank, size,
+                                           ctypes.byref(d.params)) == 0
+        got = d.get("P1")
+        assert np.array_equal(got, want)
+    finally:
+        d.free()
