@@ -389,7 +389,7 @@ def main():
     dom = max((s for s in PLAIN_STAGES if PLAIN_BYTES[s]), key=lambda s: stage_ms[s])
     dom_bytes = PLAIN_BYTES[dom] * cells
     achieved = dom_bytes / (stage_ms[dom] * 1e-3)
-    tag = f"c2_{args.n}"
+    tag = f"plain_{args.n}cube"
     traffic = load_traffic(tag, KERNEL_NAMES[dom])
     step_bytes = sum(PLAIN_BYTES.values()) * cells
     line = {
