@@ -70,30 +70,64 @@ __device__ inline double upwind(double a2, double a1, double Xc, double rc, doub
             (a2 - fabs(a2)) / 2.0 * (-1) * Xp * rp) / h;
 }
 
+// Bit-pattern equality (also for NaN, and telling -0 from +0).
+__device__ __forceinline__ bool same_bits(double a, double b)
+{
+    return __double_as_longlong(a) == __double_as_longlong(b);
+}
+
 // Newton iterations of cpu.cpp:293-320 on one cell, in registers.
 // With P_d == 0 the capillary terms P_k = P_d*S_e^(-1/lam) and
 // PkS ~ P_d*S_e^(-1/lam-1) are exactly zero whenever those powers are finite,
 // which holds for S_e >= 2^-10 and lam >= 0.1 (|exponent| <= 11); the pows
 // are then skipped.  Elsewhere they are evaluated as the reference does, so
 // inf*0 = NaN propagates identically.
+//
+// That test needs no division: with d = 1 - S_wr > 0 and n = 1 - S - S_wr,
+// RN(n/d) >= 2^-10 exactly when n >= d*2^-10.  (d*2^-10 is exact; the
+// values that round to 2^-10 or above are those >= 2^-10*(1 - 2^-54), the
+// midpoint to the next double below, and no double n lies strictly between
+// pred(d*2^-10) and d*2^-10 while n/d >= 2^-10*(1 - 2^-54).)
+//
+// An iteration is a function of (P, S) alone.  Once its output repeats its
+// input bit for bit (a fixed point) every later iteration returns it too;
+// once it repeats the state of two iterations back (a 2-cycle between two
+// neighbouring doubles, which about 2% of cells settle into) the states
+// alternate.  Stopping there and taking the state the remaining count
+// lands on gives the full iteration's result bit for bit.
+//
+// kLean = true keeps the division and only the fixed-point exit: six fewer
+// live registers, for kernels at the edge of an occupancy step.
+template <bool kLean = false>
 __device__ inline void newton_cell(double& P, double& S, double roS1, double roS2, double lam,
                                    double swr, double pd, const filt_params& p)
 {
     const double ro01 = p.ro01, ro02 = p.ro02, b1 = p.beta1, b2 = p.beta2, Pa = p.P_atm;
     const bool no_cap = (pd == 0.0) && (lam >= 0.1);
-    for (int w = 1; w <= p.newton_iterations; w++) {
-        const double S_e = (1 - S - swr) / (1 - swr);
+    const double d = 1 - swr;
+    const bool fast = !kLean && no_cap && d > 0.0;
+    const double d_t = d * 0.0009765625;
+    const int n = p.newton_iterations;
+    double Pq = P, Sq = S;  // the state two iterations back
+    for (int w = 1; w <= n; w++) {
+        const double num = 1 - S - swr;
         double P_k, PkS;
-        if (no_cap && S_e >= 0.0009765625) {
+        if (fast && num >= d_t) {
             // P_d*S_e^(-1/lam) and S_e^(-1/lam-1)*P_d/(lam*(1-swr)) with finite
             // powers and P_d = 0: both are zero, and a zero PkS enters F2S
             // only as S*PkS added to a nonzero sum.
             P_k = 0.0;
             PkS = 0.0;
         } else {
-            P_k = pd * pow(S_e, (-1.) / lam);
-            const double AAA = pow(S_e, ((-1.) / lam) - 1.);
-            PkS = AAA * pd / (lam * (1 - swr));
+            const double S_e = num / d;
+            if (no_cap && S_e >= 0.0009765625) {
+                P_k = 0.0;
+                PkS = 0.0;
+            } else {
+                P_k = pd * pow(S_e, (-1.) / lam);
+                const double AAA = pow(S_e, ((-1.) / lam) - 1.);
+                PkS = AAA * pd / (lam * (1 - swr));
+            }
         }
         const double F1 = ro01 * (1. + b1 * (P - Pa)) * (1. - S) - roS1;
         const double F2 = ro02 * (1. + b2 * (P + P_k - Pa)) * S - roS2;
@@ -104,15 +138,22 @@ __device__ inline void newton_cell(double& P, double& S, double roS1, double roS
         const double det = F1P * F2S - F1S * F2P;
         const double Pn = P - (1 / det) * (F2S * F1 - F1S * F2);
         const double Sn = S - (1 / det) * (F1P * F2 - F2P * F1);
-        // An iteration is a function of (P, S) alone: once it returns its
-        // input, the remaining iterations would too, so stopping there gives
-        // the reference's result bit for bit (NaN never compares equal and
-        // runs all iterations, as the reference does).
-        const bool fixed = (Pn == P) && (Sn == S);
+        if (same_bits(Pn, P) && same_bits(Sn, S))
+            break;  // fixed point: (P, S) is the result
+        if (!kLean && w >= 2 && same_bits(Pn, Pq) && same_bits(Sn, Sq)) {
+            // 2-cycle: x_{w+1} = x_{w-1}, x_{w+2} = x_w, ...
+            if (((n - w) & 1) == 0) {
+                P = Pn;
+                S = Sn;
+            }
+            break;
+        }
+        if (!kLean) {
+            Pq = P;
+            Sq = S;
+        }
         P = Pn;
         S = Sn;
-        if (fixed)
-            break;
     }
 }
 

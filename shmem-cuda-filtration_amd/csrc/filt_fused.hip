@@ -123,7 +123,7 @@ __device__ inline void fp64_update(const FusedGrid& G, long long c, const C1& cc
     const double A2 = cc.r2 * S - (p.dt / phi) * (g1 + g2 + g3);
     nP = cc.P;
     nS = S;
-    newton_cell(nP, nS, A1, A2, p.lambda[md], p.S_wr[md], p.P_d[md], p);
+    newton_cell<true>(nP, nS, A1, A2, p.lambda[md], p.S_wr[md], p.P_d[md], p);
 }
 
 // ----------------------------------------------------------- mixed form --
@@ -241,11 +241,12 @@ __device__ inline void mixed_update(const FusedGrid& G, long long c, const M1& c
         r = fma(fma(-det, r, 1.0), r, r);
         const double Pn = P - r * (F2S * F1v - F1S * F2v);
         const double Sn = Sv - r * (F1P * F2v - F2P * F1v);
-        const bool fixed = (Pn == P) && (Sn == Sv);  // later iterations repeat it
+        // a fixed point ends the iteration exactly (newton_cell); the 2-cycle
+        // exit costs this kernel an occupancy step (measured: slower)
+        if (same_bits(Pn, P) && same_bits(Sn, Sv))
+            break;
         P = Pn;
         Sv = Sn;
-        if (fixed)
-            break;
     }
     nP = P;
     nS = Sv;
