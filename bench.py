@@ -91,6 +91,47 @@ def load_traffic(workload_tag, kernel):
         return json.load(fh).get("bytes_per_launch")
 
 
+def mixed_errors(P64, S64, m64, Pm, Sm, mm):
+    """Mixed-form errors against the FP64 form after the same steps."""
+    import numpy as np
+    return {
+        "S2_linf": float(np.max(np.abs(Sm - S64))),
+        "P1_rel_linf": float(np.max(np.abs(Pm - P64)) / np.max(np.abs(P64))),
+        "water_mass_rel": abs(mm[0] - m64[0]) / m64[0],
+        "napl_mass_rel": abs(mm[1] - m64[1]) / m64[1] if m64[1] > 0 else 0.0,
+        "total_mass_rel": abs(mm[0] + mm[1] - m64[0] - m64[1]) / (m64[0] + m64[1]),
+        "reference": "fused FP64 (bit-identical to the plain path) after the same steps",
+    }
+
+
+def run_slab(args, nx, ny, nz):
+    """One large grid, un-batched: a 512 x 512 x 256 slab (sigma 1.5,
+    correlation 16 cells, ln K clipped at +-2 sigma), args.slab_steps steps
+    through the plain five-stage path, the fused FP64 form and the mixed
+    form, so that the fusion / FP32 gain shows apart from batching."""
+    import types
+    W = PKG.workloads
+    t0 = time.perf_counter()
+    p, perm, poro = W.c3_workload(nx, ny, nz)
+    gen_s = time.perf_counter() - t0
+    sub = types.SimpleNamespace(steps=args.slab_steps, warmup=3, autotune=False)
+    p, wall, stage_ms, r, finite, tuned = run_plain(sub, p, perm, poro)
+    cells = nx * ny * nz
+    plain = cells * sub.steps / wall
+    out = {"grid": [nx, ny, nz], "steps": sub.steps, "field_generation_s": gen_s,
+           "stability_r": r, "state_finite": finite,
+           "plain": {"value": plain, "ms_per_step": wall * 1e3 / sub.steps,
+                     "stages_ms": stage_ms, "stencil_tiles": tuned}}
+    f64, (P64, S64, m64) = run_fused(sub, p, perm, poro, 0)
+    mix, (Pm, Sm, mm) = run_fused(sub, p, perm, poro, 1)
+    mix["error_vs_fp64"] = mixed_errors(P64, S64, m64, Pm, Sm, mm)
+    out["fused_fp64"] = f64
+    out["mixed"] = mix
+    out["speedup_fused_fp64_vs_plain"] = f64["value"] / plain
+    out["speedup_mixed_vs_plain"] = mix["value"] / plain
+    return out
+
+
 def run_plain(args, p, perm, poro):
     from importlib import import_module
     hiprt = import_module(PKG.__name__ + ".hiprt")
@@ -144,11 +185,10 @@ def run_plain(args, p, perm, poro):
     wall = time.perf_counter() - t0
     stage_ms = {name: sum(ev[s][q].elapsed_ms(ev[s][q + 1]) for s in range(K)) / K
                 for q, name in enumerate(PLAIN_STAGES)}
-    out = g.copy()
-    d.download(out, ["P1", "S2"])
+    P1, S2 = d.get("P1"), d.get("S2")
     d.free()
     import numpy as np
-    finite = bool(np.isfinite(out["P1"]).all() and np.isfinite(out["S2"]).all())
+    finite = bool(np.isfinite(P1).all() and np.isfinite(S2).all())
     return p, wall, stage_ms, r, finite, tuned
 
 
@@ -345,6 +385,8 @@ def main():
     ap.add_argument("--c4", type=int, default=96,
                     help="realizations of the batched-ensemble leg (0: skip)")
     ap.add_argument("--c4-steps", type=int, default=300)
+    ap.add_argument("--slab-steps", type=int, default=100,
+                    help="steps of the 512x512x256 and 256x512x512 slab legs (0: skip)")
     ap.add_argument("--serve-duration", type=float, default=20.0,
                     help="arrival window of the serving leg in seconds (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0)
@@ -371,15 +413,11 @@ def main():
         import numpy as np
         f64, (P64, S64, m64) = run_fused(args, p, perm, poro, 0)
         mix, (Pm, Sm, mm) = run_fused(args, p, perm, poro, 1)
-        mix["error_vs_fp64"] = {
-            "S2_linf": float(np.max(np.abs(Sm - S64))),
-            "P1_rel_linf": float(np.max(np.abs(Pm - P64)) / np.max(np.abs(P64))),
-            "water_mass_rel": abs(mm[0] - m64[0]) / m64[0],
-            "napl_mass_rel": abs(mm[1] - m64[1]) / m64[1],
-            "total_mass_rel": abs(mm[0] + mm[1] - m64[0] - m64[1]) / (m64[0] + m64[1]),
-            "reference": "fused FP64 (bit-identical to the plain path) after the same steps",
-        }
+        mix["error_vs_fp64"] = mixed_errors(P64, S64, m64, Pm, Sm, mm)
         forms = {"fused_fp64": f64, "mixed": mix}
+        if args.slab_steps > 0:
+            for shape in ((512, 512, 256), (256, 512, 512)):
+                forms["slab_%dx%dx%d" % shape] = run_slab(args, *shape)
         if args.c4 > 0:
             forms["batched_ensemble"] = run_c4(args)
         if args.serve_duration > 0:

@@ -164,3 +164,15 @@ def test_workload_initial_state(pkg):
     assert g["S2"].sum() == pytest.approx(16 * 0.5)
     assert np.all(g["S2"][14:18, 0, 14:18] == 0.5)
     assert g["P1"][0, 3, 0] == 100000 + 3 * 1000 * 9.8 * p.h2
+
+
+@pytest.mark.parametrize("shape", [(512, 512, 256), (256, 512, 512)])
+def test_slab_workload_passes_the_guard(pkg, shape):
+    """The 512x512x256 slab (sigma 1.5, correlation 16) and its transposed
+    orientation: with ln K clipped at +-2 sigma the guard stays below 2 (at
+    +-2.5 sigma it would read ~2.6)."""
+    W = pkg.workloads
+    p, perm, poro = W.c3_workload(*shape)
+    assert perm.shape == shape[::-1]
+    r = W.stability(p, perm, poro)
+    assert 0.5 < r < 1.5
