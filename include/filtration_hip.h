@@ -359,6 +359,69 @@ int filt_serve_is_done(const filt_server* s, int id);
  * the server forgets it afterwards. */
 int filt_serve_take(filt_server* s, int id, double* P1, double* S2, double* mass2);
 
+/* ------------------------------------------- checkpoint, resume, plots -- */
+
+/* save (main.cpp:485-537): append this rank's block
+ *   [int j][P1][S2][x][y][z][roS1_old][roS2_old][media:int32]
+ * (localNx*Ny*Nz values each, native byte order) to path, creating its
+ * folder; truncate != 0 creates the file first (rank 0 of the reference).
+ * Ranks call in rank order, as the reference's barrier loop orders them.
+ * The device form copies P1, S2, roS1_old, roS2_old and media from the
+ * device; x, y, z are the values initial_data gives (I*h1, j*h2, k*h3), as
+ * in the _host form when its x, y or z is null. */
+int filt_save(const char* path, const filt_fields* dev, const filt_params* p,
+              int localNx, int rank, int size, int j, int truncate);
+int filt_save_host(const char* path, const filt_fields* host, const filt_params* p,
+                   int localNx, int rank, int size, int j, int truncate);
+
+/* restore (main.cpp:540-569) then the uploads of Initialize
+ * (main.cpp:217-228): read the blocks of ranks 0..rank in turn (each of its
+ * own N_to_local width), keep rank's, and copy P1, S2, roS1_old, roS2_old,
+ * media (and x, y, z where non-null) into the fields; *j receives the saved
+ * step.  localNx must be rank's width.  FILT_ERR_IO when the file is
+ * missing or short. */
+int filt_restore(const char* path, const filt_fields* dev, const filt_params* p,
+                 int localNx, int rank, int size, int* j);
+int filt_restore_host(const char* path, const filt_fields* host, const filt_params* p,
+                      int localNx, int rank, int size, int* j);
+
+/* save_data_plots (main.cpp:319-482): Tecplot ASCII files
+ * plot_S2/S2=%012.4f.dat, plot_P1/P1=..., plot_u/u=..., plot_S2_y/S2=...,
+ * plot_S2_x/S2=... and media.dat under dir (NULL: the working folder, as
+ * the reference), byte for byte as the reference writes them: rank 0
+ * creates the folders and files with their headers, then each rank, in
+ * rank order, appends its lines.  The device form copies P1, S2, u2x, u2y,
+ * u2z and media; x, y, z as for filt_save.  u must be current: a driver
+ * that skips u_calculation (filt_time_step with_u = 0) runs it on the
+ * output steps, where the reference computes it in the same step. */
+int filt_save_plots(const char* dir, const filt_fields* dev, const filt_params* p,
+                    int localNx, int rank, int size, double t);
+int filt_save_plots_host(const char* dir, const filt_fields* host, const filt_params* p,
+                         int localNx, int rank, int size, double t);
+
+/* Device-resident snapshots with asynchronous writers (no reference
+ * counterpart: the reference stops to copy and format, main.cpp:50-55).
+ * take: stream-ordered copy of the grid's P1, S2 (+ roS1_old, roS2_old with
+ * FILT_SNAP_SAVE; + u2x, u2y, u2z with FILT_SNAP_PLOTS) and media into the
+ * snapshot's own device buffer; returns at once and later steps may run.
+ * write_async: a host thread copies the snapshot to pinned memory on a
+ * second stream and writes the save block (save_path, or NULL) and / or the
+ * plots (plot_dir, or NULL) as filt_save / filt_save_plots would have at
+ * the moment of the take.  wait: joins the writer and returns its status.
+ * restore: stream-ordered copy back into a grid (rollback, preemption).
+ * A take or write_async waits for the snapshot's previous write. */
+#define FILT_SNAP_SAVE 1
+#define FILT_SNAP_PLOTS 2
+typedef struct filt_snapshot filt_snapshot;
+int filt_snapshot_create(filt_snapshot** out, const filt_params* p, int localNx,
+                         int rank, int size, int what);
+int filt_snapshot_destroy(filt_snapshot* s);
+int filt_snapshot_take(filt_snapshot* s, const filt_fields* dev, int j, double t);
+int filt_snapshot_write_async(filt_snapshot* s, const char* save_path, int truncate,
+                              const char* plot_dir);
+int filt_snapshot_wait(filt_snapshot* s);
+int filt_snapshot_restore(filt_snapshot* s, const filt_fields* dev, int* j);
+
 /* ------------------------------------------------------------- runtime -- */
 
 /* Wait for all work the library enqueued. */

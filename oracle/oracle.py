@@ -200,6 +200,12 @@ class RefCPU:
         lib.ref_drive.argtypes = [C, c_int, c_void_p, c_void_p, c_size_t]
         lib.ref_drive.restype = c_int
         lib.ref_exchange.argtypes = [c_int, c_void_p, c_void_p, c_int, c_int, c_int, C]
+        lib.ref_save.argtypes = [c_char_p, A, C, c_int, c_int, c_int, c_int]
+        lib.ref_save.restype = c_int
+        lib.ref_restore.argtypes = [c_char_p, A, C, POINTER(c_int), c_int, c_int, c_int]
+        lib.ref_restore.restype = c_int
+        lib.ref_save_data_plots.argtypes = [c_char_p, A, C, c_double, c_int, c_int, c_int]
+        lib.ref_save_data_plots.restype = c_int
         assert lib.ref_sizeof_consts() == ctypes.sizeof(RefConsts)
         assert lib.ref_sizeof_ptr_arrays() == ctypes.sizeof(RefArrays)
 
@@ -208,6 +214,27 @@ class RefCPU:
         save_exchange_data (which = 1) on host arrays."""
         self.lib.ref_exchange(which, array.ctypes.data, buffer.ctypes.data, localNx, rank, size,
                               ctypes.byref(to_ref_consts(p)))
+
+    def save(self, workdir, g, p, j, rank=0, size=1):
+        """The reference's save (main.cpp:485-537): writes
+        workdir/save/save.dat from the host grid g."""
+        rc = self.lib.ref_save(str(workdir).encode(), ref_arrays(g), to_ref_consts(p), j, g.nx,
+                               rank, size)
+        assert rc == 0, rc
+
+    def restore(self, workdir, g, p, rank=0, size=1):
+        """The reference's restore (main.cpp:540-569) into g; returns j."""
+        j = c_int(-1)
+        rc = self.lib.ref_restore(str(workdir).encode(), ref_arrays(g), to_ref_consts(p),
+                                  ctypes.byref(j), g.nx, rank, size)
+        assert rc == 0, rc
+        return j.value
+
+    def save_data_plots(self, workdir, g, p, t, rank=0, size=1):
+        """The reference's save_data_plots (main.cpp:319-347) in workdir."""
+        rc = self.lib.ref_save_data_plots(str(workdir).encode(), ref_arrays(g), to_ref_consts(p),
+                                          t, g.nx, rank, size)
+        assert rc == 0, rc
 
     def drive(self, p, nsteps):
         """The reference driver end to end (Initialize, time_step_function x

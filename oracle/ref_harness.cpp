@@ -107,6 +107,38 @@ void ref_exchange(int which, double* array, double* buffer, int localNx, int ran
         save_exchange_data(array, array, buffer, buffer, localNx, 0, 0, rank, size, *def);
 }
 
+// The reference driver's file output, run in workdir (its paths are
+// relative): save (main.cpp:485-537) writes save/save.dat, restore
+// (main.cpp:540-569) reads it, save_data_plots (main.cpp:319-347) writes the
+// plot_* folders and media.dat.  The CPU backend's load_data_to_host is a
+// no-op, so the host arrays are what they write.
+int ref_save(const char* workdir, const ptr_Arrays* host, const consts* def, int j, int localNx,
+             int rank, int size)
+{
+    char old[4096];
+    if (!getcwd(old, sizeof old) || chdir(workdir) != 0) return 1;
+    save(*host, *host, j, rank, size, localNx, *def);
+    return chdir(old) != 0 ? 3 : 0;
+}
+
+int ref_restore(const char* workdir, const ptr_Arrays* host, const consts* def, int* j,
+                int localNx, int rank, int size)
+{
+    char old[4096];
+    if (!getcwd(old, sizeof old) || chdir(workdir) != 0) return 1;
+    restore(*host, j, rank, size, localNx, *def);
+    return chdir(old) != 0 ? 3 : 0;
+}
+
+int ref_save_data_plots(const char* workdir, const ptr_Arrays* host, const consts* def, double t,
+                        int localNx, int rank, int size)
+{
+    char old[4096];
+    if (!getcwd(old, sizeof old) || chdir(workdir) != 0) return 1;
+    save_data_plots(*host, *host, t, size, rank, localNx, *def);
+    return chdir(old) != 0 ? 3 : 0;
+}
+
 extern double* DevBuffer;  // main.cpp:7
 
 // The reference driver end to end with whichever backend this library links:

@@ -22,6 +22,8 @@ FILT_ERR_UNSTABLE = 4
 FILT_ERR_IO = 5
 FILT_ERR_UNSUPPORTED = 6
 FILT_ERR_NOT_FOUND = 7
+FILT_SNAP_SAVE = 1
+FILT_SNAP_PLOTS = 2
 
 _D = c_double * FILT_MAX_MEDIA
 
@@ -174,6 +176,18 @@ def load_library(path=HIP_LIB):
          POINTER(ctypes.c_longlong))
     _sig(lib, "filt_serve_is_done", c_int, c_void_p, c_int)
     _sig(lib, "filt_serve_take", c_int, c_void_p, c_int, c_void_p, c_void_p, POINTER(c_double))
+    for name in ("filt_save", "filt_save_host"):
+        _sig(lib, name, c_int, c_char_p, F, P, c_int, c_int, c_int, c_int, c_int)
+    for name in ("filt_restore", "filt_restore_host"):
+        _sig(lib, name, c_int, c_char_p, F, P, c_int, c_int, c_int, pi)
+    for name in ("filt_save_plots", "filt_save_plots_host"):
+        _sig(lib, name, c_int, c_char_p, F, P, c_int, c_int, c_int, c_double)
+    _sig(lib, "filt_snapshot_create", c_int, POINTER(c_void_p), P, c_int, c_int, c_int, c_int)
+    _sig(lib, "filt_snapshot_destroy", c_int, c_void_p)
+    _sig(lib, "filt_snapshot_take", c_int, c_void_p, F, c_int, c_double)
+    _sig(lib, "filt_snapshot_write_async", c_int, c_void_p, c_char_p, c_int, c_char_p)
+    _sig(lib, "filt_snapshot_wait", c_int, c_void_p)
+    _sig(lib, "filt_snapshot_restore", c_int, c_void_p, F, pi)
     _sig(lib, "filt_synchronize", c_int)
     _sig(lib, "filt_set_stream", c_int, c_void_p)
     _sig(lib, "filt_get_stream", c_void_p)

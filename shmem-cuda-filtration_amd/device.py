@@ -90,6 +90,25 @@ class DeviceGrid:
     def sync(self):
         check(self.lib, self.lib.filt_synchronize(), "filt_synchronize")
 
+    # -- checkpoint / resume, plots (save, restore, save_data_plots) --------
+    def save(self, path, j, truncate=True, rank=0, size=1):
+        check(self.lib, self.lib.filt_save(str(path).encode(), ctypes.byref(self.f),
+                                           ctypes.byref(self.params), self.nx, rank, size, j,
+                                           1 if truncate else 0), "filt_save")
+
+    def restore(self, path, rank=0, size=1):
+        """Load a save file into this grid; returns its step j."""
+        j = ctypes.c_int(-1)
+        check(self.lib, self.lib.filt_restore(str(path).encode(), ctypes.byref(self.f),
+                                              ctypes.byref(self.params), self.nx, rank, size,
+                                              ctypes.byref(j)), "filt_restore")
+        return j.value
+
+    def save_plots(self, folder, t, rank=0, size=1):
+        check(self.lib, self.lib.filt_save_plots(str(folder).encode(), ctypes.byref(self.f),
+                                                 ctypes.byref(self.params), self.nx, rank, size,
+                                                 t), "filt_save_plots")
+
     def free(self):
         if self.f.x:
             self.lib.filt_device_memory_free(ctypes.byref(self.f), self.buf)
@@ -219,3 +238,45 @@ class Backend:
 
     def sync(self):
         check(self.lib, self.lib.filt_synchronize(), "filt_synchronize")
+
+
+class Snapshot:
+    """Device-resident snapshot of a grid with an asynchronous writer
+    (filt_snapshot_* in the C ABI)."""
+
+    def __init__(self, grid, what=3, rank=0, size=1):
+        self.lib = grid.lib
+        self.h = ctypes.c_void_p()
+        check(self.lib, self.lib.filt_snapshot_create(ctypes.byref(self.h),
+                                                      ctypes.byref(grid.params), grid.nx, rank,
+                                                      size, what), "filt_snapshot_create")
+
+    def take(self, grid, j, t=0.0):
+        check(self.lib, self.lib.filt_snapshot_take(self.h, ctypes.byref(grid.f), j, t),
+              "filt_snapshot_take")
+
+    def write_async(self, save_path=None, plot_dir=None, truncate=True):
+        check(self.lib, self.lib.filt_snapshot_write_async(
+            self.h, str(save_path).encode() if save_path else None, 1 if truncate else 0,
+            str(plot_dir).encode() if plot_dir else None), "filt_snapshot_write_async")
+
+    def wait(self):
+        check(self.lib, self.lib.filt_snapshot_wait(self.h), "filt_snapshot_wait")
+
+    def restore(self, grid):
+        j = ctypes.c_int(-1)
+        check(self.lib, self.lib.filt_snapshot_restore(self.h, ctypes.byref(grid.f),
+                                                       ctypes.byref(j)), "filt_snapshot_restore")
+        return j.value
+
+    def close(self):
+        if self.h:
+            rc = self.lib.filt_snapshot_destroy(self.h)
+            self.h = ctypes.c_void_p()
+            check(self.lib, rc, "filt_snapshot_destroy")
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
