@@ -58,7 +58,7 @@ typedef struct filt_params {
     double P_d[FILT_MAX_MEDIA];    /* entry (threshold) pressure, Pa */
     double h1, h2, h3;             /* node spacing in x, y, z, m */
     double dt;                     /* time step, s */
-    double tau, l_w, l_n, c;       /* divgrad-scheme constants (unused on this path) */
+    double tau, l_w, l_n, c;       /* constants of the divgrad scheme (roS_scheme 1) */
     double beta1, beta2;           /* compressibility of water / NAPL, 1/Pa */
     int NX, Ny, Nz;                /* global grid */
     int source;                    /* half-width of the reference's source patch */
@@ -73,7 +73,15 @@ typedef struct filt_params {
     /* source patch on j = 0: src_x0 <= I <= src_x1 and src_z0 <= k <= src_z1 */
     int src_x0, src_x1, src_z0, src_z1;
     int n_media;                   /* entries in use in the media table */
+    /* roS_calculation's scheme: FILT_SCHEME_UPWIND, the one the reference
+     * calls (assign_rS_nr, cpu.cpp:218-291), or FILT_SCHEME_DIVGRAD, the
+     * tau-relaxed scheme of assign_rS (cpu.cpp:162-216), which reads u and
+     * roS*_old of the previous step */
+    int roS_scheme;
 } filt_params;
+
+#define FILT_SCHEME_UPWIND 0
+#define FILT_SCHEME_DIVGRAD 1
 
 /*
  * Field arrays of one grid, host or device: the members of ptr_Arrays_tag
@@ -112,7 +120,7 @@ void filt_params_finalize(filt_params* p);
  * reads plus the keys defines.ini documents but the reference ignores
  * (K[i], LAMBDA[i], S_WR[i], M[i], P_D[i], MU1, MU2, RO_W, RO_N, S2_GR,
  * P_ATM, G_CONST) and this backend's extensions (NEWTON_ITERATIONS,
- * SRC_X0, SRC_X1, SRC_Z0, SRC_Z1, N_MEDIA).  Keys absent from the file keep
+ * SRC_X0, SRC_X1, SRC_Z0, SRC_Z1, N_MEDIA, ROS_SCHEME).  Keys absent from the file keep
  * their value in *p.  Returns FILT_ERR_IO when the file cannot be opened
  * (the reference prints and continues, main.cpp:620-624). */
 int filt_read_defines(const char* path, filt_params* p);

@@ -20,6 +20,10 @@
  *     bit for bit; bounds from `source` equal the reference patch).
  *   orc_mass: no reference counterpart; pinned by closed forms on uniform
  *     states.
+ *   orc_roS_calculation with roS_scheme = FILT_SCHEME_DIVGRAD (assign_rS,
+ *     cpu.cpp:162-216): bit-exact against the reference's own assign_rS
+ *     called through oracle/_ref, per stage and over trajectories
+ *     (tests/test_divgrad.py).
  */
 #include "filt_oracle.h"
 
@@ -282,6 +286,76 @@ static void cell_roS(const filt_fields* f, const filt_params* p, int i, int j, i
     f->roS2[c] = A2;
 }
 
+/* cpu.cpp:162-216 (assign_rS): the tau-relaxed "divgrad" scheme, selected by
+ * p->roS_scheme.  It adds a capillary-like diffusion of ro*S with
+ * coefficient m*l*c/2 and central transport terms of ro*u, and relaxes in
+ * time with tau through roS_old (plain Euler on the first step, t < 2 dt).
+ * The reference's own z transport terms are kept as written: water reads
+ * ro1*u1x at (i+1, j, k+1) and (i, j, k-1), NAPL reads ro2*u2y at k+-1
+ * (cpu.cpp:184-185). */
+static void cell_roS_divgrad(const filt_fields* f, const filt_params* p, int i, int j, int k,
+                             int localNx, double t)
+{
+    if (!is_interior(i, j, k, localNx, p))
+        return;
+    const ptrdiff_t c = IDX(i, j, k);
+    const int md = cell_media(f, c);
+    const double* r1 = f->ro1;
+    const double* r2 = f->ro2;
+    const double* S = f->S2;
+
+    f->roS1[c] = r1[c] * (1 - S[c]);
+    f->roS2[c] = r2[c] * S[c];
+
+    const double m = cell_phi(f, p, c, md);
+    const double k1 = m * p->l_w * p->c / 2.;
+    const double k2 = m * p->l_n * p->c / 2.;
+    double dg1, dg2, Tz1, Tz2;
+    if (p->Nz < 2) {
+        dg1 = 0;
+        dg2 = 0;
+        Tz1 = 0;
+        Tz2 = 0;
+    } else {
+        const ptrdiff_t zp = IDX(i, j, k + 1), zm = IDX(i, j, k - 1), xz = IDX(i + 1, j, k + 1);
+        dg1 = k1 * (r1[zp] * (1. - S[zp]) - 2 * r1[c] * (1. - S[c]) + r1[zm] * (1. - S[zm])) /
+              (p->h3 * p->h3);
+        dg2 = k2 * (r2[zp] * S[zp] - 2 * r2[c] * S[c] + r2[zm] * (S[zm])) / (p->h3 * p->h3);
+        Tz1 = (r1[xz] * f->u1x[xz] - r1[zm] * f->u1x[zm]) / (2. * p->h3);
+        Tz2 = (r2[zp] * f->u2y[zp] - r2[zm] * f->u2y[zm]) / (2. * p->h3);
+    }
+    const ptrdiff_t xp = IDX(i + 1, j, k), xm = IDX(i - 1, j, k);
+    const ptrdiff_t yp = IDX(i, j + 1, k), ym = IDX(i, j - 1, k);
+    dg1 += k1 * ((r1[xp] * (1. - S[xp]) - 2 * r1[c] * (1. - S[c]) + r1[xm] * (1. - S[xm])) /
+                     (p->h1 * p->h1) +
+                 (r1[yp] * (1. - S[yp]) - 2 * r1[c] * (1. - S[c]) + r1[ym] * (1. - S[ym])) /
+                     (p->h2 * p->h2));
+    dg2 += k2 * ((r2[xp] * S[xp] - 2 * r2[c] * S[c] + r2[xm] * (S[xm])) / (p->h1 * p->h1) +
+                 (r2[yp] * S[yp] - 2 * r2[c] * S[c] + r2[ym] * (S[ym])) / (p->h2 * p->h2));
+    const double Tx1 = (r1[xp] * f->u1x[xp] - r1[xm] * f->u1x[xm]) / (2. * p->h1);
+    const double Ty1 = (r1[yp] * f->u1y[yp] - r1[ym] * f->u1y[ym]) / (2. * p->h2);
+    const double Tx2 = (r2[xp] * f->u2x[xp] - r2[xm] * f->u2x[xm]) / (2. * p->h1);
+    const double Ty2 = (r2[yp] * f->u2y[yp] - r2[ym] * f->u2y[ym]) / (2. * p->h2);
+
+    const double dt = p->dt, tau = p->tau;
+    double A1, A2;
+    if (t < 2 * dt) {
+        A1 = f->roS1[c] + (dt / m) * (dg1 - Tx1 - Ty1 - Tz1);
+        A2 = f->roS2[c] + (dt / m) * (dg2 - Tx2 - Ty2 - Tz2);
+    } else {
+        A1 = (2. * dt * dt) / (m * (dt + 2. * tau)) *
+             (dg1 - Tx1 - Ty1 - Tz1 + (2. * f->roS1[c] * m * tau) / (dt * dt) +
+              f->roS1_old[c] * m * (dt - 2. * tau) / (2. * dt * dt));
+        A2 = (2. * dt * dt) / (m * (dt + 2. * tau)) *
+             (dg2 - Tx2 - Ty2 - Tz2 + (2. * f->roS2[c] * m * tau) / (dt * dt) +
+              f->roS2_old[c] * m * (dt - 2. * tau) / (2. * dt * dt));
+    }
+    f->roS1_old[c] = f->roS1[c];
+    f->roS2_old[c] = f->roS2[c];
+    f->roS1[c] = A1;
+    f->roS2[c] = A2;
+}
+
 /* cpu.cpp:293-320: Newton iterations on
  *   F1 = ro01(1+b1(P-Pa))(1-S) - roS1,  F2 = ro02(1+b2(P+Pk-Pa))S - roS2
  * with the 2x2 Jacobian inverted through its adjugate. */
@@ -436,12 +510,24 @@ void orc_u_calculation(const filt_fields* f, const filt_params* p, int localNx, 
     sweep(cell_u, f, p, localNx, rank, size);
 }
 
-/* cpu.cpp:26-33 (t is unused by assign_rS_nr) */
+/* cpu.cpp:26-33 (t is unused by assign_rS_nr); the divgrad scheme reads
+ * only ro, S2, u and roS_old and writes only roS, roS_old of its own cell,
+ * so its cells are independent as well. */
 void orc_roS_calculation(const filt_fields* f, const filt_params* p, double t, int localNx,
                          int rank, int size)
 {
-    (void)t;
-    sweep(cell_roS, f, p, localNx, rank, size);
+    if (p->roS_scheme != FILT_SCHEME_DIVGRAD) {
+        sweep(cell_roS, f, p, localNx, rank, size);
+        return;
+    }
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
+#endif
+    for (int k = 0; k < p->Nz; k++)
+        for (int j = 0; j < p->Ny; j++)
+            for (int i = 0; i < localNx; i++)
+                if (orc_is_active_point(i, localNx, rank, size))
+                    cell_roS_divgrad(f, p, i, j, k, localNx, t);
 }
 
 /* cpu.cpp:35-42 */
@@ -470,7 +556,7 @@ void orc_time_step(const filt_fields* f, const filt_params* p, double t, int loc
                    int rank, int size, int with_u)
 {
     orc_ro_P2_Xi_calculation(f, p, localNx, rank, size);
-    if (with_u)
+    if (with_u || p->roS_scheme == FILT_SCHEME_DIVGRAD)  /* divgrad reads u */
         orc_u_calculation(f, p, localNx, rank, size);
     orc_roS_calculation(f, p, t, localNx, rank, size);
     orc_P1_S2_calculation(f, p, localNx, rank, size);

@@ -200,6 +200,9 @@ class RefCPU:
         lib.ref_drive.argtypes = [C, c_int, c_void_p, c_void_p, c_size_t]
         lib.ref_drive.restype = c_int
         lib.ref_exchange.argtypes = [c_int, c_void_p, c_void_p, c_int, c_int, c_int, C]
+        if hasattr(lib, "ref_roS_divgrad"):  # absent where another backend replaces cpu.cpp
+            lib.ref_roS_divgrad.argtypes = [A, C, c_double, c_int, c_int, c_int]
+            lib.ref_run_divgrad.argtypes = [A, C, c_int, c_int, c_int, c_int, c_int]
         lib.ref_save.argtypes = [c_char_p, A, C, c_int, c_int, c_int, c_int]
         lib.ref_save.restype = c_int
         lib.ref_restore.argtypes = [c_char_p, A, C, POINTER(c_int), c_int, c_int, c_int]
@@ -214,6 +217,15 @@ class RefCPU:
         save_exchange_data (which = 1) on host arrays."""
         self.lib.ref_exchange(which, array.ctypes.data, buffer.ctypes.data, localNx, rank, size,
                               ctypes.byref(to_ref_consts(p)))
+
+    def roS_divgrad(self, g, p, t, rank=0, size=1):
+        """roS_calculation's loop over assign_rS (cpu.cpp:162-216)."""
+        self.lib.ref_roS_divgrad(ref_arrays(g), to_ref_consts(p), t, g.nx, rank, size)
+
+    def run_divgrad(self, g, p, nsteps, step0=0, rank=0, size=1):
+        """time_step_function with the divgrad roS stage."""
+        self.lib.ref_run_divgrad(ref_arrays(g), to_ref_consts(p), g.nx, rank, size, step0,
+                                 nsteps)
 
     def save(self, workdir, g, p, j, rank=0, size=1):
         """The reference's save (main.cpp:485-537): writes

@@ -12,6 +12,12 @@
 
 extern double* HostBuffer;  // main.cpp:6
 
+#ifndef REF_HARNESS_NO_CPU
+// assign_rS (cpu.cpp:162-216): defined in cpu.cpp, not declared in defines.h.
+// Left out of the build that links another backend in place of cpu.cpp.
+void assign_rS(ptr_Arrays HostArraysPtr, double t, int i, int j, int k, int localNx, consts def);
+#endif
+
 extern "C" {
 
 size_t ref_sizeof_consts(void) { return sizeof(consts); }
@@ -48,6 +54,42 @@ int ref_stage(int stage, const ptr_Arrays* host, const consts* def, int localNx,
     }
     return 1;
 }
+
+#ifndef REF_HARNESS_NO_CPU
+// roS_calculation's loop (cpu.cpp:26-33) with assign_rS in place of
+// assign_rS_nr.
+static void roS_divgrad(const ptr_Arrays& h, const consts& def, double t, int localNx, int rank,
+                        int size)
+{
+    for (int i = 0; i < localNx; i++)
+        for (int j = 0; j < def.Ny; j++)
+            for (int k = 0; k < def.Nz; k++)
+                if (is_active_point(i, localNx, rank, size))
+                    assign_rS(h, t, i, j, k, localNx, def);
+}
+
+void ref_roS_divgrad(const ptr_Arrays* host, const consts* def, double t, int localNx, int rank,
+                     int size)
+{
+    roS_divgrad(*host, *def, t, localNx, rank, size);
+}
+
+// time_step_function (main.cpp:83-94) with the divgrad roS stage, nsteps
+// times from step0, t = s*dt as main.cpp:43.
+void ref_run_divgrad(const ptr_Arrays* host, const consts* def, int localNx, int rank, int size,
+                     int step0, int nsteps)
+{
+    ptr_Arrays h = *host;
+    for (int s = step0 + 1; s <= step0 + nsteps; s++) {
+        const double t = s * def->dt;
+        ro_P2_Xi_calculation(h, h, *def, localNx, rank, size, 0, 0, 0);
+        u_calculation(h, h, localNx, rank, size, 0, 0, 0, *def);
+        roS_divgrad(h, *def, t, localNx, rank, size);
+        P1_S2_calculation(h, h, *def, localNx, rank, size, 0, 0, 0);
+        boundary_conditions(h, h, localNx, rank, size, 0, 0, 0, *def);
+    }
+}
+#endif
 
 // time_step_function (main.cpp:83-94), nsteps times, t = s*dt as main.cpp:43.
 void ref_run(const ptr_Arrays* host, const consts* def, int localNx, int localNy, int rank,

@@ -24,6 +24,8 @@ FILT_ERR_UNSUPPORTED = 6
 FILT_ERR_NOT_FOUND = 7
 FILT_SNAP_SAVE = 1
 FILT_SNAP_PLOTS = 2
+FILT_SCHEME_UPWIND = 0
+FILT_SCHEME_DIVGRAD = 1
 
 _D = c_double * FILT_MAX_MEDIA
 
@@ -43,6 +45,7 @@ class FiltParams(ctypes.Structure):
         ("newton_iterations", c_int),
         ("src_x0", c_int), ("src_x1", c_int), ("src_z0", c_int), ("src_z1", c_int),
         ("n_media", c_int),
+        ("roS_scheme", c_int),
     ]
 
     def copy(self):

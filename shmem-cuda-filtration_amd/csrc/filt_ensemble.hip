@@ -194,6 +194,10 @@ int filt_ensemble_add(filt_ensemble* e, const filt_params* p, const double* P1, 
         return set_error(FILT_ERR_INVALID, "null argument");
     if (p->NX < 3 || p->Ny < 3 || p->Nz < 3)
         return set_error(FILT_ERR_UNSUPPORTED, "the fused step needs every extent >= 3");
+    if (p->roS_scheme != FILT_SCHEME_UPWIND)
+        return set_error(FILT_ERR_UNSUPPORTED,
+                         "the fused step implements the upwind roS scheme only (the divgrad "
+                         "scheme runs on the stage path)");
     if ((double)p->NX * p->Ny * p->Nz > 2.0e9)
         return set_error(FILT_ERR_INVALID, "grid too large");
     if (p->n_media < 0 || p->n_media > FILT_MAX_MEDIA || p->newton_iterations < 0)

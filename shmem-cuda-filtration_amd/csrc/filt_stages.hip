@@ -200,6 +200,77 @@ __global__ __launch_bounds__(TX* TY) void k_roS(filt_fields f, filt_params p, in
     f.roS2[c] = roS2 - (p.dt / phi) * (g1 + g2 + g3);
 }
 
+// ------------------------------------------------- stage 3, divgrad scheme --
+// cpu.cpp:162-216 (assign_rS), selected by p.roS_scheme: diffusion of ro*S
+// with coefficient m*l*c/2, central transport of ro*u, and a tau relaxation
+// through roS_old (plain Euler while t < 2 dt).  The z transport terms are
+// the reference's own: water reads ro1*u1x at (i+1, j, k+1) and (i, j, k-1),
+// NAPL reads ro2*u2y at k+-1.  One thread per cell; this scheme is not on the
+// benchmarked path, so neighbours come straight from global memory (L2).
+__global__ __launch_bounds__(TX* TY) void k_roS_divgrad(filt_fields f, filt_params p, int nx,
+                                                        int rank, int size, double t)
+{
+    const Cell e = cell_of_thread(nx, p.Ny);
+    if (!e.valid || !is_active(e.i, nx, rank, size))
+        return;
+    const int ny = p.Ny, nz = p.Nz;
+    if (!is_interior(e.i, e.j, e.k, nx, ny, nz))
+        return;
+    const long long c = e.c;
+    const int md = media_of(f, p, c);
+    const double* __restrict__ r1 = f.ro1;
+    const double* __restrict__ r2 = f.ro2;
+    const double* __restrict__ S = f.S2;
+    const double roS1 = r1[c] * (1 - S[c]);
+    const double roS2 = r2[c] * S[c];
+    const double m = f.poro ? f.poro[c] : p.m[md];
+    const double k1 = m * p.l_w * p.c / 2.;
+    const double k2 = m * p.l_n * p.c / 2.;
+    double dg1, dg2, Tz1, Tz2;
+    if (nz < 2) {
+        dg1 = 0;
+        dg2 = 0;
+        Tz1 = 0;
+        Tz2 = 0;
+    } else {
+        const long long sz = (long long)nx * ny;
+        const long long zp = c + sz, zm = c - sz, xz = c + 1 + sz;
+        dg1 = k1 * (r1[zp] * (1. - S[zp]) - 2 * r1[c] * (1. - S[c]) + r1[zm] * (1. - S[zm])) /
+              (p.h3 * p.h3);
+        dg2 = k2 * (r2[zp] * S[zp] - 2 * r2[c] * S[c] + r2[zm] * (S[zm])) / (p.h3 * p.h3);
+        Tz1 = (r1[xz] * f.u1x[xz] - r1[zm] * f.u1x[zm]) / (2. * p.h3);
+        Tz2 = (r2[zp] * f.u2y[zp] - r2[zm] * f.u2y[zm]) / (2. * p.h3);
+    }
+    const long long xp = c + 1, xm = c - 1, yp = c + nx, ym = c - nx;
+    dg1 += k1 * ((r1[xp] * (1. - S[xp]) - 2 * r1[c] * (1. - S[c]) + r1[xm] * (1. - S[xm])) /
+                     (p.h1 * p.h1) +
+                 (r1[yp] * (1. - S[yp]) - 2 * r1[c] * (1. - S[c]) + r1[ym] * (1. - S[ym])) /
+                     (p.h2 * p.h2));
+    dg2 += k2 * ((r2[xp] * S[xp] - 2 * r2[c] * S[c] + r2[xm] * (S[xm])) / (p.h1 * p.h1) +
+                 (r2[yp] * S[yp] - 2 * r2[c] * S[c] + r2[ym] * (S[ym])) / (p.h2 * p.h2));
+    const double Tx1 = (r1[xp] * f.u1x[xp] - r1[xm] * f.u1x[xm]) / (2. * p.h1);
+    const double Ty1 = (r1[yp] * f.u1y[yp] - r1[ym] * f.u1y[ym]) / (2. * p.h2);
+    const double Tx2 = (r2[xp] * f.u2x[xp] - r2[xm] * f.u2x[xm]) / (2. * p.h1);
+    const double Ty2 = (r2[yp] * f.u2y[yp] - r2[ym] * f.u2y[ym]) / (2. * p.h2);
+    const double dt = p.dt, tau = p.tau;
+    double A1, A2;
+    if (t < 2 * dt) {
+        A1 = roS1 + (dt / m) * (dg1 - Tx1 - Ty1 - Tz1);
+        A2 = roS2 + (dt / m) * (dg2 - Tx2 - Ty2 - Tz2);
+    } else {
+        A1 = (2. * dt * dt) / (m * (dt + 2. * tau)) *
+             (dg1 - Tx1 - Ty1 - Tz1 + (2. * roS1 * m * tau) / (dt * dt) +
+              f.roS1_old[c] * m * (dt - 2. * tau) / (2. * dt * dt));
+        A2 = (2. * dt * dt) / (m * (dt + 2. * tau)) *
+             (dg2 - Tx2 - Ty2 - Tz2 + (2. * roS2 * m * tau) / (dt * dt) +
+              f.roS2_old[c] * m * (dt - 2. * tau) / (2. * dt * dt));
+    }
+    f.roS1_old[c] = roS1;
+    f.roS2_old[c] = roS2;
+    f.roS1[c] = A1;
+    f.roS2[c] = A2;
+}
+
 // ---------------------------------------------------------------- stage 4 --
 // cpu.cpp:35-42, 293-320 (interior cells).  Bytes per cell: media, S2, P1,
 // roS1, roS2 in; P1, S2 out.
@@ -411,6 +482,8 @@ int launch_check(const filt_fields* dev, const filt_params* p, int localNx)
         return set_error(FILT_ERR_INVALID, "n_media out of range");
     if (p->newton_iterations < 0 || p->newton_iterations > 1000)
         return set_error(FILT_ERR_INVALID, "newton_iterations out of range");
+    if (p->roS_scheme != FILT_SCHEME_UPWIND && p->roS_scheme != FILT_SCHEME_DIVGRAD)
+        return set_error(FILT_ERR_INVALID, "unknown roS_scheme %d", p->roS_scheme);
     return FILT_OK;
 }
 
@@ -457,9 +530,17 @@ int filt_u_calculation(const filt_fields* dev, const filt_params* p, int localNx
 int filt_roS_calculation(const filt_fields* dev, const filt_params* p, double t, int localNx,
                          int rank, int size)
 {
-    (void)t;  // unused by assign_rS_nr as well (cpu.cpp:218)
     int rc = launch_check(dev, p, localNx);
     if (rc) return rc;
+    if (p->roS_scheme == FILT_SCHEME_DIVGRAD) {
+        if (!dev->u1x || !dev->u1y || !dev->u2x || !dev->u2y || !dev->roS1_old || !dev->roS2_old)
+            return set_error(FILT_ERR_INVALID, "the divgrad scheme reads u and roS_old");
+        hipLaunchKernelGGL(k_roS_divgrad, cell_grid(p, localNx), dim3(TX, TY), 0, stream(), *dev,
+                           *p, localNx, rank, size, t);
+        FILT_LAUNCH_CHECK("k_roS_divgrad");
+        return FILT_OK;
+    }
+    // t is unused by the upwind scheme, as by assign_rS_nr (cpu.cpp:218)
     if (use_tiled(p, localNx))
         return launch_roS_zm(*dev, *p, localNx);
     hipLaunchKernelGGL(k_roS, cell_grid(p, localNx), dim3(TX, TY), 0, stream(), *dev, *p, localNx,
@@ -509,7 +590,10 @@ int filt_time_step(const filt_fields* dev, const filt_params* p, double t, int l
 {
     int rc;
     if ((rc = filt_ro_P2_Xi_calculation(dev, p, localNx, rank, size))) return rc;
-    if (with_u && (rc = filt_u_calculation(dev, p, localNx, rank, size))) return rc;
+    // the divgrad scheme reads this step's u (cpu.cpp:187-190)
+    if ((with_u || p->roS_scheme == FILT_SCHEME_DIVGRAD) &&
+        (rc = filt_u_calculation(dev, p, localNx, rank, size)))
+        return rc;
     if ((rc = filt_roS_calculation(dev, p, t, localNx, rank, size))) return rc;
     if ((rc = filt_P1_S2_calculation(dev, p, localNx, rank, size))) return rc;
     return filt_boundary_conditions(dev, p, localNx, rank, size);

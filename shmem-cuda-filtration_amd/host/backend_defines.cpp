@@ -77,6 +77,10 @@ filt_params to_params(const consts& def)
     p.newton_iterations = k_newton_iterations;
     p.n_media = 2;
     p.src_x0 = -1;
+    // The reference switches roS_calculation to assign_rS (cpu.cpp:162-216)
+    // only by editing cpu.cpp:31; FILT_ROS_SCHEME=1 selects it here.
+    const char* scheme = getenv("FILT_ROS_SCHEME");
+    p.roS_scheme = scheme ? atoi(scheme) : FILT_SCHEME_UPWIND;
     filt_params_finalize(&p);
     return p;
 }

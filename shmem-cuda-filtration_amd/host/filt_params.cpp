@@ -129,6 +129,7 @@ int filt_read_defines(const char* path, filt_params* p)
         // extensions of this backend
         else if (key == "NEWTON_ITERATIONS") p->newton_iterations = atoi(s);
         else if (key == "N_MEDIA") p->n_media = atoi(s);
+        else if (key == "ROS_SCHEME") p->roS_scheme = atoi(s);
         else if (key == "SRC_X0") { p->src_x0 = atoi(s); explicit_patch = true; }
         else if (key == "SRC_X1") { p->src_x1 = atoi(s); explicit_patch = true; }
         else if (key == "SRC_Z0") { p->src_z0 = atoi(s); explicit_patch = true; }

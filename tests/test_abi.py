@@ -76,8 +76,8 @@ def test_backend_names_match_reference_backend(ref):
 
 
 def test_struct_layouts_match_header(pkg):
-    """ctypes mirrors have the C sizes (filt_params 504 bytes, 23 pointers)."""
-    assert ctypes.sizeof(pkg.abi.FiltParams) == 504
+    """ctypes mirrors have the C sizes (filt_params 512 bytes, 23 pointers)."""
+    assert ctypes.sizeof(pkg.abi.FiltParams) == 512
     assert ctypes.sizeof(pkg.abi.FiltFields) == 23 * 8
 
 

@@ -1,0 +1,82 @@
+"""The divgrad roS scheme (assign_rS, cpu.cpp:162-216) on the GPU against
+the oracle (itself bit-exact against the reference's assign_rS): bit for
+bit per stage and over trajectories; the fused forms refuse it."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from cases import FIELD_NAMES_ALL, HostGrid, random_state, ref_params
+
+pytestmark = pytest.mark.gpu
+
+
+def divgrad_params(*dims, **kw):
+    p = ref_params(*dims, **kw)
+    p.roS_scheme = 1
+    return p
+
+
+@pytest.mark.parametrize("dims", [(7, 6, 5), (70, 9, 6), (9, 8, 1), (33, 17, 11)])
+@pytest.mark.parametrize("tstep", [1, 4])
+def test_divgrad_stage_bitwise(hip, orc, dims, tstep):
+    p = divgrad_params(*dims)
+    g = random_state(p, np.random.default_rng(dims[0] + 100 * tstep))
+    d = hip.grid(p)
+    try:
+        d.upload(g)
+        d.stage(3, t=tstep * p.dt)
+        out = g.copy()
+        d.download(out)
+    finally:
+        d.free()
+    orc.stage(3, g, p, t=tstep * p.dt)
+    for n in FIELD_NAMES_ALL:
+        assert np.array_equal(out[n], g[n]), (n, dims, tstep)
+
+
+@pytest.mark.parametrize("dims,steps", [((20, 16, 12), 60), ((10, 9, 1), 30)])
+def test_divgrad_trajectory_matches_oracle(hip, orc, dims, steps):
+    """Stages 1 (pow: ulp-level), u, divgrad roS, Newton, boundary: the
+    trajectory bar of the plain path, relative L-inf <= 1e-10."""
+    from cases import max_rel
+    p = divgrad_params(*dims, source=1)
+    a = HostGrid(p)
+    orc.initial_data(a, p)
+    d = hip.grid(p)
+    try:
+        d.upload(a)
+        d.run(steps, with_u=0)  # the scheme needs u: the library computes it anyway
+        out = a.copy()
+        d.download(out)
+    finally:
+        d.free()
+    orc.run(a, p, steps)
+    for n in ("P1", "S2", "roS1", "roS2", "u2x", "u2y"):
+        assert max_rel(out[n], a[n]) <= 1e-10, n
+    assert np.isfinite(out["P1"]).all()
+
+
+def test_fused_forms_refuse_divgrad(hip):
+    p = divgrad_params(8, 8, 8)
+    e = hip.ensemble(0)
+    try:
+        P = np.full((8, 8, 8), 1e5)
+        S = np.zeros((8, 8, 8))
+        mid = ctypes.c_int()
+        rc = hip.lib.filt_ensemble_add(e.h, ctypes.byref(p), P.ctypes.data, S.ctypes.data,
+                                       None, None, None, ctypes.byref(mid))
+        assert rc == 6  # FILT_ERR_UNSUPPORTED
+    finally:
+        e.close()
+
+
+def test_unknown_scheme_is_rejected(hip):
+    p = ref_params(6, 6, 6)
+    p.roS_scheme = 7
+    d = hip.grid(p)
+    try:
+        rc = hip.lib.filt_roS_calculation(ctypes.byref(d.f), ctypes.byref(d.params), 0.0, 6, 0, 1)
+        assert rc == 1  # FILT_ERR_INVALID
+    finally:
+        d.free()
