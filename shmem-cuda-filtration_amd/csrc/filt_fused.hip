@@ -68,14 +68,29 @@ struct C1 {
     double P, X1, X2, r1, r2;
 };
 
-__device__ inline C1 fp64_cell(const FusedGrid& G, const double* Pin, const double* Sin,
-                               long long c)
+// Inputs of one cell, loaded a plane ahead of their use: state, K and, for
+// the cell a thread updates, the porosity.
+struct RawD {
+    double P, S, K, phi;
+    int md;
+};
+
+__device__ inline RawD fp64_raw(const FusedGrid& G, const double* Pin, const double* Sin,
+                                long long c, bool centre)
 {
-    const int md = media_at(G, c);
-    const double K = G.perm ? G.perm[c] : G.p.K[md];
-    const double P = Pin[c];
-    const Stage1 r = stage1_cell(P, Sin[c], K, G.p.lambda[md], G.p.S_wr[md], G.p);
-    return C1{P, r.Xi1, r.Xi2, r.ro1, r.ro2};
+    RawD r;
+    r.md = media_at(G, c);
+    r.P = Pin[c];
+    r.S = Sin[c];
+    r.K = G.perm ? G.perm[c] : G.p.K[r.md];
+    r.phi = centre ? (G.poro ? G.poro[c] : G.p.m[r.md]) : 0.0;
+    return r;
+}
+
+__device__ inline C1 fp64_eval(const FusedGrid& G, const RawD& q)
+{
+    const Stage1 r = stage1_cell(q.P, q.S, q.K, G.p.lambda[q.md], G.p.S_wr[q.md], G.p);
+    return C1{q.P, r.Xi1, r.Xi2, r.ro1, r.ro2};
 }
 
 // LDS planes of the FP64 form: P, Xi1, ro1, Xi2, ro2.
@@ -98,9 +113,9 @@ struct Fp64Lds {
 // roS_calculation (cpu.cpp:218-291) then Newton (cpu.cpp:293-320) for an
 // interior cell, expression for expression as in filt_stages.hip; neighbours
 // in x / y from LDS slots, in z from the registers zm / zp.
-__device__ inline void fp64_update(const FusedGrid& G, long long c, const C1& cc, const C1& zm,
-                                   const C1& zp, const Fp64Lds& L, int s, double S, double& nP,
-                                   double& nS)
+__device__ inline void fp64_update(const FusedGrid& G, const C1& cc, const C1& zm,
+                                   const C1& zp, const Fp64Lds& L, int s, const RawD& q,
+                                   double& nP, double& nS)
 {
     const filt_params& p = G.p;
     const int xm = s - 1, xp = s + 1, ym = s - SX, yp = s + SX;
@@ -109,8 +124,8 @@ __device__ inline void fp64_update(const FusedGrid& G, long long c, const C1& cc
     const double az2 = -(zp.P - p1) / p.h3, az1 = -(p1 - zm.P) / p.h3;
     const double ax2 = -(L.P[xp] - p1) / p.h1, ax1 = -(p1 - L.P[xm]) / p.h1;
     const double dy2 = -(L.P[yp] - p1) / p.h2, dy1 = -(p1 - L.P[ym]) / p.h2;
-    const int md = media_at(G, c);
-    const double phi = G.poro ? G.poro[c] : p.m[md];
+    const int md = q.md;
+    const double phi = q.phi, S = q.S;
     const double f3 = upwind(az2, az1, cc.X1, cc.r1, zm.X1, zm.r1, zp.X1, zp.r1, p.h3);
     const double f1 = upwind(ax2, ax1, cc.X1, cc.r1, L.X1[xm], L.R1[xm], L.X1[xp], L.R1[xp], p.h1);
     const double f2 = upwind(dy2 + cc.r1 * g, dy1 + cc.r1 * g, cc.X1, cc.r1, L.X1[ym], L.R1[ym],
@@ -138,15 +153,32 @@ struct M1 {
     float L1, L2;
 };
 
-__device__ inline M1 mixed_cell(const FusedGrid& G, const double* Pin, const double* Sin,
-                                long long c)
+struct RawM {
+    double P, S;
+    float K, phi;
+    int md;
+};
+
+__device__ inline RawM mixed_raw(const FusedGrid& G, const double* Pin, const double* Sin,
+                                 long long c, bool centre)
 {
-    const int md = media_at(G, c);
-    const float K = G.perm32 ? G.perm32[c] : (float)G.p.K[md];
-    const double P = Pin[c];
+    RawM r;
+    r.md = media_at(G, c);
+    r.P = Pin[c];
+    r.S = Sin[c];
+    r.K = G.perm32 ? G.perm32[c] : (float)G.p.K[r.md];
+    r.phi = (centre && G.poro32) ? G.poro32[c] : 0.f;
+    return r;
+}
+
+__device__ inline M1 mixed_eval(const FusedGrid& G, const RawM& q)
+{
+    const int md = q.md;
+    const float K = q.K;
+    const double P = q.P;
     // 1 - S_e = S/(1-S_wr) formed directly: at small NAPL saturation the FP32
     // difference 1 - S_e would cancel to a few significant bits.
-    const float oms = (float)(Sin[c] * G.inv_1mswr[md]);
+    const float oms = (float)(q.S * G.inv_1mswr[md]);
     const float se = 1.f - oms;
     float k1, k2;
     if (G.p.lambda[md] == 2.0) {  // Brooks-Corey exponents 4 and 2
@@ -186,10 +218,11 @@ __device__ __forceinline__ float uflux(float a2, float a1, float Lc, float Lm, f
     return (fmaxf(a2, 0.f) - fminf(a1, 0.f)) * Lc - fmaxf(a1, 0.f) * Lm + fminf(a2, 0.f) * Lp;
 }
 
-__device__ inline void mixed_update(const FusedGrid& G, long long c, const M1& cc, const M1& zm,
-                                    const M1& zp, const MixedLds& L, int s, double S, double& nP,
-                                    double& nS)
+__device__ inline void mixed_update(const FusedGrid& G, const M1& cc, const M1& zm,
+                                    const M1& zp, const MixedLds& L, int s, const RawM& q,
+                                    double& nP, double& nS)
 {
+    const double S = q.S;
     const filt_params& p = G.p;
     const int xm = s - 1, xp = s + 1, ym = s - SX, yp = s + SX;
     const double p1 = cc.P;
@@ -209,9 +242,9 @@ __device__ inline void mixed_update(const FusedGrid& G, long long c, const M1& c
                      uflux((float)(dy2 + r2c * g), (float)(dy1 + r2c * g), cc.L2, L.L2[ym],
                            L.L2[yp]) * fy +
                      uflux(az2, az1, cc.L2, zm.L2, zp.L2) * fz;
-    const int md = media_at(G, c);
+    const int md = q.md;
     // dt/phi in FP64: an FP32 factor would bias every increment alike
-    const double phi = G.poro32 ? (double)G.poro32[c] : p.m[md];
+    const double phi = G.poro32 ? (double)q.phi : p.m[md];
     double rphi = __builtin_amdgcn_rcp(phi);
     rphi = fma(fma(-phi, rphi, 1.0), rphi, rphi);
     const double dtphi = p.dt * rphi;
@@ -222,6 +255,30 @@ __device__ inline void mixed_update(const FusedGrid& G, long long c, const M1& c
     const double swr = p.S_wr[md], pd = p.P_d[md], lam = p.lambda[md];
     const double inv1 = G.inv_1mswr[md];
     const bool no_cap = (pd == 0.0) && (lam >= 0.1);
+    // Without capillary pressure (P_k = 0) the two equations
+    //   ro01 (1 + b1 x)(1 - S) = A1,   ro02 (1 + b2 x) S = A2,   x = P - P_atm
+    // reduce to one quadratic al x^2 + be x + ga = 0.  Its root near the
+    // linear solution -ga/be is -2 ga / (be + sign(be) sqrt(be^2 - 4 al ga)),
+    // free of cancellation, and S = A2 / (ro02 (1 + b2 x)): the point the
+    // Newton iteration converges to, for one square root and two divisions.
+    // Outside that case (capillary terms, S_e < 2^-10, no finite real root)
+    // the Newton iteration runs as the reference's.
+    if (no_cap && p.newton_iterations > 0) {
+        const double a1 = ro01 * b1, c1 = ro02 * b2;
+        const double al = a1 * c1;
+        const double be = ro01 * c1 + a1 * ro02 - A2 * a1 - A1 * c1;
+        const double ga = ro01 * ro02 - A2 * ro01 - A1 * ro02;
+        const double disc = be * be - 4.0 * al * ga;
+        if (disc >= 0.0) {
+            const double x = -2.0 * ga / (be + copysign(sqrt(disc), be));
+            const double Sx = A2 / (ro02 + c1 * x);
+            if (isfinite(x) && isfinite(Sx) && (1 - Sx - swr) * inv1 >= 0.0009765625) {
+                nP = Pa + x;
+                nS = Sx;
+                return;
+            }
+        }
+    }
     double P = p1, Sv = S;
     for (int w = 1; w <= p.newton_iterations; w++) {
         const double se = (1 - Sv - swr) * inv1;
@@ -335,17 +392,24 @@ __device__ inline void shell_cell(const FusedGrid& G, const double* Pin, const d
     So[c] = (k == 0) ? Sin[ii + sz] : (in_patch(ii, k < nz - 1 ? k : nz - 2, G.p) ? G.p.S2_gr : 0.0);
 }
 
-// ------------------------------------------------------------- the kernel --
-
 // ------------------------------------------------------------- the tiles --
-// One tile: TX x TY columns, planes [kb, ke) of the interior.  Per plane:
-// every thread evaluates stage 1 of its own column at k+1 (registers keep
-// k-1 and k), writes plane k into the LDS tile; threads t < RING (wave 0)
-// evaluate the one-cell ring; sync; interior cells update; sync.
+// One tile: FTX x FTY columns, planes [kb, ke) of the interior.  Registers
+// hold the stage-1 values of the own column at k-1, k, k+1.  In the mixed
+// form the loads are software-pipelined: while plane k is updated, the raw
+// inputs (state, K, porosity) of the own column at k+2 and of the ring cell
+// at k+1 are in flight, and they are evaluated at the top of the next
+// iteration.  The
+// one-cell ring is spread one cell per four threads, so each wave evaluates
+// 16 of its 64 cells.  Two LDS tiles alternate, so one barrier per plane
+// orders both the writes before the reads and the reads of plane k before
+// the overwrite at plane k+2.
 template <bool MIXED>
 __device__ void interior_tile(const FusedGrid& G, long long lb, int parity, char* lds)
 {
     using V = typename std::conditional<MIXED, M1, C1>::type;
+    using Raw = typename std::conditional<MIXED, RawM, RawD>::type;
+    using Lds = typename std::conditional<MIXED, MixedLds, Fp64Lds>::type;
+    constexpr int BUF = MIXED ? SN * 16 : 5 * SN * 8;  // bytes of one LDS tile
     const double* Pin = G.P[parity];
     const double* Sin = G.S[parity];
     double* Po = G.P[parity ^ 1];
@@ -357,78 +421,133 @@ __device__ void interior_tile(const FusedGrid& G, long long lb, int parity, char
     const int bz = (int)(r / G.nby);
     const int i0 = 1 + bx * FTX, j0 = 1 + by * FTY;
     const int kb = 1 + bz * G.kz, ke = min(kb + G.kz, nz - 1);
+    if (kb >= ke)
+        return;  // uniform over the block
     const int tx = threadIdx.x, ty = threadIdx.y, t = ty * FTX + tx;
     const int i = i0 + tx, j = j0 + ty;
     const bool in_grid = i < nx && j < ny;
     const bool inner = i <= nx - 2 && j <= ny - 2;
     const long long sy = nx, sz = (long long)nx * ny;
-    // ring cell of this thread (t < RING)
+    // ring cell h of this thread: in the mixed form one per four threads, so
+    // each wave evaluates 16; the FP64 form keeps them on wave 0 (measured
+    // faster there)
+    const int h = MIXED ? ((t & 3) == 0 ? (t >> 2) : RING) : t;
     int rx = 0, ry = 0;
-    if (t < FTX) { rx = t; ry = -1; }
-    else if (t < 2 * FTX) { rx = t - FTX; ry = FTY; }
-    else if (t < 2 * FTX + FTY) { rx = -1; ry = t - 2 * FTX; }
-    else if (t < RING) { rx = FTX; ry = t - 2 * FTX - FTY; }
-    const bool ring = t < RING && i0 + rx < nx && j0 + ry < ny;
+    if (h < FTX) { rx = h; ry = -1; }
+    else if (h < 2 * FTX) { rx = h - FTX; ry = FTY; }
+    else if (h < 2 * FTX + FTY) { rx = -1; ry = h - 2 * FTX; }
+    else if (h < RING) { rx = FTX; ry = h - 2 * FTX - FTY; }
+    const bool ring = h < RING && i0 + rx < nx && j0 + ry < ny;
     const long long rcol = (i0 + rx) + (j0 + ry) * sy;
     const long long col = i + j * sy;
     const int s = slot(tx, ty), rs = slot(rx, ry);
 
-    typename std::conditional<MIXED, MixedLds, Fp64Lds>::type L;
-    if constexpr (MIXED) {
-        L.P = (double*)lds;
-        L.L1 = (float*)(lds + SN * sizeof(double));
-        L.L2 = L.L1 + SN;
-    } else {
-        double* d = (double*)lds;
-        L.P = d;
-        L.X1 = d + SN;
-        L.R1 = d + 2 * SN;
-        L.X2 = d + 3 * SN;
-        L.R2 = d + 4 * SN;
-    }
-    auto cell = [&](long long c) -> V {
+    auto tile = [&](int q) {
+        char* base = lds + q * BUF;
+        Lds L;
+        if constexpr (MIXED) {
+            L.P = (double*)base;
+            L.L1 = (float*)(base + SN * sizeof(double));
+            L.L2 = L.L1 + SN;
+        } else {
+            double* d = (double*)base;
+            L.P = d;
+            L.X1 = d + SN;
+            L.R1 = d + 2 * SN;
+            L.X2 = d + 3 * SN;
+            L.R2 = d + 4 * SN;
+        }
+        return L;
+    };
+    auto raw = [&](long long c, bool centre) -> Raw {
         if constexpr (MIXED)
-            return mixed_cell(G, Pin, Sin, c);
+            return mixed_raw(G, Pin, Sin, c, centre);
         else
-            return fp64_cell(G, Pin, Sin, c);
+            return fp64_raw(G, Pin, Sin, c, centre);
+    };
+    auto eval = [&](const Raw& q) -> V {
+        if constexpr (MIXED)
+            return mixed_eval(G, q);
+        else
+            return fp64_eval(G, q);
     };
 
     V zm{}, zc{}, zp{};
-    if (in_grid) {
-        zm = cell(col + (kb - 1) * sz);
-        zc = cell(col + kb * sz);
-    }
-    for (int k = kb; k < ke; k++) {
+    if constexpr (!MIXED) {
+        // The FP64 form has no registers to spare for inputs in flight (they
+        // cost it an occupancy step): it loads and evaluates each plane at the
+        // top of its iteration, into one LDS tile, with a barrier after the
+        // writes and one after the reads.
+        const Lds L = tile(0);
         if (in_grid) {
-            zp = cell(col + (k + 1) * sz);
+            zm = eval(raw(col + (kb - 1) * sz, false));
+            zc = eval(raw(col + kb * sz, false));
+        }
+        for (int k = kb; k < ke; k++) {
+            if (in_grid) {
+                zp = eval(raw(col + (k + 1) * sz, false));  // k + 1 <= ke <= nz - 1
+                L.put(s, zc);
+            }
+            if (ring)
+                L.put(rs, eval(raw(rcol + k * sz, false)));
+            __syncthreads();
+            if (in_grid && inner) {
+                const long long c = col + k * sz;
+                double nP, nS;
+                fp64_update(G, zc, zm, zp, L, s, raw(c, true), nP, nS);
+                write_cell(G, Po, So, Pin, i, j, k, nP, nS);
+            }
+            __syncthreads();
+            zm = zc;
+            zc = zp;
+        }
+        return;
+    }
+    Raw qc{}, qp{}, rq{};  // own column at k and (in flight) k+1; ring cell at k
+    if (in_grid) {
+        zm = eval(raw(col + (kb - 1) * sz, false));
+        qc = raw(col + kb * sz, true);
+        zc = eval(qc);
+        qp = raw(col + (kb + 1) * sz, true);  // kb + 1 <= ke <= nz - 1
+    }
+    if (ring)
+        rq = raw(rcol + kb * sz, false);
+    for (int k = kb; k < ke; k++) {
+        const Lds L = tile((k - kb) & 1);
+        Raw qn = qp;
+        if (in_grid) {
+            zp = eval(qp);
             L.put(s, zc);
         }
         if (ring)
-            L.put(rs, cell(rcol + k * sz));
-        __syncthreads();
-        if (in_grid && inner) {
-            const long long c = col + k * sz;
-            double nP, nS;
-            if constexpr (MIXED)
-                mixed_update(G, c, zc, zm, zp, L, s, Sin[c], nP, nS);
-            else
-                fp64_update(G, c, zc, zm, zp, L, s, Sin[c], nP, nS);
-            write_cell(G, Po, So, Pin, i, j, k, nP, nS);
+            L.put(rs, eval(rq));
+        if (k + 1 < ke) {  // the next plane is updated: fetch its inputs
+            if (in_grid)
+                qp = raw(col + (k + 2) * sz, true);  // k + 2 <= ke <= nz - 1
+            if (ring)
+                rq = raw(rcol + (k + 1) * sz, false);
         }
         __syncthreads();
+        if (in_grid && inner) {
+            double nP, nS;
+            if constexpr (MIXED)
+                mixed_update(G, zc, zm, zp, L, s, qc, nP, nS);
+            write_cell(G, Po, So, Pin, i, j, k, nP, nS);
+        }
         zm = zc;
         zc = zp;
+        qc = qn;
     }
 }
 
-constexpr size_t LDS_BYTES = 5 * SN * sizeof(double);  // the larger (FP64) layout
+constexpr size_t LDS_BYTES = 5 * SN * sizeof(double);  // one FP64 tile (two mixed ones fit)
 
 template <bool MIXED>
 __global__ __launch_bounds__(FT) void k_fused(const FusedGrid* __restrict__ grids,
                                               const int* __restrict__ block_grid,
                                               long long total_blocks, int parity, int order)
 {
-    __shared__ __attribute__((aligned(16))) char lds[MIXED ? SN * 16 : LDS_BYTES];
+    __shared__ __attribute__((aligned(16))) char lds[MIXED ? 2 * SN * 16 : LDS_BYTES];
     long long b = blockIdx.x;
     if (order == 0 && total_blocks % 8 == 0)
         b = (b % 8) * (total_blocks / 8) + b / 8;
