@@ -259,7 +259,9 @@ int filt_save_exchange_data(double* dev_array, double* dev_buffer,
  * 16x16 or 64x8), kz planes per z-chunk, order 0 = XCD-grouped tile
  * numbering, 1 = hardware order.  Used by grids with every extent >= 3.
  * tx = 0 restores the built-in default (the fastest configuration on a
- * 256^3 grid: roS 64x4, kz 16, order 1; u 64x8, kz 64, order 1).
+ * 256^3 grid: roS 64x4, kz 16, order 1; u 64x8, kz 64, order 1); tx = -1
+ * selects the one-cell-per-thread kernels, which grids with an extent below
+ * 3 or arrays of 4 GiB and more use anyway (results are the same).
  * The setting is process-wide; results do not depend on it. */
 int filt_set_tile_config(int kernel, int tx, int ty, int kz, int order);
 /* The configuration used for an nx*ny*nz grid (tuned or default). */

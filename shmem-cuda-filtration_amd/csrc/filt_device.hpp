@@ -98,60 +98,72 @@ __device__ __forceinline__ bool same_bits(double a, double b)
 //
 // kLean = true keeps the division and only the fixed-point exit: six fewer
 // live registers, for kernels at the edge of an occupancy step.
+// One Newton iteration of cpu.cpp:300-317 from (P, S) with the capillary
+// terms P_k and PkS.
+__device__ inline void newton_step(double P, double S, double P_k, double PkS, double roS1,
+                                   double roS2, const filt_params& p, double& Pn, double& Sn)
+{
+    const double ro01 = p.ro01, ro02 = p.ro02, b1 = p.beta1, b2 = p.beta2, Pa = p.P_atm;
+    const double F1 = ro01 * (1. + b1 * (P - Pa)) * (1. - S) - roS1;
+    const double F2 = ro02 * (1. + b2 * (P + P_k - Pa)) * S - roS2;
+    const double F1P = ro01 * b1 * (1 - S);
+    const double F2P = ro02 * b2 * S;
+    const double F1S = (-1) * ro01 * (1 + b1 * (P - Pa));
+    const double F2S = ro02 * (1 + b2 * (P + P_k - Pa + (S * PkS)));
+    const double det = F1P * F2S - F1S * F2P;
+    Pn = P - (1 / det) * (F2S * F1 - F1S * F2);
+    Sn = S - (1 / det) * (F1P * F2 - F2P * F1);
+}
+
 template <bool kLean = false>
 __device__ inline void newton_cell(double& P, double& S, double roS1, double roS2, double lam,
                                    double swr, double pd, const filt_params& p)
 {
-    const double ro01 = p.ro01, ro02 = p.ro02, b1 = p.beta1, b2 = p.beta2, Pa = p.P_atm;
     const bool no_cap = (pd == 0.0) && (lam >= 0.1);
     const double d = 1 - swr;
-    const bool fast = !kLean && no_cap && d > 0.0;
-    const double d_t = d * 0.0009765625;
     const int n = p.newton_iterations;
-    double Pq = P, Sq = S;  // the state two iterations back
-    for (int w = 1; w <= n; w++) {
-        const double num = 1 - S - swr;
-        double P_k, PkS;
-        if (fast && num >= d_t) {
-            // P_d*S_e^(-1/lam) and S_e^(-1/lam-1)*P_d/(lam*(1-swr)) with finite
-            // powers and P_d = 0: both are zero, and a zero PkS enters F2S
-            // only as S*PkS added to a nonzero sum.
-            P_k = 0.0;
-            PkS = 0.0;
-        } else {
-            const double S_e = num / d;
-            if (no_cap && S_e >= 0.0009765625) {
-                P_k = 0.0;
-                PkS = 0.0;
-            } else {
-                P_k = pd * pow(S_e, (-1.) / lam);
-                const double AAA = pow(S_e, ((-1.) / lam) - 1.);
-                PkS = AAA * pd / (lam * (1 - swr));
+    int w = 1;
+    if (!kLean && no_cap && d > 0.0) {
+        // Capillary-free iterations while S_e >= 2^-10: P_k and PkS are
+        // exactly zero (a zero PkS enters F2S only as S*PkS added to a
+        // nonzero sum), so no pow and no division for S_e.  The first
+        // iteration that needs the general form continues below.
+        const double d_t = d * 0.0009765625;
+        double Pq = P, Sq = S;  // the state two iterations back
+        for (; w <= n; w++) {
+            if (!(1 - S - swr >= d_t))
+                break;
+            double Pn, Sn;
+            newton_step(P, S, 0.0, 0.0, roS1, roS2, p, Pn, Sn);
+            if (same_bits(Pn, P) && same_bits(Sn, S))
+                return;  // fixed point: (P, S) is the result
+            if (w >= 2 && same_bits(Pn, Pq) && same_bits(Sn, Sq)) {
+                // 2-cycle: x_{w+1} = x_{w-1}, x_{w+2} = x_w, ...
+                if (((n - w) & 1) == 0) {
+                    P = Pn;
+                    S = Sn;
+                }
+                return;
             }
-        }
-        const double F1 = ro01 * (1. + b1 * (P - Pa)) * (1. - S) - roS1;
-        const double F2 = ro02 * (1. + b2 * (P + P_k - Pa)) * S - roS2;
-        const double F1P = ro01 * b1 * (1 - S);
-        const double F2P = ro02 * b2 * S;
-        const double F1S = (-1) * ro01 * (1 + b1 * (P - Pa));
-        const double F2S = ro02 * (1 + b2 * (P + P_k - Pa + (S * PkS)));
-        const double det = F1P * F2S - F1S * F2P;
-        const double Pn = P - (1 / det) * (F2S * F1 - F1S * F2);
-        const double Sn = S - (1 / det) * (F1P * F2 - F2P * F1);
-        if (same_bits(Pn, P) && same_bits(Sn, S))
-            break;  // fixed point: (P, S) is the result
-        if (!kLean && w >= 2 && same_bits(Pn, Pq) && same_bits(Sn, Sq)) {
-            // 2-cycle: x_{w+1} = x_{w-1}, x_{w+2} = x_w, ...
-            if (((n - w) & 1) == 0) {
-                P = Pn;
-                S = Sn;
-            }
-            break;
-        }
-        if (!kLean) {
             Pq = P;
             Sq = S;
+            P = Pn;
+            S = Sn;
         }
+    }
+    for (; w <= n; w++) {
+        const double S_e = (1 - S - swr) / d;
+        double P_k = 0.0, PkS = 0.0;
+        if (!(no_cap && S_e >= 0.0009765625)) {
+            // the reference's powers; inf*0 = NaN propagates as there
+            P_k = pd * pow(S_e, (-1.) / lam);
+            const double AAA = pow(S_e, ((-1.) / lam) - 1.);
+            PkS = AAA * pd / (lam * (1 - swr));
+        }
+        double Pn, Sn;
+        newton_step(P, S, P_k, PkS, roS1, roS2, p, Pn, Sn);
+        if (same_bits(Pn, P) && same_bits(Sn, S))
+            return;
         P = Pn;
         S = Sn;
     }

@@ -138,7 +138,7 @@ __device__ inline void fp64_update(const FusedGrid& G, const C1& cc, const C1& z
     const double A2 = cc.r2 * S - (p.dt / phi) * (g1 + g2 + g3);
     nP = cc.P;
     nS = S;
-    newton_cell<true>(nP, nS, A1, A2, p.lambda[md], p.S_wr[md], p.P_d[md], p);
+    newton_cell<false>(nP, nS, A1, A2, p.lambda[md], p.S_wr[md], p.P_d[md], p);
 }
 
 // ----------------------------------------------------------- mixed form --
@@ -474,19 +474,23 @@ __device__ void interior_tile(const FusedGrid& G, long long lb, int parity, char
 
     V zm{}, zc{}, zp{};
     if constexpr (!MIXED) {
-        // The FP64 form has no registers to spare for inputs in flight (they
-        // cost it an occupancy step): it loads and evaluates each plane at the
-        // top of its iteration, into one LDS tile, with a barrier after the
-        // writes and one after the reads.
-        const Lds L = tile(0);
+        // The FP64 form loads and evaluates each plane at the top of its
+        // iteration (registers for inputs in flight would cost it an
+        // occupancy step) and keeps the own column's k-1 and k in the two
+        // LDS tiles rather than in registers: the tile of plane k-1 is read
+        // during plane k and overwritten only after the barrier that closes
+        // plane k.  The registers this frees pay for the Newton's 2-cycle
+        // exit.
+        auto get = [&](const Lds& T, int q) { return C1{T.P[q], T.X1[q], T.X2[q], T.R1[q], T.R2[q]}; };
         if (in_grid) {
-            zm = eval(raw(col + (kb - 1) * sz, false));
-            zc = eval(raw(col + kb * sz, false));
+            tile(1).put(s, eval(raw(col + (kb - 1) * sz, false)));  // read as plane kb - 1
+            zp = eval(raw(col + kb * sz, false));
         }
         for (int k = kb; k < ke; k++) {
+            const Lds L = tile((k - kb) & 1), Lm = tile((k - kb + 1) & 1);
             if (in_grid) {
+                L.put(s, zp);
                 zp = eval(raw(col + (k + 1) * sz, false));  // k + 1 <= ke <= nz - 1
-                L.put(s, zc);
             }
             if (ring)
                 L.put(rs, eval(raw(rcol + k * sz, false)));
@@ -494,12 +498,10 @@ __device__ void interior_tile(const FusedGrid& G, long long lb, int parity, char
             if (in_grid && inner) {
                 const long long c = col + k * sz;
                 double nP, nS;
-                fp64_update(G, zc, zm, zp, L, s, raw(c, true), nP, nS);
+                fp64_update(G, get(L, s), get(Lm, s), zp, L, s, raw(c, true), nP, nS);
                 write_cell(G, Po, So, Pin, i, j, k, nP, nS);
             }
             __syncthreads();
-            zm = zc;
-            zc = zp;
         }
         return;
     }
@@ -540,7 +542,7 @@ __device__ void interior_tile(const FusedGrid& G, long long lb, int parity, char
     }
 }
 
-constexpr size_t LDS_BYTES = 5 * SN * sizeof(double);  // one FP64 tile (two mixed ones fit)
+constexpr size_t LDS_BYTES = 2 * 5 * SN * sizeof(double);  // two tiles of the FP64 layout
 
 template <bool MIXED>
 __global__ __launch_bounds__(FT) void k_fused(const FusedGrid* __restrict__ grids,

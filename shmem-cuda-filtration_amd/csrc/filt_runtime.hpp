@@ -25,6 +25,8 @@ int check_shape(const filt_params* p, int localNx);
 // z-marching stencil launchers (filt_stencil.hip), for extents >= 3 and
 // arrays below 4 GiB (tiled_fits).
 bool tiled_fits(int nx, int ny, int nz);
+// False after filt_set_tile_config(kernel, -1, ...): kernel 0 roS, 1 u.
+bool tiled_enabled(int kernel);
 int launch_roS_zm(const filt_fields& f, const filt_params& p, int nx);
 int launch_u_zm(const filt_fields& f, const filt_params& p, int nx, int rank, int size);
 

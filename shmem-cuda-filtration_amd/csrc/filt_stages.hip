@@ -494,9 +494,10 @@ dim3 cell_grid(const filt_params* p, int nx)
 
 // The z-marching stencil kernels (filt_stencil.hip) need every extent >= 3;
 // smaller grids take the one-cell-per-thread kernels above.
-bool use_tiled(const filt_params* p, int nx)
+bool use_tiled(const filt_params* p, int nx, int kernel)
 {
-    return nx >= 3 && p->Ny >= 3 && p->Nz >= 3 && tiled_fits(nx, p->Ny, p->Nz);
+    return nx >= 3 && p->Ny >= 3 && p->Nz >= 3 && tiled_fits(nx, p->Ny, p->Nz) &&
+           tiled_enabled(kernel);
 }
 
 }  // namespace
@@ -519,7 +520,7 @@ int filt_u_calculation(const filt_fields* dev, const filt_params* p, int localNx
 {
     int rc = launch_check(dev, p, localNx);
     if (rc) return rc;
-    if (use_tiled(p, localNx) && p->NX > 2)
+    if (use_tiled(p, localNx, 1) && p->NX > 2)
         return launch_u_zm(*dev, *p, localNx, rank, size);
     hipLaunchKernelGGL(k_u, cell_grid(p, localNx), dim3(TX, TY), 0, stream(), *dev, *p, localNx,
                        rank, size);
@@ -541,7 +542,7 @@ int filt_roS_calculation(const filt_fields* dev, const filt_params* p, double t,
         return FILT_OK;
     }
     // t is unused by the upwind scheme, as by assign_rS_nr (cpu.cpp:218)
-    if (use_tiled(p, localNx))
+    if (use_tiled(p, localNx, 0))
         return launch_roS_zm(*dev, *p, localNx);
     hipLaunchKernelGGL(k_roS, cell_grid(p, localNx), dim3(TX, TY), 0, stream(), *dev, *p, localNx,
                        rank, size);

@@ -405,6 +405,10 @@ constexpr TileConfig kRoSDefault = {64, 4, 16, 1};
 constexpr TileConfig kUDefault = {64, 8, 64, 1};
 TileConfig g_roS_cfg = kRoSDefault;
 TileConfig g_u_cfg = kUDefault;
+// Per kernel (0 roS, 1 u): run the one-cell-per-thread kernels of
+// filt_stages.hip even where the tiled ones apply (filt_set_tile_config
+// with tx = -1).
+bool g_untiled[2] = {false, false};
 
 // Autotuned configurations by grid shape (filt_autotune_stencil).
 std::mutex g_tune_mu;
@@ -431,6 +435,12 @@ TileConfig cfg_for(bool roS, int nx, int ny, int nz)
 bool tiled_fits(int nx, int ny, int nz)
 {
     return (double)nx * ny * nz * 8.0 < 4294967296.0 - 65536.0;
+}
+
+bool tiled_enabled(int kernel)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    return !g_untiled[kernel];
 }
 
 // Launchers used by filt_stages.hip (grids with NX, Ny, Nz >= 3).
@@ -461,6 +471,11 @@ extern "C" int filt_set_tile_config(int kernel, int tx, int ty, int kz, int orde
     filt::TileConfig t = {tx, ty, kz, order};
     if (kernel != 0 && kernel != 1)
         return filt::set_error(FILT_ERR_INVALID, "kernel must be 0 (roS) or 1 (u)");
+    if (tx == -1) {  // the untiled kernels
+        std::lock_guard<std::mutex> lk(filt::g_tune_mu);
+        filt::g_untiled[kernel] = true;
+        return FILT_OK;
+    }
     if (tx == 0)  // back to the built-in default
         t = kernel == 0 ? filt::kRoSDefault : filt::kUDefault;
     if (!filt::valid_cfg(t, kernel == 0))
@@ -468,6 +483,7 @@ extern "C" int filt_set_tile_config(int kernel, int tx, int ty, int kz, int orde
                                kz, order);
     std::lock_guard<std::mutex> lk(filt::g_tune_mu);
     (kernel == 0 ? filt::g_roS_cfg : filt::g_u_cfg) = t;
+    filt::g_untiled[kernel] = false;
     filt::g_tuned.clear();
     return FILT_OK;
 }

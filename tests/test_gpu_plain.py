@@ -258,3 +258,24 @@ def test_exchange_pack_unpack_match_reference(hip, ref, rank, size):
         assert np.array_equal(got, want)
     finally:
         d.free()
+
+
+@pytest.mark.parametrize("params", ["ref", "bl"])
+@pytest.mark.parametrize("dims", [(33, 9, 7), (70, 19, 6)])
+def test_untiled_stencils_3d_bitwise(hip, orc, params, dims):
+    """The one-cell-per-thread u / roS kernels (the path of arrays of 4 GiB
+    and more) on 3D grids, selected with filt_set_tile_config(kernel, -1):
+    the oracle's bits, like the tiled kernels."""
+    lib = hip.lib
+    p = ref_params(*dims) if params == "ref" else bl_params(*dims)
+    g = random_state(p, np.random.default_rng(dims[0] + 5))
+    try:
+        for kern in (0, 1):
+            assert lib.filt_set_tile_config(kern, -1, 0, 0, 0) == 0
+        for stage in (2, 3):
+            out, want = run_stage_pair(hip, orc, p, g, stage)
+            for n in FIELD_NAMES_ALL:
+                assert np.array_equal(out[n], want[n], equal_nan=True), (n, stage, dims)
+    finally:
+        for kern in (0, 1):
+            lib.filt_set_tile_config(kern, 0, 0, 0, 0)
