@@ -19,6 +19,7 @@
 #include "../../include/filtration_hip.h"
 #include "filt_fused.hpp"
 #include "filt_runtime.hpp"
+#include "../host/filt_guard.hpp"
 
 using namespace filt;
 
@@ -130,22 +131,24 @@ extern "C" {
 
 int filt_ensemble_create(filt_ensemble** out, int precision)
 {
-    if (!out)
-        return set_error(FILT_ERR_INVALID, "null out");
-    if (precision != FILT_PREC_FP64 && precision != FILT_PREC_MIXED)
-        return set_error(FILT_ERR_INVALID, "precision must be FILT_PREC_FP64 or FILT_PREC_MIXED");
-    filt_ensemble* e = new filt_ensemble;
-    e->precision = precision;
-    hipError_t r = hipMalloc((void**)&e->d_mass, 2 * sizeof(double));
-    if (r == hipSuccess)
-        r = hipMalloc((void**)&e->d_one, sizeof(FusedGrid));
-    if (r != hipSuccess) {
-        if (e->d_mass) (void)hipFree(e->d_mass);
-        delete e;
-        return hip_fail(r, "filt_ensemble_create");
-    }
-    *out = e;
-    return FILT_OK;
+    return filt::guarded([&]() -> int {
+        if (!out)
+            return set_error(FILT_ERR_INVALID, "null out");
+        if (precision != FILT_PREC_FP64 && precision != FILT_PREC_MIXED)
+            return set_error(FILT_ERR_INVALID, "precision must be FILT_PREC_FP64 or FILT_PREC_MIXED");
+        filt_ensemble* e = new filt_ensemble;
+        e->precision = precision;
+        hipError_t r = hipMalloc((void**)&e->d_mass, 2 * sizeof(double));
+        if (r == hipSuccess)
+            r = hipMalloc((void**)&e->d_one, sizeof(FusedGrid));
+        if (r != hipSuccess) {
+            if (e->d_mass) (void)hipFree(e->d_mass);
+            delete e;
+            return hip_fail(r, "filt_ensemble_create");
+        }
+        *out = e;
+        return FILT_OK;
+    });
 }
 
 int filt_ensemble_destroy(filt_ensemble* e)
@@ -190,100 +193,104 @@ size_t filt_ensemble_member_bytes(const filt_params* p, int precision, int with_
 int filt_ensemble_add(filt_ensemble* e, const filt_params* p, const double* P1, const double* S2,
                       const double* perm, const double* poro, const int* media, int* id)
 {
-    if (!e || !p || !P1 || !S2)
-        return set_error(FILT_ERR_INVALID, "null argument");
-    if (p->NX < 3 || p->Ny < 3 || p->Nz < 3)
-        return set_error(FILT_ERR_UNSUPPORTED, "the fused step needs every extent >= 3");
-    if (p->roS_scheme != FILT_SCHEME_UPWIND)
-        return set_error(FILT_ERR_UNSUPPORTED,
-                         "the fused step implements the upwind roS scheme only (the divgrad "
-                         "scheme runs on the stage path)");
-    if ((double)p->NX * p->Ny * p->Nz > 2.0e9)
-        return set_error(FILT_ERR_INVALID, "grid too large");
-    if (p->n_media < 0 || p->n_media > FILT_MAX_MEDIA || p->newton_iterations < 0)
-        return set_error(FILT_ERR_INVALID, "bad n_media / newton_iterations");
-    const size_t n = (size_t)p->NX * p->Ny * p->Nz;
-    const bool mixed = e->precision == FILT_PREC_MIXED;
-    const size_t bytes = filt_ensemble_member_bytes(p, e->precision, perm != nullptr,
-                                                    poro != nullptr, media != nullptr);
-    char* base = nullptr;
-    FILT_HIP_TRY(hipMalloc((void**)&base, bytes));
-    filt_ensemble::Member m;
-    m.id = e->next_id++;
-    m.block = base;
-    m.bytes = bytes;
-    memset(&m.g, 0, sizeof m.g);
-    FusedGrid& g = m.g;
-    g.p = *p;
-    g.nx = p->NX;
-    g.ny = p->Ny;
-    g.nz = p->Nz;
-    size_t off = 0;
-    for (int b = 0; b < 2; b++) {
-        g.P[b] = (double*)(base + off);
-        off += round256(n * 8);
-        g.S[b] = (double*)(base + off);
-        off += round256(n * 8);
-    }
-    hipError_t r = hipSuccess;
-    auto put = [&](void* dst, const void* src, size_t nb) {
-        if (r == hipSuccess)
-            r = hipMemcpy(dst, src, nb, hipMemcpyHostToDevice);
-    };
-    put(g.P[e->parity], P1, n * 8);
-    put(g.S[e->parity], S2, n * 8);
-    std::vector<float> tmp;
-    for (int which = 0; which < 2; which++) {
-        const double* src = which == 0 ? perm : poro;
-        if (!src)
-            continue;
-        if (mixed) {
-            tmp.resize(n);
-            for (size_t c = 0; c < n; c++)
-                tmp[c] = (float)src[c];
-            float* d = (float*)(base + off);
-            put(d, tmp.data(), n * 4);
-            (which == 0 ? g.perm32 : g.poro32) = d;
-            off += round256(n * 4);
-        } else {
-            double* d = (double*)(base + off);
-            put(d, src, n * 8);
-            (which == 0 ? g.perm : g.poro) = d;
+    return filt::guarded([&]() -> int {
+        if (!e || !p || !P1 || !S2)
+            return set_error(FILT_ERR_INVALID, "null argument");
+        if (p->NX < 3 || p->Ny < 3 || p->Nz < 3)
+            return set_error(FILT_ERR_UNSUPPORTED, "the fused step needs every extent >= 3");
+        if (p->roS_scheme != FILT_SCHEME_UPWIND)
+            return set_error(FILT_ERR_UNSUPPORTED,
+                             "the fused step implements the upwind roS scheme only (the divgrad "
+                             "scheme runs on the stage path)");
+        if ((double)p->NX * p->Ny * p->Nz > 2.0e9)
+            return set_error(FILT_ERR_INVALID, "grid too large");
+        if (p->n_media < 0 || p->n_media > FILT_MAX_MEDIA || p->newton_iterations < 0)
+            return set_error(FILT_ERR_INVALID, "bad n_media / newton_iterations");
+        const size_t n = (size_t)p->NX * p->Ny * p->Nz;
+        const bool mixed = e->precision == FILT_PREC_MIXED;
+        const size_t bytes = filt_ensemble_member_bytes(p, e->precision, perm != nullptr,
+                                                        poro != nullptr, media != nullptr);
+        char* base = nullptr;
+        FILT_HIP_TRY(hipMalloc((void**)&base, bytes));
+        filt_ensemble::Member m;
+        m.id = e->next_id++;
+        m.block = base;
+        m.bytes = bytes;
+        memset(&m.g, 0, sizeof m.g);
+        FusedGrid& g = m.g;
+        g.p = *p;
+        g.nx = p->NX;
+        g.ny = p->Ny;
+        g.nz = p->Nz;
+        size_t off = 0;
+        for (int b = 0; b < 2; b++) {
+            g.P[b] = (double*)(base + off);
+            off += round256(n * 8);
+            g.S[b] = (double*)(base + off);
             off += round256(n * 8);
         }
-    }
-    if (media) {
-        int* d = (int*)(base + off);
-        put(d, media, n * 4);
-        g.media = d;
-        off += round256(n * 4);
-    }
-    if (r != hipSuccess) {
-        (void)hipFree(base);
-        return hip_fail(r, "filt_ensemble_add upload");
-    }
-    e->members.push_back(m);
-    e->dirty = true;
-    drop_graph(e);
-    if (id)
-        *id = m.id;
-    return FILT_OK;
+        hipError_t r = hipSuccess;
+        auto put = [&](void* dst, const void* src, size_t nb) {
+            if (r == hipSuccess)
+                r = hipMemcpy(dst, src, nb, hipMemcpyHostToDevice);
+        };
+        put(g.P[e->parity], P1, n * 8);
+        put(g.S[e->parity], S2, n * 8);
+        std::vector<float> tmp;
+        for (int which = 0; which < 2; which++) {
+            const double* src = which == 0 ? perm : poro;
+            if (!src)
+                continue;
+            if (mixed) {
+                tmp.resize(n);
+                for (size_t c = 0; c < n; c++)
+                    tmp[c] = (float)src[c];
+                float* d = (float*)(base + off);
+                put(d, tmp.data(), n * 4);
+                (which == 0 ? g.perm32 : g.poro32) = d;
+                off += round256(n * 4);
+            } else {
+                double* d = (double*)(base + off);
+                put(d, src, n * 8);
+                (which == 0 ? g.perm : g.poro) = d;
+                off += round256(n * 8);
+            }
+        }
+        if (media) {
+            int* d = (int*)(base + off);
+            put(d, media, n * 4);
+            g.media = d;
+            off += round256(n * 4);
+        }
+        if (r != hipSuccess) {
+            (void)hipFree(base);
+            return hip_fail(r, "filt_ensemble_add upload");
+        }
+        e->members.push_back(m);
+        e->dirty = true;
+        drop_graph(e);
+        if (id)
+            *id = m.id;
+        return FILT_OK;
+    });
 }
 
 int filt_ensemble_remove(filt_ensemble* e, int id)
 {
-    if (!e)
-        return set_error(FILT_ERR_INVALID, "null ensemble");
-    for (size_t q = 0; q < e->members.size(); q++)
-        if (e->members[q].id == id) {
-            FILT_HIP_TRY(hipStreamSynchronize(stream()));
-            (void)hipFree(e->members[q].block);
-            e->members.erase(e->members.begin() + q);
-            e->dirty = true;
-            drop_graph(e);
-            return FILT_OK;
-        }
-    return set_error(FILT_ERR_NOT_FOUND, "no member %d", id);
+    return filt::guarded([&]() -> int {
+        if (!e)
+            return set_error(FILT_ERR_INVALID, "null ensemble");
+        for (size_t q = 0; q < e->members.size(); q++)
+            if (e->members[q].id == id) {
+                FILT_HIP_TRY(hipStreamSynchronize(stream()));
+                (void)hipFree(e->members[q].block);
+                e->members.erase(e->members.begin() + q);
+                e->dirty = true;
+                drop_graph(e);
+                return FILT_OK;
+            }
+        return set_error(FILT_ERR_NOT_FOUND, "no member %d", id);
+    });
 }
 
 int filt_ensemble_count(const filt_ensemble* e) { return e ? (int)e->members.size() : 0; }
@@ -299,60 +306,64 @@ long long filt_ensemble_cells(const filt_ensemble* e)
 
 int filt_ensemble_step(filt_ensemble* e, int nsteps)
 {
-    if (!e || nsteps < 0)
-        return set_error(FILT_ERR_INVALID, "bad arguments");
-    int rc = sync_tables(e);
-    if (rc)
-        return rc;
-    for (int s = 0; s < nsteps; s++) {
-        rc = fused_launch(e->d_grids, e->d_block_grid, e->total_blocks, e->parity, e->precision,
-                          e->order, stream());
+    return filt::guarded([&]() -> int {
+        if (!e || nsteps < 0)
+            return set_error(FILT_ERR_INVALID, "bad arguments");
+        int rc = sync_tables(e);
         if (rc)
             return rc;
-        e->parity ^= 1;
-    }
-    for (auto& m : e->members)
-        m.steps += nsteps;
-    return FILT_OK;
+        for (int s = 0; s < nsteps; s++) {
+            rc = fused_launch(e->d_grids, e->d_block_grid, e->total_blocks, e->parity, e->precision,
+                              e->order, stream());
+            if (rc)
+                return rc;
+            e->parity ^= 1;
+        }
+        for (auto& m : e->members)
+            m.steps += nsteps;
+        return FILT_OK;
+    });
 }
 
 int filt_ensemble_capture(filt_ensemble* e, int nsteps)
 {
-    if (!e || nsteps < 1)
-        return set_error(FILT_ERR_INVALID, "bad arguments");
-    int rc = sync_tables(e);
-    if (rc)
-        return rc;
-    drop_graph(e);
-    FILT_HIP_TRY(hipStreamSynchronize(stream()));
-    for (int start = 0; start < 2; start++) {
-        FILT_HIP_TRY(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
-        int par = start;
-        for (int s = 0; s < nsteps && rc == FILT_OK; s++) {
-            rc = fused_launch(e->d_grids, e->d_block_grid, e->total_blocks, par, e->precision,
-                              e->order, stream());
-            par ^= 1;
-        }
-        hipGraph_t g = nullptr;
-        hipError_t r = hipStreamEndCapture(stream(), &g);
-        if (rc) {
-            if (g) (void)hipGraphDestroy(g);
-            drop_graph(e);
+    return filt::guarded([&]() -> int {
+        if (!e || nsteps < 1)
+            return set_error(FILT_ERR_INVALID, "bad arguments");
+        int rc = sync_tables(e);
+        if (rc)
             return rc;
+        drop_graph(e);
+        FILT_HIP_TRY(hipStreamSynchronize(stream()));
+        for (int start = 0; start < 2; start++) {
+            FILT_HIP_TRY(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
+            int par = start;
+            for (int s = 0; s < nsteps && rc == FILT_OK; s++) {
+                rc = fused_launch(e->d_grids, e->d_block_grid, e->total_blocks, par, e->precision,
+                                  e->order, stream());
+                par ^= 1;
+            }
+            hipGraph_t g = nullptr;
+            hipError_t r = hipStreamEndCapture(stream(), &g);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                drop_graph(e);
+                return rc;
+            }
+            if (r != hipSuccess) {
+                drop_graph(e);
+                return hip_fail(r, "hipStreamEndCapture");
+            }
+            r = hipGraphInstantiate(&e->exec[start], g, nullptr, nullptr, 0);
+            e->graph[start] = g;
+            if (r != hipSuccess) {
+                drop_graph(e);
+                return hip_fail(r, "hipGraphInstantiate");
+            }
         }
-        if (r != hipSuccess) {
-            drop_graph(e);
-            return hip_fail(r, "hipStreamEndCapture");
-        }
-        r = hipGraphInstantiate(&e->exec[start], g, nullptr, nullptr, 0);
-        e->graph[start] = g;
-        if (r != hipSuccess) {
-            drop_graph(e);
-            return hip_fail(r, "hipGraphInstantiate");
-        }
-    }
-    e->graph_steps = nsteps;
-    return FILT_OK;
+        e->graph_steps = nsteps;
+        return FILT_OK;
+    });
 }
 
 int filt_ensemble_replay(filt_ensemble* e)
@@ -442,44 +453,46 @@ int filt_device_free_bytes(size_t* free_b, size_t* total_b)
 int filt_ensemble_autotune(filt_ensemble* e, int reps, int* best_kz, int* best_order,
                            double* best_ms)
 {
-    if (!e || reps < 1)
-        return set_error(FILT_ERR_INVALID, "bad arguments");
-    static const int kzs[] = {4, 8, 16, 32, 64};
-    hipEvent_t a, b;
-    FILT_HIP_TRY(hipEventCreate(&a));
-    FILT_HIP_TRY(hipEventCreate(&b));
-    double best = 1e30;
-    int bk = e->kz, bo = e->order;
-    for (int kz : kzs)
-        for (int order = 0; order < 2; order++) {
-            int rc = filt_ensemble_set_tiling(e, kz, order);
-            if (!rc) rc = filt_ensemble_step(e, 1);  // warm
-            if (rc) {
-                (void)hipEventDestroy(a);
-                (void)hipEventDestroy(b);
-                return rc;
+    return filt::guarded([&]() -> int {
+        if (!e || reps < 1)
+            return set_error(FILT_ERR_INVALID, "bad arguments");
+        static const int kzs[] = {4, 8, 16, 32, 64};
+        hipEvent_t a, b;
+        FILT_HIP_TRY(hipEventCreate(&a));
+        FILT_HIP_TRY(hipEventCreate(&b));
+        double best = 1e30;
+        int bk = e->kz, bo = e->order;
+        for (int kz : kzs)
+            for (int order = 0; order < 2; order++) {
+                int rc = filt_ensemble_set_tiling(e, kz, order);
+                if (!rc) rc = filt_ensemble_step(e, 1);  // warm
+                if (rc) {
+                    (void)hipEventDestroy(a);
+                    (void)hipEventDestroy(b);
+                    return rc;
+                }
+                FILT_HIP_TRY(hipEventRecord(a, stream()));
+                rc = filt_ensemble_step(e, reps);
+                if (rc)
+                    return rc;
+                FILT_HIP_TRY(hipEventRecord(b, stream()));
+                FILT_HIP_TRY(hipEventSynchronize(b));
+                float ms = 0;
+                FILT_HIP_TRY(hipEventElapsedTime(&ms, a, b));
+                if (ms / reps < best) {
+                    best = ms / reps;
+                    bk = kz;
+                    bo = order;
+                }
             }
-            FILT_HIP_TRY(hipEventRecord(a, stream()));
-            rc = filt_ensemble_step(e, reps);
-            if (rc)
-                return rc;
-            FILT_HIP_TRY(hipEventRecord(b, stream()));
-            FILT_HIP_TRY(hipEventSynchronize(b));
-            float ms = 0;
-            FILT_HIP_TRY(hipEventElapsedTime(&ms, a, b));
-            if (ms / reps < best) {
-                best = ms / reps;
-                bk = kz;
-                bo = order;
-            }
-        }
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
-    int rc = filt_ensemble_set_tiling(e, bk, bo);
-    if (best_kz) *best_kz = bk;
-    if (best_order) *best_order = bo;
-    if (best_ms) *best_ms = best;
-    return rc;
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        int rc = filt_ensemble_set_tiling(e, bk, bo);
+        if (best_kz) *best_kz = bk;
+        if (best_order) *best_order = bo;
+        if (best_ms) *best_ms = best;
+        return rc;
+    });
 }
 
 }  // extern "C"

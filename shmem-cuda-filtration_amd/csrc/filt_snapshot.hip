@@ -19,6 +19,7 @@
 #include "../../include/filtration_hip.h"
 #include "../host/filt_io.hpp"
 #include "filt_runtime.hpp"
+#include "../host/filt_guard.hpp"
 
 namespace {
 
@@ -120,48 +121,50 @@ extern "C" {
 int filt_snapshot_create(filt_snapshot** out, const filt_params* p, int localNx, int rank,
                          int size, int what)
 {
-    using namespace filt;
-    if (!out)
-        return set_error(FILT_ERR_INVALID, "null handle");
-    *out = nullptr;
-    int rc = check_shape(p, localNx);
-    if (rc)
-        return rc;
-    if (!(what & (FILT_SNAP_SAVE | FILT_SNAP_PLOTS)) || (what & ~(FILT_SNAP_SAVE | FILT_SNAP_PLOTS)))
-        return set_error(FILT_ERR_INVALID, "what must combine FILT_SNAP_SAVE and FILT_SNAP_PLOTS");
-    if (size < 1 || rank < 0 || rank >= size)
-        return set_error(FILT_ERR_INVALID, "bad rank / size");
-    filt_snapshot* s = new filt_snapshot();
-    s->p = *p;
-    s->nx = localNx, s->rank = rank, s->size = size, s->what = what;
-    s->n = (size_t)localNx * p->Ny * p->Nz;
-    s->has[P1] = s->has[S2] = true;
-    s->has[R1] = s->has[R2] = (what & FILT_SNAP_SAVE) != 0;
-    s->has[UX] = s->has[UY] = s->has[UZ] = (what & FILT_SNAP_PLOTS) != 0;
-    size_t o = 0;
-    for (int q = 0; q < NSLOT; q++) {
-        s->off[q] = o;
-        if (s->has[q])
-            o += (s->n * sizeof(double) + 255) & ~(size_t)255;
-    }
-    s->off[NSLOT] = o;
-    s->bytes = o + s->n * sizeof(int);
-    hipError_t e = hipGetDevice(&s->device);
-    if (e == hipSuccess)
-        e = hipMalloc((void**)&s->dbuf, s->bytes);
-    if (e == hipSuccess)
-        e = hipHostMalloc((void**)&s->hbuf, s->bytes, hipHostMallocDefault);
-    if (e == hipSuccess)
-        e = hipEventCreateWithFlags(&s->taken, hipEventDisableTiming);
-    if (e == hipSuccess)
-        e = hipStreamCreate(&s->io);  // default priority
-    if (e != hipSuccess) {
-        filt_snapshot_destroy(s);
-        return e == hipErrorOutOfMemory ? set_error(FILT_ERR_NOMEM, "snapshot buffers")
-                                        : hip_fail(e, "filt_snapshot_create");
-    }
-    *out = s;
-    return FILT_OK;
+    return filt::guarded([&]() -> int {
+        using namespace filt;
+        if (!out)
+            return set_error(FILT_ERR_INVALID, "null handle");
+        *out = nullptr;
+        int rc = check_shape(p, localNx);
+        if (rc)
+            return rc;
+        if (!(what & (FILT_SNAP_SAVE | FILT_SNAP_PLOTS)) || (what & ~(FILT_SNAP_SAVE | FILT_SNAP_PLOTS)))
+            return set_error(FILT_ERR_INVALID, "what must combine FILT_SNAP_SAVE and FILT_SNAP_PLOTS");
+        if (size < 1 || rank < 0 || rank >= size)
+            return set_error(FILT_ERR_INVALID, "bad rank / size");
+        filt_snapshot* s = new filt_snapshot();
+        s->p = *p;
+        s->nx = localNx, s->rank = rank, s->size = size, s->what = what;
+        s->n = (size_t)localNx * p->Ny * p->Nz;
+        s->has[P1] = s->has[S2] = true;
+        s->has[R1] = s->has[R2] = (what & FILT_SNAP_SAVE) != 0;
+        s->has[UX] = s->has[UY] = s->has[UZ] = (what & FILT_SNAP_PLOTS) != 0;
+        size_t o = 0;
+        for (int q = 0; q < NSLOT; q++) {
+            s->off[q] = o;
+            if (s->has[q])
+                o += (s->n * sizeof(double) + 255) & ~(size_t)255;
+        }
+        s->off[NSLOT] = o;
+        s->bytes = o + s->n * sizeof(int);
+        hipError_t e = hipGetDevice(&s->device);
+        if (e == hipSuccess)
+            e = hipMalloc((void**)&s->dbuf, s->bytes);
+        if (e == hipSuccess)
+            e = hipHostMalloc((void**)&s->hbuf, s->bytes, hipHostMallocDefault);
+        if (e == hipSuccess)
+            e = hipEventCreateWithFlags(&s->taken, hipEventDisableTiming);
+        if (e == hipSuccess)
+            e = hipStreamCreate(&s->io);  // default priority
+        if (e != hipSuccess) {
+            filt_snapshot_destroy(s);
+            return e == hipErrorOutOfMemory ? set_error(FILT_ERR_NOMEM, "snapshot buffers")
+                                            : hip_fail(e, "filt_snapshot_create");
+        }
+        *out = s;
+        return FILT_OK;
+    });
 }
 
 int filt_snapshot_destroy(filt_snapshot* s)
@@ -217,26 +220,28 @@ int filt_snapshot_take(filt_snapshot* s, const filt_fields* dev, int j, double t
 int filt_snapshot_write_async(filt_snapshot* s, const char* save_path, int truncate,
                               const char* plot_dir)
 {
-    using namespace filt;
-    if (!s)
-        return set_error(FILT_ERR_INVALID, "null snapshot");
-    if (!s->valid)
-        return set_error(FILT_ERR_INVALID, "nothing taken yet");
-    if (save_path && !(s->what & FILT_SNAP_SAVE))
-        return set_error(FILT_ERR_INVALID, "snapshot made without FILT_SNAP_SAVE");
-    if (plot_dir && !(s->what & FILT_SNAP_PLOTS))
-        return set_error(FILT_ERR_INVALID, "snapshot made without FILT_SNAP_PLOTS");
-    join(s);
-    const std::string sp = save_path ? save_path : "", pd = plot_dir ? plot_dir : "";
-    const bool tr = truncate != 0;
-    s->status = FILT_OK;
-    s->busy = true;
-    s->writer = std::thread([s, sp, tr, pd] {
-        s->status = write_files(s, sp, tr, pd);
-        if (s->status)
-            s->err = filt_last_error();
+    return filt::guarded([&]() -> int {
+        using namespace filt;
+        if (!s)
+            return set_error(FILT_ERR_INVALID, "null snapshot");
+        if (!s->valid)
+            return set_error(FILT_ERR_INVALID, "nothing taken yet");
+        if (save_path && !(s->what & FILT_SNAP_SAVE))
+            return set_error(FILT_ERR_INVALID, "snapshot made without FILT_SNAP_SAVE");
+        if (plot_dir && !(s->what & FILT_SNAP_PLOTS))
+            return set_error(FILT_ERR_INVALID, "snapshot made without FILT_SNAP_PLOTS");
+        join(s);
+        const std::string sp = save_path ? save_path : "", pd = plot_dir ? plot_dir : "";
+        const bool tr = truncate != 0;
+        s->status = FILT_OK;
+        s->busy = true;
+        s->writer = std::thread([s, sp, tr, pd] {
+            s->status = filt::guarded([&] { return write_files(s, sp, tr, pd); });
+            if (s->status)
+                s->err = filt_last_error();
+        });
+        return FILT_OK;
     });
-    return FILT_OK;
 }
 
 int filt_snapshot_wait(filt_snapshot* s)

@@ -31,6 +31,7 @@
 #include "filt_device.hpp"
 #include "filt_runtime.hpp"
 #include "filt_tiles.hpp"
+#include "../host/filt_guard.hpp"
 
 namespace filt {
 
@@ -468,24 +469,26 @@ int launch_u_zm(const filt_fields& f, const filt_params& p, int nx, int rank, in
 
 extern "C" int filt_set_tile_config(int kernel, int tx, int ty, int kz, int order)
 {
-    filt::TileConfig t = {tx, ty, kz, order};
-    if (kernel != 0 && kernel != 1)
-        return filt::set_error(FILT_ERR_INVALID, "kernel must be 0 (roS) or 1 (u)");
-    if (tx == -1) {  // the untiled kernels
+    return filt::guarded([&]() -> int {
+        filt::TileConfig t = {tx, ty, kz, order};
+        if (kernel != 0 && kernel != 1)
+            return filt::set_error(FILT_ERR_INVALID, "kernel must be 0 (roS) or 1 (u)");
+        if (tx == -1) {  // the untiled kernels
+            std::lock_guard<std::mutex> lk(filt::g_tune_mu);
+            filt::g_untiled[kernel] = true;
+            return FILT_OK;
+        }
+        if (tx == 0)  // back to the built-in default
+            t = kernel == 0 ? filt::kRoSDefault : filt::kUDefault;
+        if (!filt::valid_cfg(t, kernel == 0))
+            return filt::set_error(FILT_ERR_INVALID, "unsupported tile %dx%d kz=%d order=%d", tx, ty,
+                                   kz, order);
         std::lock_guard<std::mutex> lk(filt::g_tune_mu);
-        filt::g_untiled[kernel] = true;
+        (kernel == 0 ? filt::g_roS_cfg : filt::g_u_cfg) = t;
+        filt::g_untiled[kernel] = false;
+        filt::g_tuned.clear();
         return FILT_OK;
-    }
-    if (tx == 0)  // back to the built-in default
-        t = kernel == 0 ? filt::kRoSDefault : filt::kUDefault;
-    if (!filt::valid_cfg(t, kernel == 0))
-        return filt::set_error(FILT_ERR_INVALID, "unsupported tile %dx%d kz=%d order=%d", tx, ty,
-                               kz, order);
-    std::lock_guard<std::mutex> lk(filt::g_tune_mu);
-    (kernel == 0 ? filt::g_roS_cfg : filt::g_u_cfg) = t;
-    filt::g_untiled[kernel] = false;
-    filt::g_tuned.clear();
-    return FILT_OK;
+    });
 }
 
 // Time every candidate tile configuration of the two stencil kernels on the
@@ -497,68 +500,70 @@ extern "C" int filt_set_tile_config(int kernel, int tx, int ty, int kz, int orde
 extern "C" int filt_autotune_stencil(const filt_fields* dev, const filt_params* p, int localNx,
                                      int reps, double* best_ms)
 {
-    using namespace filt;
-    int rc = check_shape(p, localNx);
-    if (rc)
-        return rc;
-    if (!dev || !dev->P1)
-        return set_error(FILT_ERR_INVALID, "null device fields");
-    const int nx = localNx, ny = p->Ny, nz = p->Nz;
-    if (nx < 3 || ny < 3 || nz < 3 || !tiled_fits(nx, ny, nz))
-        return set_error(FILT_ERR_UNSUPPORTED, "grid does not use the tiled kernels");
-    if (reps < 1)
-        reps = 3;
-    static const int shapes[4][2] = {{64, 4}, {32, 8}, {16, 16}, {64, 8}};
-    std::vector<int> kzs = {8, 16, 32, 64};
-    if (nz > 64)
-        kzs.push_back(nz);
-    hipEvent_t e0, e1;
-    FILT_HIP_TRY(hipEventCreate(&e0));
-    FILT_HIP_TRY(hipEventCreate(&e1));
-    TileConfig best[2];
-    double best_t[2] = {1e30, 1e30};
-    for (int kern = 0; kern < 2; kern++)
-        for (auto& s : shapes)
-            for (int kz : kzs)
-                for (int order = 0; order < 2; order++) {
-                    const TileConfig t = {s[0], s[1], kz, order};
-                    const TileMap m = make_map(nx, ny, nz, t);
-                    std::vector<float> ms;
-                    for (int r = 0; r <= reps; r++) {
-                        FILT_HIP_TRY(hipEventRecord(e0, stream()));
-                        if (kern == 0)
-                            hipLaunchKernelGGL(pick_roS(t.tx, t.ty), dim3((unsigned)m.count()),
-                                               dim3(t.tx, t.ty), 0, stream(), *dev, *p, nx, m, t.kz);
-                        else
-                            hipLaunchKernelGGL(pick_u(t.tx, t.ty), dim3((unsigned)m.count()),
-                                               dim3(t.tx, t.ty), 0, stream(), *dev, *p, nx, 0, 1,
-                                               m, t.kz);
-                        FILT_LAUNCH_CHECK("autotune launch");
-                        FILT_HIP_TRY(hipEventRecord(e1, stream()));
-                        FILT_HIP_TRY(hipEventSynchronize(e1));
-                        float v = 0;
-                        FILT_HIP_TRY(hipEventElapsedTime(&v, e0, e1));
-                        if (r > 0)  // the first launch warms caches and code
-                            ms.push_back(v);
+    return filt::guarded([&]() -> int {
+        using namespace filt;
+        int rc = check_shape(p, localNx);
+        if (rc)
+            return rc;
+        if (!dev || !dev->P1)
+            return set_error(FILT_ERR_INVALID, "null device fields");
+        const int nx = localNx, ny = p->Ny, nz = p->Nz;
+        if (nx < 3 || ny < 3 || nz < 3 || !tiled_fits(nx, ny, nz))
+            return set_error(FILT_ERR_UNSUPPORTED, "grid does not use the tiled kernels");
+        if (reps < 1)
+            reps = 3;
+        static const int shapes[4][2] = {{64, 4}, {32, 8}, {16, 16}, {64, 8}};
+        std::vector<int> kzs = {8, 16, 32, 64};
+        if (nz > 64)
+            kzs.push_back(nz);
+        hipEvent_t e0, e1;
+        FILT_HIP_TRY(hipEventCreate(&e0));
+        FILT_HIP_TRY(hipEventCreate(&e1));
+        TileConfig best[2];
+        double best_t[2] = {1e30, 1e30};
+        for (int kern = 0; kern < 2; kern++)
+            for (auto& s : shapes)
+                for (int kz : kzs)
+                    for (int order = 0; order < 2; order++) {
+                        const TileConfig t = {s[0], s[1], kz, order};
+                        const TileMap m = make_map(nx, ny, nz, t);
+                        std::vector<float> ms;
+                        for (int r = 0; r <= reps; r++) {
+                            FILT_HIP_TRY(hipEventRecord(e0, stream()));
+                            if (kern == 0)
+                                hipLaunchKernelGGL(pick_roS(t.tx, t.ty), dim3((unsigned)m.count()),
+                                                   dim3(t.tx, t.ty), 0, stream(), *dev, *p, nx, m, t.kz);
+                            else
+                                hipLaunchKernelGGL(pick_u(t.tx, t.ty), dim3((unsigned)m.count()),
+                                                   dim3(t.tx, t.ty), 0, stream(), *dev, *p, nx, 0, 1,
+                                                   m, t.kz);
+                            FILT_LAUNCH_CHECK("autotune launch");
+                            FILT_HIP_TRY(hipEventRecord(e1, stream()));
+                            FILT_HIP_TRY(hipEventSynchronize(e1));
+                            float v = 0;
+                            FILT_HIP_TRY(hipEventElapsedTime(&v, e0, e1));
+                            if (r > 0)  // the first launch warms caches and code
+                                ms.push_back(v);
+                        }
+                        std::sort(ms.begin(), ms.end());
+                        const double med = ms[ms.size() / 2];
+                        if (med < best_t[kern]) {
+                            best_t[kern] = med;
+                            best[kern] = t;
+                        }
                     }
-                    std::sort(ms.begin(), ms.end());
-                    const double med = ms[ms.size() / 2];
-                    if (med < best_t[kern]) {
-                        best_t[kern] = med;
-                        best[kern] = t;
-                    }
-                }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    {
-        std::lock_guard<std::mutex> lk(g_tune_mu);
-        g_tuned[std::make_tuple(nx, ny, nz)] = std::make_pair(best[0], best[1]);
-    }
-    if (best_ms) {
-        best_ms[0] = best_t[0];
-        best_ms[1] = best_t[1];
-    }
-    return FILT_OK;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            g_tuned[std::make_tuple(nx, ny, nz)] = std::make_pair(best[0], best[1]);
+        }
+        if (best_ms) {
+            best_ms[0] = best_t[0];
+            best_ms[1] = best_t[1];
+        }
+        return FILT_OK;
+    });
 }
 
 extern "C" int filt_get_tile_config(int kernel, int nx, int ny, int nz, int* tx, int* ty, int* kz,

@@ -19,12 +19,14 @@
 #include <sys/stat.h>
 
 #include <algorithm>
+#include <atomic>
 #include <string>
 #include <thread>
 #include <vector>
 
 #include "../../include/filtration_hip.h"
 #include "filt_io.hpp"
+#include "filt_guard.hpp"
 
 namespace filt {
 int set_error(int code, const char* fmt, ...);
@@ -98,12 +100,21 @@ template <class Fn>
 void parallel_ranges(int n, int T, Fn fn)
 {
     std::vector<std::thread> th;
+    std::atomic<bool> failed{false};  // a worker's exception, rethrown below
     for (int t = 0; t < T; t++) {
         const int lo = (int)((long long)n * t / T), hi = (int)((long long)n * (t + 1) / T);
-        th.emplace_back([=] { fn(lo, hi, t); });
+        th.emplace_back([=, &failed] {
+            try {
+                fn(lo, hi, t);
+            } catch (...) {
+                failed = true;
+            }
+        });
     }
     for (auto& x : th)
         x.join();
+    if (failed)
+        throw std::bad_alloc();
 }
 
 struct PlotFiles {
@@ -366,130 +377,142 @@ extern "C" {
 int filt_save_host(const char* path, const filt_fields* h, const filt_params* p, int localNx,
                    int rank, int size, int j, int truncate)
 {
-    int rc = check_args(path, h, p, localNx, rank, size);
-    if (rc)
-        return rc;
-    if (!h->P1 || !h->S2 || !h->roS1_old || !h->roS2_old || !h->media)
-        return filt::set_error(FILT_ERR_INVALID, "P1, S2, roS1_old, roS2_old and media are needed");
-    const size_t n = (size_t)localNx * p->Ny * p->Nz;
-    std::vector<double> x, y, z;
-    if (!h->x || !h->y || !h->z)
-        coordinates(p, localNx, rank, size, x, y, z);
-    View v{};
-    v.P1 = h->P1, v.S2 = h->S2, v.roS1_old = h->roS1_old, v.roS2_old = h->roS2_old;
-    v.x = h->x ? h->x : x.data();
-    v.y = h->y ? h->y : y.data();
-    v.z = h->z ? h->z : z.data();
-    v.media = h->media;
-    return write_save(path, v, n, j, truncate != 0);
+    return filt::guarded([&]() -> int {
+        int rc = check_args(path, h, p, localNx, rank, size);
+        if (rc)
+            return rc;
+        if (!h->P1 || !h->S2 || !h->roS1_old || !h->roS2_old || !h->media)
+            return filt::set_error(FILT_ERR_INVALID, "P1, S2, roS1_old, roS2_old and media are needed");
+        const size_t n = (size_t)localNx * p->Ny * p->Nz;
+        std::vector<double> x, y, z;
+        if (!h->x || !h->y || !h->z)
+            coordinates(p, localNx, rank, size, x, y, z);
+        View v{};
+        v.P1 = h->P1, v.S2 = h->S2, v.roS1_old = h->roS1_old, v.roS2_old = h->roS2_old;
+        v.x = h->x ? h->x : x.data();
+        v.y = h->y ? h->y : y.data();
+        v.z = h->z ? h->z : z.data();
+        v.media = h->media;
+        return write_save(path, v, n, j, truncate != 0);
+    });
 }
 
 int filt_restore_host(const char* path, const filt_fields* h, const filt_params* p, int localNx,
                       int rank, int size, int* j)
 {
-    int rc = check_args(path, h, p, localNx, rank, size);
-    if (rc)
-        return rc;
-    int lnx = 0;
-    filt_N_to_local(&lnx, nullptr, size, rank, p);
-    if (lnx != localNx)
-        return filt::set_error(FILT_ERR_INVALID, "localNx %d is not rank %d's %d", localNx, rank, lnx);
-    if (!h->P1 || !h->S2 || !h->roS1_old || !h->roS2_old || !h->media)
-        return filt::set_error(FILT_ERR_INVALID, "P1, S2, roS1_old, roS2_old and media are needed");
-    Out o{h->P1, h->S2, h->x, h->y, h->z, h->roS1_old, h->roS2_old, h->media};
-    return read_save(path, p, rank, size, o, j);
+    return filt::guarded([&]() -> int {
+        int rc = check_args(path, h, p, localNx, rank, size);
+        if (rc)
+            return rc;
+        int lnx = 0;
+        filt_N_to_local(&lnx, nullptr, size, rank, p);
+        if (lnx != localNx)
+            return filt::set_error(FILT_ERR_INVALID, "localNx %d is not rank %d's %d", localNx, rank, lnx);
+        if (!h->P1 || !h->S2 || !h->roS1_old || !h->roS2_old || !h->media)
+            return filt::set_error(FILT_ERR_INVALID, "P1, S2, roS1_old, roS2_old and media are needed");
+        Out o{h->P1, h->S2, h->x, h->y, h->z, h->roS1_old, h->roS2_old, h->media};
+        return read_save(path, p, rank, size, o, j);
+    });
 }
 
 int filt_save_plots_host(const char* dir, const filt_fields* h, const filt_params* p, int localNx,
                          int rank, int size, double t)
 {
-    int rc = check_args(dir ? dir : ".", h, p, localNx, rank, size);
-    if (rc)
-        return rc;
-    if (!h->P1 || !h->S2 || !h->u2x || !h->u2y || !h->u2z || !h->media)
-        return filt::set_error(FILT_ERR_INVALID, "P1, S2, u2x, u2y, u2z and media are needed");
-    std::vector<double> x, y, z;
-    if (!h->x || !h->y || !h->z)
-        coordinates(p, localNx, rank, size, x, y, z);
-    View v{};
-    v.P1 = h->P1, v.S2 = h->S2, v.u2x = h->u2x, v.u2y = h->u2y, v.u2z = h->u2z;
-    v.x = h->x ? h->x : x.data();
-    v.y = h->y ? h->y : y.data();
-    v.z = h->z ? h->z : z.data();
-    v.media = h->media;
-    return write_plots(dir, v, p, localNx, rank, size, t);
+    return filt::guarded([&]() -> int {
+        int rc = check_args(dir ? dir : ".", h, p, localNx, rank, size);
+        if (rc)
+            return rc;
+        if (!h->P1 || !h->S2 || !h->u2x || !h->u2y || !h->u2z || !h->media)
+            return filt::set_error(FILT_ERR_INVALID, "P1, S2, u2x, u2y, u2z and media are needed");
+        std::vector<double> x, y, z;
+        if (!h->x || !h->y || !h->z)
+            coordinates(p, localNx, rank, size, x, y, z);
+        View v{};
+        v.P1 = h->P1, v.S2 = h->S2, v.u2x = h->u2x, v.u2y = h->u2y, v.u2z = h->u2z;
+        v.x = h->x ? h->x : x.data();
+        v.y = h->y ? h->y : y.data();
+        v.z = h->z ? h->z : z.data();
+        v.media = h->media;
+        return write_plots(dir, v, p, localNx, rank, size, t);
+    });
 }
 
 int filt_save(const char* path, const filt_fields* dev, const filt_params* p, int localNx,
               int rank, int size, int j, int truncate)
 {
-    int rc = check_args(path, dev, p, localNx, rank, size);
-    if (rc)
-        return rc;
-    const size_t n = (size_t)localNx * p->Ny * p->Nz;
-    std::vector<double> P1, S2, r1, r2, x, y, z;
-    std::vector<int> media;
-    if ((rc = fetch(dev->P1, P1, n, "P1")) || (rc = fetch(dev->S2, S2, n, "S2")) ||
-        (rc = fetch(dev->roS1_old, r1, n, "roS1_old")) ||
-        (rc = fetch(dev->roS2_old, r2, n, "roS2_old")) || (rc = fetch_int(dev->media, media, n)))
-        return rc;
-    coordinates(p, localNx, rank, size, x, y, z);
-    View v{};
-    v.P1 = P1.data(), v.S2 = S2.data(), v.roS1_old = r1.data(), v.roS2_old = r2.data();
-    v.x = x.data(), v.y = y.data(), v.z = z.data(), v.media = media.data();
-    return write_save(path, v, n, j, truncate != 0);
+    return filt::guarded([&]() -> int {
+        int rc = check_args(path, dev, p, localNx, rank, size);
+        if (rc)
+            return rc;
+        const size_t n = (size_t)localNx * p->Ny * p->Nz;
+        std::vector<double> P1, S2, r1, r2, x, y, z;
+        std::vector<int> media;
+        if ((rc = fetch(dev->P1, P1, n, "P1")) || (rc = fetch(dev->S2, S2, n, "S2")) ||
+            (rc = fetch(dev->roS1_old, r1, n, "roS1_old")) ||
+            (rc = fetch(dev->roS2_old, r2, n, "roS2_old")) || (rc = fetch_int(dev->media, media, n)))
+            return rc;
+        coordinates(p, localNx, rank, size, x, y, z);
+        View v{};
+        v.P1 = P1.data(), v.S2 = S2.data(), v.roS1_old = r1.data(), v.roS2_old = r2.data();
+        v.x = x.data(), v.y = y.data(), v.z = z.data(), v.media = media.data();
+        return write_save(path, v, n, j, truncate != 0);
+    });
 }
 
 int filt_restore(const char* path, const filt_fields* dev, const filt_params* p, int localNx,
                  int rank, int size, int* j)
 {
-    int rc = check_args(path, dev, p, localNx, rank, size);
-    if (rc)
-        return rc;
-    int lnx = 0;
-    filt_N_to_local(&lnx, nullptr, size, rank, p);
-    if (lnx != localNx)
-        return filt::set_error(FILT_ERR_INVALID, "localNx %d is not rank %d's %d", localNx, rank, lnx);
-    if (!dev->P1 || !dev->S2 || !dev->roS1_old || !dev->roS2_old)
-        return filt::set_error(FILT_ERR_INVALID, "P1, S2, roS1_old and roS2_old are needed");
-    const size_t n = (size_t)localNx * p->Ny * p->Nz;
-    std::vector<double> a[7];
-    for (auto& v : a)
-        v.resize(n);
-    std::vector<int> media(n);
-    Out o{a[0].data(), a[1].data(), a[2].data(), a[3].data(), a[4].data(),
-          a[5].data(), a[6].data(), media.data()};
-    if ((rc = read_save(path, p, rank, size, o, j)))
-        return rc;
-    // the arrays Initialize uploads after restore (main.cpp:224-228), then
-    // x, y, z where the device bundle has them
-    double* const to[7] = {dev->P1, dev->S2, dev->x, dev->y, dev->z, dev->roS1_old, dev->roS2_old};
-    for (int q = 0; q < 7; q++)
-        if (to[q] && (rc = filt_memcpy(to[q], a[q].data(), n * sizeof(double), 0)))
+    return filt::guarded([&]() -> int {
+        int rc = check_args(path, dev, p, localNx, rank, size);
+        if (rc)
             return rc;
-    if (dev->media && (rc = filt_memcpy(dev->media, media.data(), n * sizeof(int), 0)))
-        return rc;
-    return FILT_OK;
+        int lnx = 0;
+        filt_N_to_local(&lnx, nullptr, size, rank, p);
+        if (lnx != localNx)
+            return filt::set_error(FILT_ERR_INVALID, "localNx %d is not rank %d's %d", localNx, rank, lnx);
+        if (!dev->P1 || !dev->S2 || !dev->roS1_old || !dev->roS2_old)
+            return filt::set_error(FILT_ERR_INVALID, "P1, S2, roS1_old and roS2_old are needed");
+        const size_t n = (size_t)localNx * p->Ny * p->Nz;
+        std::vector<double> a[7];
+        for (auto& v : a)
+            v.resize(n);
+        std::vector<int> media(n);
+        Out o{a[0].data(), a[1].data(), a[2].data(), a[3].data(), a[4].data(),
+              a[5].data(), a[6].data(), media.data()};
+        if ((rc = read_save(path, p, rank, size, o, j)))
+            return rc;
+        // the arrays Initialize uploads after restore (main.cpp:224-228), then
+        // x, y, z where the device bundle has them
+        double* const to[7] = {dev->P1, dev->S2, dev->x, dev->y, dev->z, dev->roS1_old, dev->roS2_old};
+        for (int q = 0; q < 7; q++)
+            if (to[q] && (rc = filt_memcpy(to[q], a[q].data(), n * sizeof(double), 0)))
+                return rc;
+        if (dev->media && (rc = filt_memcpy(dev->media, media.data(), n * sizeof(int), 0)))
+            return rc;
+        return FILT_OK;
+    });
 }
 
 int filt_save_plots(const char* dir, const filt_fields* dev, const filt_params* p, int localNx,
                     int rank, int size, double t)
 {
-    int rc = check_args(dir ? dir : ".", dev, p, localNx, rank, size);
-    if (rc)
-        return rc;
-    const size_t n = (size_t)localNx * p->Ny * p->Nz;
-    std::vector<double> P1, S2, ux, uy, uz, x, y, z;
-    std::vector<int> media;
-    if ((rc = fetch(dev->P1, P1, n, "P1")) || (rc = fetch(dev->S2, S2, n, "S2")) ||
-        (rc = fetch(dev->u2x, ux, n, "u2x")) || (rc = fetch(dev->u2y, uy, n, "u2y")) ||
-        (rc = fetch(dev->u2z, uz, n, "u2z")) || (rc = fetch_int(dev->media, media, n)))
-        return rc;
-    coordinates(p, localNx, rank, size, x, y, z);
-    View v{};
-    v.P1 = P1.data(), v.S2 = S2.data(), v.u2x = ux.data(), v.u2y = uy.data(), v.u2z = uz.data();
-    v.x = x.data(), v.y = y.data(), v.z = z.data(), v.media = media.data();
-    return write_plots(dir, v, p, localNx, rank, size, t);
+    return filt::guarded([&]() -> int {
+        int rc = check_args(dir ? dir : ".", dev, p, localNx, rank, size);
+        if (rc)
+            return rc;
+        const size_t n = (size_t)localNx * p->Ny * p->Nz;
+        std::vector<double> P1, S2, ux, uy, uz, x, y, z;
+        std::vector<int> media;
+        if ((rc = fetch(dev->P1, P1, n, "P1")) || (rc = fetch(dev->S2, S2, n, "S2")) ||
+            (rc = fetch(dev->u2x, ux, n, "u2x")) || (rc = fetch(dev->u2y, uy, n, "u2y")) ||
+            (rc = fetch(dev->u2z, uz, n, "u2z")) || (rc = fetch_int(dev->media, media, n)))
+            return rc;
+        coordinates(p, localNx, rank, size, x, y, z);
+        View v{};
+        v.P1 = P1.data(), v.S2 = S2.data(), v.u2x = ux.data(), v.u2y = uy.data(), v.u2z = uz.data();
+        v.x = x.data(), v.y = y.data(), v.z = z.data(), v.media = media.data();
+        return write_plots(dir, v, p, localNx, rank, size, t);
+    });
 }
 
 }  // extern "C"

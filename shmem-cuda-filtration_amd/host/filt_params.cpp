@@ -8,6 +8,7 @@
 #include <string>
 
 #include "../../include/filtration_hip.h"
+#include "filt_guard.hpp"
 
 namespace {
 
@@ -77,69 +78,71 @@ void filt_params_finalize(filt_params* p)
 // ignored.  Lines without '=' are skipped.
 int filt_read_defines(const char* path, filt_params* p)
 {
-    if (!path || !p)
-        return FILT_ERR_INVALID;
-    FILE* fh = fopen(path, "rt");
-    if (!fh)
-        return FILT_ERR_IO;
-    char line[250];
-    bool explicit_patch = false;
-    while (fgets(line, sizeof line, fh)) {
-        if (line[0] == '#')
-            continue;
-        const char* eq = strchr(line, '=');
-        if (!eq)
-            continue;
-        const std::string key(line, eq - line);
-        const char* v0 = eq + 1;
-        const char* v1 = strchr(v0, ' ');
-        const std::string val = v1 ? std::string(v0, v1 - v0) : std::string(v0);
-        const char* s = val.c_str();
-        int q = 0;
-        // the 16 keys of the reference
-        if (key == "H1") p->h1 = atof(s);
-        else if (key == "H2") p->h2 = atof(s);
-        else if (key == "H3") p->h3 = atof(s);
-        else if (key == "TAU") p->tau = atof(s);
-        else if (key == "DT") p->dt = atof(s);
-        else if (key == "L_W") p->l_w = atof(s);
-        else if (key == "L_N") p->l_n = atof(s);
-        else if (key == "C") p->c = atof(s);
-        else if (key == "BETA1") p->beta1 = atof(s);
-        else if (key == "BETA2") p->beta2 = atof(s);
-        else if (key == "SOURCE") p->source = atoi(s);
-        else if (key == "SAVE_PLOTS") p->save_plots = atoi(s);
-        else if (key == "PRINT_SCREEN") p->print_screen = atoi(s);
-        else if (key == "NX") p->NX = atoi(s);
-        else if (key == "NY") p->Ny = atoi(s);
-        else if (key == "NZ") p->Nz = atoi(s);
-        // keys defines.ini documents (defines.ini:52-100)
-        else if (parse_indexed(key, "K", &q)) p->K[q] = atof(s);
-        else if (parse_indexed(key, "LAMBDA", &q)) p->lambda[q] = atof(s);
-        else if (parse_indexed(key, "S_WR", &q)) p->S_wr[q] = atof(s);
-        else if (parse_indexed(key, "M", &q)) p->m[q] = atof(s);
-        else if (parse_indexed(key, "P_D", &q)) p->P_d[q] = atof(s);
-        else if (key == "MU1") p->mu1 = atof(s);
-        else if (key == "MU2") p->mu2 = atof(s);
-        else if (key == "RO_W") p->ro01 = atof(s);
-        else if (key == "RO_N") p->ro02 = atof(s);
-        else if (key == "S2_GR") p->S2_gr = atof(s);
-        else if (key == "P_ATM") p->P_atm = atof(s);
-        else if (key == "G_CONST") p->g_const = atof(s);
-        // extensions of this backend
-        else if (key == "NEWTON_ITERATIONS") p->newton_iterations = atoi(s);
-        else if (key == "N_MEDIA") p->n_media = atoi(s);
-        else if (key == "ROS_SCHEME") p->roS_scheme = atoi(s);
-        else if (key == "SRC_X0") { p->src_x0 = atoi(s); explicit_patch = true; }
-        else if (key == "SRC_X1") { p->src_x1 = atoi(s); explicit_patch = true; }
-        else if (key == "SRC_Z0") { p->src_z0 = atoi(s); explicit_patch = true; }
-        else if (key == "SRC_Z1") { p->src_z1 = atoi(s); explicit_patch = true; }
-    }
-    fclose(fh);
-    if (!explicit_patch)
-        p->src_x0 = p->src_x1 = p->src_z0 = p->src_z1 = -1;
-    filt_params_finalize(p);
-    return FILT_OK;
+    return filt::guarded([&]() -> int {
+        if (!path || !p)
+            return FILT_ERR_INVALID;
+        FILE* fh = fopen(path, "rt");
+        if (!fh)
+            return FILT_ERR_IO;
+        char line[250];
+        bool explicit_patch = false;
+        while (fgets(line, sizeof line, fh)) {
+            if (line[0] == '#')
+                continue;
+            const char* eq = strchr(line, '=');
+            if (!eq)
+                continue;
+            const std::string key(line, eq - line);
+            const char* v0 = eq + 1;
+            const char* v1 = strchr(v0, ' ');
+            const std::string val = v1 ? std::string(v0, v1 - v0) : std::string(v0);
+            const char* s = val.c_str();
+            int q = 0;
+            // the 16 keys of the reference
+            if (key == "H1") p->h1 = atof(s);
+            else if (key == "H2") p->h2 = atof(s);
+            else if (key == "H3") p->h3 = atof(s);
+            else if (key == "TAU") p->tau = atof(s);
+            else if (key == "DT") p->dt = atof(s);
+            else if (key == "L_W") p->l_w = atof(s);
+            else if (key == "L_N") p->l_n = atof(s);
+            else if (key == "C") p->c = atof(s);
+            else if (key == "BETA1") p->beta1 = atof(s);
+            else if (key == "BETA2") p->beta2 = atof(s);
+            else if (key == "SOURCE") p->source = atoi(s);
+            else if (key == "SAVE_PLOTS") p->save_plots = atoi(s);
+            else if (key == "PRINT_SCREEN") p->print_screen = atoi(s);
+            else if (key == "NX") p->NX = atoi(s);
+            else if (key == "NY") p->Ny = atoi(s);
+            else if (key == "NZ") p->Nz = atoi(s);
+            // keys defines.ini documents (defines.ini:52-100)
+            else if (parse_indexed(key, "K", &q)) p->K[q] = atof(s);
+            else if (parse_indexed(key, "LAMBDA", &q)) p->lambda[q] = atof(s);
+            else if (parse_indexed(key, "S_WR", &q)) p->S_wr[q] = atof(s);
+            else if (parse_indexed(key, "M", &q)) p->m[q] = atof(s);
+            else if (parse_indexed(key, "P_D", &q)) p->P_d[q] = atof(s);
+            else if (key == "MU1") p->mu1 = atof(s);
+            else if (key == "MU2") p->mu2 = atof(s);
+            else if (key == "RO_W") p->ro01 = atof(s);
+            else if (key == "RO_N") p->ro02 = atof(s);
+            else if (key == "S2_GR") p->S2_gr = atof(s);
+            else if (key == "P_ATM") p->P_atm = atof(s);
+            else if (key == "G_CONST") p->g_const = atof(s);
+            // extensions of this backend
+            else if (key == "NEWTON_ITERATIONS") p->newton_iterations = atoi(s);
+            else if (key == "N_MEDIA") p->n_media = atoi(s);
+            else if (key == "ROS_SCHEME") p->roS_scheme = atoi(s);
+            else if (key == "SRC_X0") { p->src_x0 = atoi(s); explicit_patch = true; }
+            else if (key == "SRC_X1") { p->src_x1 = atoi(s); explicit_patch = true; }
+            else if (key == "SRC_Z0") { p->src_z0 = atoi(s); explicit_patch = true; }
+            else if (key == "SRC_Z1") { p->src_z1 = atoi(s); explicit_patch = true; }
+        }
+        fclose(fh);
+        if (!explicit_patch)
+            p->src_x0 = p->src_x1 = p->src_z0 = p->src_z1 = -1;
+        filt_params_finalize(p);
+        return FILT_OK;
+    });
 }
 
 // Explicit-scheme guard.  In a water-filled cell stage 3 + the Newton act as

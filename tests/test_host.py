@@ -176,3 +176,14 @@ def test_slab_workload_passes_the_guard(pkg, shape):
     assert perm.shape == shape[::-1]
     r = W.stability(p, perm, poro)
     assert 0.5 < r < 1.5
+
+
+def test_host_allocation_failure_is_a_status_not_an_abort(pkg):
+    """C++ exceptions never cross the C ABI: a field of 10^15 cells cannot be
+    allocated, and the call returns FILT_ERR_NOMEM with a message."""
+    lib = pkg.abi.load_library()
+    out = np.zeros(8)
+    rc = lib.filt_gen_lognormal(out.ctypes.data, 100000, 100000, 100000, 1, 6.64e-11, 1.0, 4.0,
+                                0.0, 1)
+    assert rc == pkg.abi.FILT_ERR_NOMEM
+    assert b"memory" in lib.filt_last_error()
