@@ -279,3 +279,31 @@ def test_untiled_stencils_3d_bitwise(hip, orc, params, dims):
     finally:
         for kern in (0, 1):
             lib.filt_set_tile_config(kern, 0, 0, 0, 0)
+
+
+@pytest.mark.parametrize("rank,size", [(0, 2), (1, 3), (2, 3)])
+@pytest.mark.parametrize("stage", [1, 2, 3, 4, 5])
+def test_stage_parity_multirank_slabs(hip, orc, rank, size, stage):
+    """One rank's x slab with its halo columns (N_to_local, main.cpp:166-189):
+    inactive halo cells are left alone and every stage matches the oracle,
+    bit for bit where the single-rank stage is (stage 1 and the Newton with
+    the generic pow to 1e-13)."""
+    p = ref_params(23, 9, 7)
+    nx, _ = orc.N_to_local(size, rank, p)
+    g = random_state(p, np.random.default_rng(100 * stage + 10 * rank + size), localNx=nx)
+    d = hip.grid(p, nx)
+    try:
+        d.upload(g)
+        d.stage(stage, t=0.25, rank=rank, size=size)
+        d.sync()
+        out = g.copy()
+        d.download(out)
+    finally:
+        d.free()
+    want = g.copy()
+    orc.stage(stage, want, p, t=0.25, rank=rank, size=size)
+    for n in FIELD_NAMES_ALL:
+        if stage in (1, 4):
+            assert elem_rel(out[n], want[n]) <= 1e-13, (n, stage)
+        else:
+            assert np.array_equal(out[n], want[n], equal_nan=True), (n, stage)
