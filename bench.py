@@ -1,15 +1,18 @@
 """Benchmark of the filtration time step on one MI355X.
 
 Default workload (the configuration the headline metric is quoted on that
-fits one GPU): the plain FP64 drop-in path on a 256^3 grid with per-cell
-log-normal permeability (sigma 1, correlation length 8 cells) and porosity
-U[0.35, 0.45], all five defines.h stages per step (ro_P2_Xi, u, roS, Newton,
-boundary), 200 steps.  Inputs are resident in HBM before the timed region.
+fits one GPU): the plain FP64 path on a 256^3 grid with per-cell log-normal
+permeability (sigma 1, correlation length 8 cells) and porosity U[0.35, 0.45],
+200 steps of time_step_function as filt_time_step runs it: stage 1, stages
+2-4 (u, roS, Newton) in one z-march, the boundary conditions, every array of
+ptr_Arrays materialised with the bits the five stages give.  Inputs are
+resident in HBM before the timed region.  The five defines.h stage functions
+called one by one (the reference driver's calls) are reported beside it.
 
 Prints one JSON line (rank 0):
   value      cell-updates/s of the whole job (Nx*Ny*Nz*steps / wall time)
   roofline   the dominant kernel's algorithmic bytes per launch / its mean
-             duration (HIP events around every stage inside the timed region)
+             duration (HIP events around every launch inside the timed region)
              against MI355X's 8 TB/s; traffic from rocprofv3 PMC counters when
              a summary for this workload is committed under profiles/
   cpu_baseline  the reference CPU path (oracle/_ref, compiled from the
@@ -37,76 +40,15 @@ PROFILES = os.path.join(ROOT, "profiles")
 #   u:        Xi1, Xi2, ro1, ro2, P1, P2 in (48); six u out (48)        = 96
 #   roS:      S2, P1, P2, Xi1, Xi2, ro1, ro2, poro in (64); 4 out (32)  = 96
 #   Newton:   S2, P1, roS1, roS2 in (32); P1, S2 out (16)               = 48
-PLAIN_BYTES = {"ro_P2_Xi": 64, "u": 96, "roS": 96, "newton": 48, "boundary": 0}
-PLAIN_STAGES = ("ro_P2_Xi", "u", "roS", "newton", "boundary")
+#   u + roS + Newton in one z-march (filt_u_roS_P1_S2_calculation):
+#             P2, Xi1, Xi2, ro1, ro2, S2, poro in (56); six u, four roS,
+#             P1, S2 out (96)                                           = 152
+PLAIN_BYTES = {"ro_P2_Xi": 64, "u": 96, "roS": 96, "newton": 48, "u_roS_newton": 152,
+               "boundary": 0}
+STEP_STAGES = ("ro_P2_Xi", "u_roS_newton", "boundary")     # time_step_function
+SPLIT_STAGES = ("ro_P2_Xi", "u", "roS", "newton", "boundary")  # the five stage calls
 KERNEL_NAMES = {"ro_P2_Xi": "k_ro_P2_Xi", "u": "k_u_zm", "roS": "k_roS_zm", "newton": "k_newton",
-                "boundary": "k_bc_a+k_bc_b"}
-
-
-def cpu_baseline(args):
-    """Reference CPU path on a bounded sample: two time steps of a
-    256 x 256 x 128 slab of the same grid (the reference's rate is the same
-    per cell at 256^3; its i-outer / k-inner loop order strides whole
-    xy-planes), one thread, about 15-20 s.  Beside it, the oracle port
-    (loop-interchanged, OpenMP on up to 16 host threads, bitwise equal to the
-    reference) on the same sample."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle
-
-    nx, ny, nz = args.n, args.n, max(3, args.n // 2)
-    p = PKG.abi.default_params_py()
-    p.h1, p.h2, p.h3 = PKG.workloads.INI_H
-    p.dt, p.beta1, p.beta2 = 1e-4, 1e-4, 1e-8
-    p.NX, p.Ny, p.Nz, p.source = nx, ny, nz, 2
-    PKG.abi.finalize_patch_py(p)
-    steps = 2
-    cpu = platform.processor() or platform.machine()
-    try:
-        with open("/proc/cpuinfo") as fh:
-            for line in fh:
-                if line.startswith("model name"):
-                    cpu = line.split(":", 1)[1].strip()
-                    break
-    except OSError:
-        pass
-
-    def timed(impl):
-        g = PKG.grid.HostGrid(p)
-        impl.initial_data(g, p)
-        t0 = time.perf_counter()
-        impl.run(g, p, steps)
-        return time.perf_counter() - t0, g
-
-    if oracle.ref_available():
-        impl, kind = oracle.RefCPU(), "reference"
-    else:
-        impl, kind = oracle.Oracle(), "port"
-        impl.set_loop_order(1)
-    dt, g_ref = timed(impl)
-    out = {"value": nx * ny * nz * steps / dt, "unit": "cell-updates/s", "cores": 1,
-           "kind": kind,
-           "sample": f"{steps} steps of a {nx}x{ny}x{nz} slab (media table, same h/dt/beta), "
-                     f"{dt:.1f} s on 1 thread of {cpu}"}
-    port = oracle.Oracle()
-    threads = max(1, min(16, os.cpu_count() or 1))
-    port.set_threads(threads)
-    port.set_loop_order(0)
-    dtp, g_port = timed(port)
-    import numpy as np
-    out["best_effort_port"] = {
-        "value": nx * ny * nz * steps / dtp, "cores": threads,
-        "sample": f"same sample, the oracle port with x-fastest loops and OpenMP, {dtp:.2f} s",
-        "bitwise_equal_to_reference": bool(np.array_equal(g_port["P1"], g_ref["P1"]) and
-                                           np.array_equal(g_port["S2"], g_ref["S2"]))}
-    return out
-
-
-def load_traffic(workload_tag, kernel):
-    path = os.path.join(PROFILES, f"pmc_{kernel}_{workload_tag}.json")
-    if not os.path.exists(path):
-        return None
-    with open(path) as fh:
-        return json.load(fh).get("bytes_per_launch")
+                "u_roS_newton": "k_step_zm", "boundary": "k_bc_a+k_bc_b"}
 
 
 def mixed_errors(P64, S64, m64, Pm, Sm, mm):
@@ -150,7 +92,12 @@ def run_slab(args, nx, ny, nz):
     return out
 
 
-def run_plain(args, p, perm, poro):
+def run_plain(args, p, perm, poro, split=False, steps=None):
+    """The plain FP64 path on one grid, inputs resident.  Default: the time
+    step as filt_time_step runs it (stage 1, stages 2-4 in one z-march, the
+    boundary conditions), three launches with HIP events between them on the
+    library stream.  split=True: the five defines.h stage functions called
+    one after another, as the reference's own driver calls them."""
     from importlib import import_module
     hiprt = import_module(PKG.__name__ + ".hiprt")
     be = PKG.device.Backend()
@@ -164,31 +111,38 @@ def run_plain(args, p, perm, poro):
     stream = lib.filt_get_stream()
     fref, pref = ctypes.byref(d.f), ctypes.byref(d.params)
     nx = d.nx
-    calls = (
-        lambda t: lib.filt_ro_P2_Xi_calculation(fref, pref, nx, 0, 1),
-        lambda t: lib.filt_u_calculation(fref, pref, nx, 0, 1),
-        lambda t: lib.filt_roS_calculation(fref, pref, t, nx, 0, 1),
-        lambda t: lib.filt_P1_S2_calculation(fref, pref, nx, 0, 1),
-        lambda t: lib.filt_boundary_conditions(fref, pref, nx, 0, 1),
-    )
+    stage1 = lambda t: lib.filt_ro_P2_Xi_calculation(fref, pref, nx, 0, 1)
+    bc = lambda t: lib.filt_boundary_conditions(fref, pref, nx, 0, 1)
+    if split:
+        names = SPLIT_STAGES
+        calls = (stage1,
+                 lambda t: lib.filt_u_calculation(fref, pref, nx, 0, 1),
+                 lambda t: lib.filt_roS_calculation(fref, pref, t, nx, 0, 1),
+                 lambda t: lib.filt_P1_S2_calculation(fref, pref, nx, 0, 1),
+                 bc)
+    else:
+        names = STEP_STAGES
+        calls = (stage1,
+                 lambda t: lib.filt_u_roS_P1_S2_calculation(fref, pref, t, nx, 0, 1, 1),
+                 bc)
     best = (ctypes.c_double * 2)()
-    if args.autotune:
+    if args.autotune and split:
         # stage 1 fills the inputs the stencil kernels read, then the
         # library's autotuner picks tile shapes for this grid
-        PKG.abi.check(lib, calls[0](0.0), "stage 1")
+        PKG.abi.check(lib, stage1(0.0), "stage 1")
         PKG.abi.check(lib, lib.filt_autotune_stencil(fref, pref, nx, 5, best), "autotune")
-    tuned = {"autotuned": bool(args.autotune)}
-    for kern, name in ((0, "roS"), (1, "u")):
+    tuned = {"autotuned": bool(args.autotune and split)}
+    for kern, name in ((0, "roS"), (1, "u"), (2, "u_roS_newton")):
         v = [ctypes.c_int() for _ in range(4)]
         lib.filt_get_tile_config(kern, nx, p.Ny, p.Nz, *[ctypes.byref(x) for x in v])
         tuned[name] = {"tile": [v[0].value, v[1].value], "kz": v[2].value, "order": v[3].value}
-        if args.autotune:
+        if args.autotune and split and kern < 2:
             tuned[name]["tuned_ms"] = best[kern]
     for s in range(1, args.warmup + 1):
         for c in calls:
             PKG.abi.check(lib, c(s * p.dt), "warmup")
     d.sync()
-    K = args.steps
+    K = steps or args.steps
     ev = [[hiprt.Event() for _ in range(len(calls) + 1)] for _ in range(K)]
     hiprt.device_synchronize()
     t0 = time.perf_counter()
@@ -202,7 +156,7 @@ def run_plain(args, p, perm, poro):
     hiprt.device_synchronize()
     wall = time.perf_counter() - t0
     stage_ms = {name: sum(ev[s][q].elapsed_ms(ev[s][q + 1]) for s in range(K)) / K
-                for q, name in enumerate(PLAIN_STAGES)}
+                for q, name in enumerate(names)}
     P1, S2 = d.get("P1"), d.get("S2")
     d.free()
     import numpy as np
@@ -442,12 +396,21 @@ def main():
             forms["serving"] = run_serving(args)
     cells = p.NX * p.Ny * p.Nz
     value = cells * args.steps / wall
-    dom = max((s for s in PLAIN_STAGES if PLAIN_BYTES[s]), key=lambda s: stage_ms[s])
+    dom = max((s for s in STEP_STAGES if PLAIN_BYTES[s]), key=lambda s: stage_ms[s])
     dom_bytes = PLAIN_BYTES[dom] * cells
     achieved = dom_bytes / (stage_ms[dom] * 1e-3)
     tag = f"plain_{args.n}cube"
     traffic = load_traffic(tag, KERNEL_NAMES[dom])
-    step_bytes = sum(PLAIN_BYTES.values()) * cells
+    step_bytes = sum(PLAIN_BYTES[s] for s in STEP_STAGES) * cells
+    split_bytes = sum(PLAIN_BYTES[s] for s in SPLIT_STAGES) * cells
+    # the five stage functions one by one (the reference driver's calls)
+    _, wall5, stage5, _, finite5, _ = run_plain(args, p, perm, poro, split=True,
+                                                  steps=max(1, args.steps // 2))
+    split = {"value": cells * max(1, args.steps // 2) / wall5,
+             "ms_per_step": wall5 * 1e3 / max(1, args.steps // 2),
+             "stages_ms": stage5, "state_finite": finite5,
+             "bytes_per_cell_update": sum(PLAIN_BYTES[s] for s in SPLIT_STAGES),
+             "hbm_frac": split_bytes * max(1, args.steps // 2) / wall5 / HBM_PEAK}
     line = {
         "metric": "cell-updates/s",
         "value": value,
@@ -462,7 +425,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"plain FP64 drop-in, {args.n}^3, 5 defines.h stages per step, "
+            "workload": f"plain FP64, {args.n}^3, time_step_function per step (stage 1, "
+                        "stages 2-4 fused, boundary; every ptr_Arrays array materialised), "
                         "per-cell log-normal K (sigma 1, corr 8 cells) and porosity",
             "grid": [p.NX, p.Ny, p.Nz],
             "form": "plain",
@@ -484,10 +448,13 @@ def main():
         },
         "stages_ms": stage_ms,
         "step_roofline": {
-            "bytes_per_cell_update": sum(PLAIN_BYTES.values()),
+            "bytes_per_cell_update": sum(PLAIN_BYTES[s] for s in STEP_STAGES),
             "achieved_GBps": step_bytes * args.steps / wall / 1e9,
             "frac": step_bytes * args.steps / wall / HBM_PEAK,
+            "vs_split_model": {"bytes_per_cell_update": sum(PLAIN_BYTES[s] for s in SPLIT_STAGES),
+                               "frac": split_bytes * args.steps / wall / HBM_PEAK},
         },
+        "stage_calls": split,
         "forms": forms,
     }
     if not args.no_cpu:

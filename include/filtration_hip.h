@@ -236,9 +236,21 @@ int filt_P1_S2_calculation(const filt_fields* dev, const filt_params* p,
 int filt_boundary_conditions(const filt_fields* dev, const filt_params* p,
                              int localNx, int rank, int size);
 
-/* One whole time step in the order of time_step_function (main.cpp:83-94);
- * the exchanges are no-ops on one GPU (no_communication.cpp:3-14).
- * with_u = 0 skips u_calculation, which nothing on the P/S path reads. */
+/* Stages 2-4 of time_step_function (main.cpp:86-90): u_calculation (when
+ * with_u), roS_calculation, P1_S2_calculation, on a state stage 1 has just
+ * been applied to (so P2 holds P1's bits, P2 = P1 + 0, cpu.cpp:63).  On one
+ * rank whose grid has every extent >= 3 and arrays below 4 GiB, with the
+ * upwind scheme, one z-marching kernel computes all three: it reads P2 where
+ * the stages read P1 or P2, and writes every array the three stages write,
+ * with the same bits, moving 152 B per cell instead of 240.  Elsewhere (or
+ * after filt_set_tile_config(2, -1, ...)) the three stages run in turn. */
+int filt_u_roS_P1_S2_calculation(const filt_fields* dev, const filt_params* p,
+                                 double t, int localNx, int rank, int size, int with_u);
+
+/* One whole time step in the order of time_step_function (main.cpp:83-94):
+ * stage 1, filt_u_roS_P1_S2_calculation, boundary conditions; the exchanges
+ * are no-ops on one GPU (no_communication.cpp:3-14).  with_u = 0 skips
+ * u_calculation, which nothing on the P/S path reads. */
 int filt_time_step(const filt_fields* dev, const filt_params* p, double t,
                    int localNx, int rank, int size, int with_u);
 
@@ -254,9 +266,10 @@ int filt_save_exchange_data(double* dev_array, double* dev_buffer,
                             const double* host_buffer, int localNx, int rank,
                             int size, const filt_params* p);
 
-/* Tile shape of the z-marching stencil kernels of u_calculation (kernel 1)
- * and roS_calculation (kernel 0): tx*ty threads per block (64x4, 32x8,
- * 16x16 or 64x8), kz planes per z-chunk, order 0 = XCD-grouped tile
+/* Tile shape of the z-marching stencil kernels of u_calculation (kernel 1),
+ * roS_calculation (kernel 0) and the fused stages 2-4 (kernel 2): tx*ty
+ * threads per block (64x4, 32x8, 16x16 or 64x8), kz planes per z-chunk,
+ * order 0 = XCD-grouped tile
  * numbering, 1 = hardware order.  Used by grids with every extent >= 3.
  * tx = 0 restores the built-in default (the fastest configuration on a
  * 256^3 grid: roS 64x4, kz 16, order 1; u 64x8, kz 64, order 1); tx = -1

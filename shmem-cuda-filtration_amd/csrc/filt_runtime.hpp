@@ -25,8 +25,13 @@ int check_shape(const filt_params* p, int localNx);
 // z-marching stencil launchers (filt_stencil.hip), for extents >= 3 and
 // arrays below 4 GiB (tiled_fits).
 bool tiled_fits(int nx, int ny, int nz);
-// False after filt_set_tile_config(kernel, -1, ...): kernel 0 roS, 1 u.
+// False after filt_set_tile_config(kernel, -1, ...): kernel 0 roS, 1 u,
+// 2 the fused step.
 bool tiled_enabled(int kernel);
+// The fused step (filt_step.hip): u, roS and the Newton in one z-march;
+// false when the grid or the parameters need the stages one by one.
+bool step_fusable(const filt_params& p, int nx, int rank, int size);
+int launch_step_zm(const filt_fields& f, const filt_params& p, int nx, bool with_u);
 int launch_roS_zm(const filt_fields& f, const filt_params& p, int nx);
 int launch_u_zm(const filt_fields& f, const filt_params& p, int nx, int rank, int size);
 

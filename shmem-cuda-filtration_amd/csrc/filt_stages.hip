@@ -586,17 +586,33 @@ int filt_boundary_conditions(const filt_fields* dev, const filt_params* p, int l
     return FILT_OK;
 }
 
-int filt_time_step(const filt_fields* dev, const filt_params* p, double t, int localNx, int rank,
-                   int size, int with_u)
+int filt_u_roS_P1_S2_calculation(const filt_fields* dev, const filt_params* p, double t,
+                                 int localNx, int rank, int size, int with_u)
 {
-    int rc;
-    if ((rc = filt_ro_P2_Xi_calculation(dev, p, localNx, rank, size))) return rc;
+    int rc = launch_check(dev, p, localNx);
+    if (rc) return rc;
+    if (step_fusable(*p, localNx, rank, size)) {
+        if (!dev->P2 || !dev->Xi1 || !dev->Xi2 || !dev->ro1 || !dev->ro2 || !dev->roS1 ||
+            !dev->roS2 || !dev->roS1_old || !dev->roS2_old ||
+            (with_u && (!dev->u1x || !dev->u1y || !dev->u1z || !dev->u2x || !dev->u2y ||
+                        !dev->u2z)))
+            return set_error(FILT_ERR_INVALID, "null device fields");
+        return launch_step_zm(*dev, *p, localNx, with_u != 0);
+    }
     // the divgrad scheme reads this step's u (cpu.cpp:187-190)
     if ((with_u || p->roS_scheme == FILT_SCHEME_DIVGRAD) &&
         (rc = filt_u_calculation(dev, p, localNx, rank, size)))
         return rc;
     if ((rc = filt_roS_calculation(dev, p, t, localNx, rank, size))) return rc;
-    if ((rc = filt_P1_S2_calculation(dev, p, localNx, rank, size))) return rc;
+    return filt_P1_S2_calculation(dev, p, localNx, rank, size);
+}
+
+int filt_time_step(const filt_fields* dev, const filt_params* p, double t, int localNx, int rank,
+                   int size, int with_u)
+{
+    int rc;
+    if ((rc = filt_ro_P2_Xi_calculation(dev, p, localNx, rank, size))) return rc;
+    if ((rc = filt_u_roS_P1_S2_calculation(dev, p, t, localNx, rank, size, with_u))) return rc;
     return filt_boundary_conditions(dev, p, localNx, rank, size);
 }
 

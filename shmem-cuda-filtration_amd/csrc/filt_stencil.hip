@@ -31,77 +31,14 @@
 #include "filt_device.hpp"
 #include "filt_runtime.hpp"
 #include "filt_tiles.hpp"
+#include "filt_zmarch.hpp"
 #include "../host/filt_guard.hpp"
 
 namespace filt {
 
 namespace {
 
-template <int TX, int TY>
-struct Lds {
-    static constexpr int SX = TX + 2;
-    static constexpr int SN = SX * (TY + 2);
-    __device__ static inline int at(int a, int y, int x) { return a * SN + y * SX + x; }
-};
-
-__device__ __forceinline__ double ld(const double* base, unsigned off)
-{
-    return *(const double*)((const char*)base + off);
-}
-
-__device__ __forceinline__ void st(double* base, unsigned off, double v)
-{
-    *(double*)((char*)base + off) = v;
-}
-
-__device__ __forceinline__ int ldi(const int* base, unsigned off4)
-{
-    return *(const int*)((const char*)base + off4);
-}
-
-// The halo ring of a TX x TY tile without its corners (a cross stencil never
-// reads them): 2*TX + 2*TY cells, cell h at LDS row ly, column lx and at
-// (di, dj) from the tile origin.  Rows y-1 and y+TY first, then columns x-1
-// and x+TX.
-template <int TX, int TY>
-struct Ring {
-    static constexpr int N = 2 * TX + 2 * TY;
-    __device__ static inline void cell(int h, int& ly, int& lx, int& di, int& dj)
-    {
-        if (h < TX) {
-            ly = 0, lx = h + 1, di = h, dj = -1;
-        } else if (h < 2 * TX) {
-            ly = TY + 1, lx = h - TX + 1, di = h - TX, dj = TY;
-        } else if (h < 2 * TX + TY) {
-            ly = h - 2 * TX + 1, lx = 0, di = -1, dj = ly - 1;
-        } else {
-            ly = h - 2 * TX - TY + 1, lx = TX + 1, di = TX, dj = ly - 1;
-        }
-    }
-};
-
-// Thread (tx, ty)'s ring cell, when it has one inside the grid: its LDS
-// slot (array 0) and its column's byte offset.
-template <int TX, int TY>
-__device__ inline bool ring_slot(int tx, int ty, int i0, int j0, int nx, int ny, unsigned sy,
-                                 int& slot, unsigned& hcol)
-{
-    using R = Ring<TX, TY>;
-    static_assert(R::N <= TX * TY, "one ring cell per thread at most");
-    const int h = tx + TX * ty;
-    slot = 0;
-    hcol = 0;
-    if (h >= R::N)
-        return false;
-    int ly, lx, di, dj;
-    R::cell(h, ly, lx, di, dj);
-    const int gi = i0 + di, gj = j0 + dj;
-    if (gi < 0 || gi >= nx || gj < 0 || gj >= ny)
-        return false;
-    slot = Lds<TX, TY>::at(0, ly, lx);
-    hcol = 8u * gi + sy * gj;
-    return true;
-}
+using namespace zm;
 
 // ---------------------------------------------------------------- roS ----
 // cpu.cpp:218-291 for interior cells; arrays P1, P2, Xi1, Xi2, ro1, ro2 are
@@ -390,44 +327,39 @@ UFn pick_u(int tx, int ty)
     return nullptr;
 }
 
-TileMap make_map(int nx, int ny, int nz, const TileConfig& t)
-{
-    TileMap m;
-    m.nbx = (nx + t.tx - 1) / t.tx;
-    m.nby = (ny + t.ty - 1) / t.ty;
-    m.nbz = (nz + t.kz - 1) / t.kz;
-    m.order = t.order;
-    return m;
-}
-
 // Defaults: the fastest configurations of a sweep over every shape, kz in
 // {16, 64} and both orders on a 256^3 grid on MI355X.
 constexpr TileConfig kRoSDefault = {64, 4, 16, 1};
 constexpr TileConfig kUDefault = {64, 8, 64, 1};
+constexpr TileConfig kStepDefault = {64, 4, 16, 1};
 TileConfig g_roS_cfg = kRoSDefault;
 TileConfig g_u_cfg = kUDefault;
-// Per kernel (0 roS, 1 u): run the one-cell-per-thread kernels of
-// filt_stages.hip even where the tiled ones apply (filt_set_tile_config
-// with tx = -1).
-bool g_untiled[2] = {false, false};
+TileConfig g_step_cfg = kStepDefault;
+// Per kernel (0 roS, 1 u, 2 the fused step): run the one-cell-per-thread
+// kernels of filt_stages.hip (for 2: the stages one by one) even where the
+// tiled ones apply (filt_set_tile_config with tx = -1).
+bool g_untiled[3] = {false, false, false};
 
 // Autotuned configurations by grid shape (filt_autotune_stencil).
 std::mutex g_tune_mu;
 std::map<std::tuple<int, int, int>, std::pair<TileConfig, TileConfig>> g_tuned;
 
-bool valid_cfg(const TileConfig& t, bool for_roS)
+// The fused step offers the tile shapes of the roS kernel.
+bool valid_cfg(const TileConfig& t, int kernel)
 {
     return t.kz >= 1 && t.kz <= 4096 && (t.order == 0 || t.order == 1) &&
-           (for_roS ? pick_roS(t.tx, t.ty) != nullptr : pick_u(t.tx, t.ty) != nullptr);
+           (kernel != 1 ? pick_roS(t.tx, t.ty) != nullptr : pick_u(t.tx, t.ty) != nullptr);
 }
 
-TileConfig cfg_for(bool roS, int nx, int ny, int nz)
+TileConfig cfg_for(int kernel, int nx, int ny, int nz)
 {
     std::lock_guard<std::mutex> lk(g_tune_mu);
+    if (kernel == 2)
+        return g_step_cfg;
     auto it = g_tuned.find(std::make_tuple(nx, ny, nz));
     if (it != g_tuned.end())
-        return roS ? it->second.first : it->second.second;
-    return roS ? g_roS_cfg : g_u_cfg;
+        return kernel == 0 ? it->second.first : it->second.second;
+    return kernel == 0 ? g_roS_cfg : g_u_cfg;
 }
 
 }  // namespace
@@ -444,10 +376,12 @@ bool tiled_enabled(int kernel)
     return !g_untiled[kernel];
 }
 
+TileConfig step_tile_config(int nx, int ny, int nz) { return cfg_for(2, nx, ny, nz); }
+
 // Launchers used by filt_stages.hip (grids with NX, Ny, Nz >= 3).
 int launch_roS_zm(const filt_fields& f, const filt_params& p, int nx)
 {
-    const TileConfig t = cfg_for(true, nx, p.Ny, p.Nz);
+    const TileConfig t = cfg_for(0, nx, p.Ny, p.Nz);
     const TileMap m = make_map(nx, p.Ny, p.Nz, t);
     hipLaunchKernelGGL(pick_roS(t.tx, t.ty), dim3((unsigned)m.count()), dim3(t.tx, t.ty), 0,
                        stream(), f, p, nx, m, t.kz);
@@ -457,7 +391,7 @@ int launch_roS_zm(const filt_fields& f, const filt_params& p, int nx)
 
 int launch_u_zm(const filt_fields& f, const filt_params& p, int nx, int rank, int size)
 {
-    const TileConfig t = cfg_for(false, nx, p.Ny, p.Nz);
+    const TileConfig t = cfg_for(1, nx, p.Ny, p.Nz);
     const TileMap m = make_map(nx, p.Ny, p.Nz, t);
     hipLaunchKernelGGL(pick_u(t.tx, t.ty), dim3((unsigned)m.count()), dim3(t.tx, t.ty), 0,
                        stream(), f, p, nx, rank, size, m, t.kz);
@@ -471,22 +405,23 @@ extern "C" int filt_set_tile_config(int kernel, int tx, int ty, int kz, int orde
 {
     return filt::guarded([&]() -> int {
         filt::TileConfig t = {tx, ty, kz, order};
-        if (kernel != 0 && kernel != 1)
-            return filt::set_error(FILT_ERR_INVALID, "kernel must be 0 (roS) or 1 (u)");
+        if (kernel < 0 || kernel > 2)
+            return filt::set_error(FILT_ERR_INVALID, "kernel must be 0 (roS), 1 (u) or 2 (step)");
         if (tx == -1) {  // the untiled kernels
             std::lock_guard<std::mutex> lk(filt::g_tune_mu);
             filt::g_untiled[kernel] = true;
             return FILT_OK;
         }
         if (tx == 0)  // back to the built-in default
-            t = kernel == 0 ? filt::kRoSDefault : filt::kUDefault;
-        if (!filt::valid_cfg(t, kernel == 0))
+            t = kernel == 0 ? filt::kRoSDefault : kernel == 1 ? filt::kUDefault : filt::kStepDefault;
+        if (!filt::valid_cfg(t, kernel))
             return filt::set_error(FILT_ERR_INVALID, "unsupported tile %dx%d kz=%d order=%d", tx, ty,
                                    kz, order);
         std::lock_guard<std::mutex> lk(filt::g_tune_mu);
-        (kernel == 0 ? filt::g_roS_cfg : filt::g_u_cfg) = t;
+        (kernel == 0 ? filt::g_roS_cfg : kernel == 1 ? filt::g_u_cfg : filt::g_step_cfg) = t;
         filt::g_untiled[kernel] = false;
-        filt::g_tuned.clear();
+        if (kernel != 2)
+            filt::g_tuned.clear();
         return FILT_OK;
     });
 }
@@ -569,9 +504,9 @@ extern "C" int filt_autotune_stencil(const filt_fields* dev, const filt_params* 
 extern "C" int filt_get_tile_config(int kernel, int nx, int ny, int nz, int* tx, int* ty, int* kz,
                                     int* order)
 {
-    if (kernel != 0 && kernel != 1)
-        return filt::set_error(FILT_ERR_INVALID, "kernel must be 0 (roS) or 1 (u)");
-    const filt::TileConfig t = filt::cfg_for(kernel == 0, nx, ny, nz);
+    if (kernel < 0 || kernel > 2)
+        return filt::set_error(FILT_ERR_INVALID, "kernel must be 0 (roS), 1 (u) or 2 (step)");
+    const filt::TileConfig t = filt::cfg_for(kernel, nx, ny, nz);
     if (tx) *tx = t.tx;
     if (ty) *ty = t.ty;
     if (kz) *kz = t.kz;

@@ -1,0 +1,253 @@
+// filt_step.hip -- stages 2-4 of time_step_function (main.cpp:83-94) in one
+// z-march: u_calculation (cpu.cpp:77-160), roS_calculation (cpu.cpp:218-291)
+// and P1_S2_calculation (cpu.cpp:293-320), for one rank whose grid has every
+// extent >= 3 and arrays below 4 GiB, after stage 1 has run on the state.
+//
+// Stage 1 writes P2 = P1 + P_k with P_k = 0 (cpu.cpp:63), so P2 holds P1's
+// bits (for every P1 but -0.0, which no pressure takes).  The kernel reads
+// P2 wherever the stages read P1 or P2; the Newton then rewrites P1 and S2
+// of its own cell in place without any neighbour needing their old values
+// (the stencils read P2, Xi and ro; S2 only at the centre).  Every array is
+// written with the values the three stages write one after another, so the
+// state after the step is the same bit for bit, while HBM sees 152 B per
+// cell instead of 240: the stencil operands are read once for u and roS,
+// roS is not read back by the Newton, and P1, S2 are not read twice.
+//
+// Structure as in filt_stencil.hip: TX x TY tiles of columns march over a
+// chunk of planes; per plane the five stencil operands (P2, Xi1, Xi2, ro1,
+// ro2) of the tile and its one-cell ring go to one of two LDS tiles, the own
+// column's k-1, k, k+1 stay in registers, the ring and the point operands
+// of plane k+1 and the own column of plane k+2 are in flight while plane k
+// is computed, and one barrier orders each plane.  u is computed on every
+// plane of the grid; roS and the Newton on the interior.
+#include <hip/hip_runtime.h>
+
+#include "../../include/filtration_hip.h"
+#include "filt_device.hpp"
+#include "filt_runtime.hpp"
+#include "filt_tiles.hpp"
+#include "filt_zmarch.hpp"
+
+namespace filt {
+
+namespace {
+
+using namespace zm;
+
+constexpr int NA = 5;  // stencil operands: P2, Xi1, Xi2, ro1, ro2
+enum { AP = 0, AX1 = 1, AX2 = 2, AR1 = 3, AR2 = 4 };
+
+template <int TX, int TY, bool WITH_U>
+__global__ __launch_bounds__(TX* TY) void k_step_zm(filt_fields f, filt_params p, int nx,
+                                                    TileMap map, int kz)
+{
+    using L = Lds<TX, TY>;
+    __shared__ double shb[2][NA * L::SN];
+    int bx, by, bz;
+    map.decode(blockIdx.x, bx, by, bz);
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int i0 = bx * TX, j0 = by * TY;
+    const int i = i0 + tx, j = j0 + ty;
+    const int ny = p.Ny, nz = p.Nz;
+    const bool valid = i < nx && j < ny;
+    const bool inner = i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2;
+    const unsigned sy = 8u * nx, sz = 8u * nx * ny;  // byte strides
+    // u on every plane; roS and the Newton on planes 1 .. nz-2
+    const int kb = max(bz * kz, WITH_U ? 0 : 1);
+    const int ke = min(bz * kz + kz, WITH_U ? nz : nz - 1);
+    if (kb >= ke)
+        return;  // uniform over the block
+    const double* const A[NA] = {f.P2, f.Xi1, f.Xi2, f.ro1, f.ro2};
+    const unsigned col = 8u * i + sy * j;
+    int hslot;
+    unsigned hcol;
+    const bool hv = ring_slot<TX, TY>(tx, ty, i0, j0, nx, ny, sy, hslot, hcol);
+    const bool per_cell_media = f.media && p.n_media > 1;
+    // Own column at k-1, k, k+1 and (in flight) k+2; the ring cell and the
+    // point operands (S2, porosity, medium) of the next plane.
+    double zm[NA] = {}, zc[NA] = {}, zp[NA] = {}, zq[NA] = {}, hn[NA] = {};
+    double Sc = 0, Sn = 0, phic = 0, phin = 0;
+    int mdc = 0, mdn = 0;
+    if (valid) {
+        const unsigned c = col + kb * sz;
+#pragma unroll
+        for (int a = 0; a < NA; a++) {
+            if (kb >= 1)
+                zm[a] = ld(A[a], c - sz);
+            zc[a] = ld(A[a], c);
+            if (kb + 1 < nz)
+                zp[a] = ld(A[a], c + sz);
+        }
+        Sc = ld(f.S2, c);
+        if (f.poro)
+            phic = ld(f.poro, c);
+        if (per_cell_media)
+            mdc = ldi(f.media, c >> 1);
+    }
+    if (hv)
+#pragma unroll
+        for (int a = 0; a < NA; a++) hn[a] = ld(A[a], hcol + kb * sz);
+    const int cx = tx + 1, cy = ty + 1;
+    const double g = p.g_const;
+    for (int k = kb; k < ke; k++) {
+        double* const sh = shb[(k - kb) & 1];
+        const unsigned c = col + k * sz;
+        if (valid)
+#pragma unroll
+            for (int a = 0; a < NA; a++) sh[L::at(a, cy, cx)] = zc[a];
+        if (hv)
+#pragma unroll
+            for (int a = 0; a < NA; a++) sh[hslot + a * L::SN] = hn[a];
+        if (k + 1 < ke) {  // the next plane is computed: fetch its operands
+            if (valid) {
+                if (k + 2 < nz)
+#pragma unroll
+                    for (int a = 0; a < NA; a++) zq[a] = ld(A[a], c + 2 * sz);
+                Sn = ld(f.S2, c + sz);
+                if (f.poro)
+                    phin = ld(f.poro, c + sz);
+                if (per_cell_media)
+                    mdn = ldi(f.media, (c + sz) >> 1);
+            }
+            if (hv)
+#pragma unroll
+                for (int a = 0; a < NA; a++) hn[a] = ld(A[a], hcol + (k + 1) * sz);
+        }
+        __syncthreads();
+        const double Pc = zc[AP];
+        if (WITH_U && valid) {
+            // u_calculation, cpu.cpp:77-160 (P1 = P2)
+            const double X1 = zc[AX1], X2 = zc[AX2];
+            double u1, u2;
+            if (i == 0) {
+                u1 = X1 * (sh[L::at(AP, cy, cx + 1)] - Pc) / p.h1;
+                u2 = X2 * (sh[L::at(AP, cy, cx + 1)] - Pc) / p.h1;
+            } else if (i == nx - 1) {
+                u1 = X1 * (Pc - sh[L::at(AP, cy, cx - 1)]) / p.h1;
+                u2 = X2 * (Pc - sh[L::at(AP, cy, cx - 1)]) / p.h1;
+            } else {
+                u1 = X1 * ((sh[L::at(AP, cy, cx + 1)] - sh[L::at(AP, cy, cx - 1)]) / (2. * p.h1));
+                u2 = X2 * ((sh[L::at(AP, cy, cx + 1)] - sh[L::at(AP, cy, cx - 1)]) / (2. * p.h1));
+            }
+            st(f.u1x, c, u1);
+            st(f.u2x, c, u2);
+            const double r1 = zc[AR1], r2 = zc[AR2];
+            if (j == 0) {
+                u1 = X1 * ((sh[L::at(AP, cy + 1, cx)] - Pc) / p.h2 - r1 * g);
+                u2 = X2 * ((sh[L::at(AP, cy + 1, cx)] - Pc) / p.h2 - r2 * g);
+            } else if (j == ny - 1) {
+                u1 = X1 * ((Pc - sh[L::at(AP, cy - 1, cx)]) / p.h2 - r1 * g);
+                u2 = X2 * ((Pc - sh[L::at(AP, cy - 1, cx)]) / p.h2 - r2 * g);
+            } else {
+                u1 = X1 * ((sh[L::at(AP, cy + 1, cx)] - sh[L::at(AP, cy - 1, cx)]) / (2. * p.h2) -
+                           r1 * g);
+                u2 = X2 * ((sh[L::at(AP, cy + 1, cx)] - sh[L::at(AP, cy - 1, cx)]) / (2. * p.h2) -
+                           r2 * g);
+            }
+            st(f.u1y, c, u1);
+            st(f.u2y, c, u2);
+            if (k == 0) {
+                u1 = X1 * (zp[AP] - Pc) / p.h3;
+                u2 = X2 * (zp[AP] - Pc) / p.h3;
+            } else if (k == nz - 1) {
+                u1 = X1 * (Pc - zm[AP]) / p.h3;
+                u2 = X2 * (Pc - zm[AP]) / p.h3;
+            } else {
+                u1 = X1 * ((zp[AP] - zm[AP]) / (2. * p.h3));
+                u2 = X2 * ((zp[AP] - zm[AP]) / (2. * p.h3));
+            }
+            st(f.u1z, c, u1);
+            st(f.u2z, c, u2);
+        }
+        if (valid && inner && k >= 1 && k <= nz - 2) {
+            // roS_calculation, cpu.cpp:218-291 (P1 = P2)
+            const double X1c = zc[AX1], X2c = zc[AX2], r1c = zc[AR1], r2c = zc[AR2];
+            const double S = Sc;
+            const double roS1 = r1c * (1 - S);
+            const double roS2 = r2c * S;
+            double a1, a2;
+            a2 = -(zp[AP] - Pc) / p.h3;
+            a1 = -(Pc - zm[AP]) / p.h3;
+            const double f3 = upwind(a2, a1, X1c, r1c, zm[AX1], zm[AR1], zp[AX1], zp[AR1], p.h3);
+            const double g3 = upwind(a2, a1, X2c, r2c, zm[AX2], zm[AR2], zp[AX2], zp[AR2], p.h3);
+            const int xp = cx + 1, xm = cx - 1, yp = cy + 1, ym = cy - 1;
+            a2 = -(sh[L::at(AP, cy, xp)] - Pc) / p.h1;
+            a1 = -(Pc - sh[L::at(AP, cy, xm)]) / p.h1;
+            const double f1 = upwind(a2, a1, X1c, r1c, sh[L::at(AX1, cy, xm)],
+                                     sh[L::at(AR1, cy, xm)], sh[L::at(AX1, cy, xp)],
+                                     sh[L::at(AR1, cy, xp)], p.h1);
+            const double g1 = upwind(a2, a1, X2c, r2c, sh[L::at(AX2, cy, xm)],
+                                     sh[L::at(AR2, cy, xm)], sh[L::at(AX2, cy, xp)],
+                                     sh[L::at(AR2, cy, xp)], p.h1);
+            const double dy2 = -(sh[L::at(AP, yp, cx)] - Pc) / p.h2;
+            const double dy1 = -(Pc - sh[L::at(AP, ym, cx)]) / p.h2;
+            const double f2 = upwind(dy2 + r1c * g, dy1 + r1c * g, X1c, r1c,
+                                     sh[L::at(AX1, ym, cx)], sh[L::at(AR1, ym, cx)],
+                                     sh[L::at(AX1, yp, cx)], sh[L::at(AR1, yp, cx)], p.h2);
+            const double g2 = upwind(dy2 + r2c * g, dy1 + r2c * g, X2c, r2c,
+                                     sh[L::at(AX2, ym, cx)], sh[L::at(AR2, ym, cx)],
+                                     sh[L::at(AX2, yp, cx)], sh[L::at(AR2, yp, cx)], p.h2);
+            const int md = mdc;
+            const double phi = f.poro ? phic : p.m[md];
+            const double A1 = roS1 - (p.dt / phi) * (f1 + f2 + f3);
+            const double A2 = roS2 - (p.dt / phi) * (g1 + g2 + g3);
+            st(f.roS1_old, c, roS1);
+            st(f.roS2_old, c, roS2);
+            st(f.roS1, c, A1);
+            st(f.roS2, c, A2);
+            // P1_S2_calculation, cpu.cpp:293-320, on this cell's registers
+            double P = Pc, Sv = S;
+            newton_cell(P, Sv, A1, A2, p.lambda[md], p.S_wr[md], p.P_d[md], p);
+            st(f.P1, c, P);
+            st(f.S2, c, Sv);
+        }
+#pragma unroll
+        for (int a = 0; a < NA; a++) {
+            zm[a] = zc[a];
+            zc[a] = zp[a];
+            zp[a] = zq[a];
+        }
+        Sc = Sn;
+        phic = phin;
+        mdc = mdn;
+    }
+}
+
+template <bool WITH_U>
+using StepFn = void (*)(filt_fields, filt_params, int, TileMap, int);
+
+template <bool WITH_U>
+StepFn<WITH_U> pick_step(int tx, int ty)
+{
+    if (tx == 64 && ty == 4) return k_step_zm<64, 4, WITH_U>;
+    if (tx == 32 && ty == 8) return k_step_zm<32, 8, WITH_U>;
+    if (tx == 16 && ty == 16) return k_step_zm<16, 16, WITH_U>;
+    if (tx == 64 && ty == 8) return k_step_zm<64, 8, WITH_U>;
+    return nullptr;
+}
+
+}  // namespace
+
+bool step_fusable(const filt_params& p, int nx, int rank, int size)
+{
+    return size == 1 && nx == p.NX && nx >= 3 && p.Ny >= 3 && p.Nz >= 3 &&
+           tiled_fits(nx, p.Ny, p.Nz) && p.roS_scheme == FILT_SCHEME_UPWIND &&
+           tiled_enabled(2);
+}
+
+int launch_step_zm(const filt_fields& f, const filt_params& p, int nx, bool with_u)
+{
+    const TileConfig t = step_tile_config(nx, p.Ny, p.Nz);
+    const TileMap m = make_map(nx, p.Ny, p.Nz, t);
+    if (with_u) {
+        hipLaunchKernelGGL(pick_step<true>(t.tx, t.ty), dim3((unsigned)m.count()),
+                           dim3(t.tx, t.ty), 0, stream(), f, p, nx, m, t.kz);
+    } else {
+        hipLaunchKernelGGL(pick_step<false>(t.tx, t.ty), dim3((unsigned)m.count()),
+                           dim3(t.tx, t.ty), 0, stream(), f, p, nx, m, t.kz);
+    }
+    FILT_LAUNCH_CHECK("k_step_zm");
+    return FILT_OK;
+}
+
+}  // namespace filt

@@ -39,9 +39,23 @@ struct TileMap {
 
 // Tile configuration of one stencil kernel.
 struct TileConfig {
-    int tx, ty;  // threads per block in x and y (tx * ty = 256)
+    int tx, ty;  // threads per block in x and y
     int kz;      // planes per z-chunk
     int order;   // TileMap order
 };
+
+// The fused step's configuration for a grid (filt_stencil.hip).
+TileConfig step_tile_config(int nx, int ny, int nz);
+
+// Tiles and z-chunks covering an nx * ny * nz grid.
+inline TileMap make_map(int nx, int ny, int nz, const TileConfig& t)
+{
+    TileMap m;
+    m.nbx = (nx + t.tx - 1) / t.tx;
+    m.nby = (ny + t.ty - 1) / t.ty;
+    m.nbz = (nz + t.kz - 1) / t.kz;
+    m.order = t.order;
+    return m;
+}
 
 }  // namespace filt

@@ -1,0 +1,79 @@
+// filt_zmarch.hpp -- pieces shared by the z-marching tile kernels
+// (filt_stencil.hip, filt_step.hip): the LDS tile layout with its one-cell
+// ring, the ring cell of each thread, and loads / stores through 32-bit byte
+// offsets from uniform array bases (one VGPR per address).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace filt {
+namespace zm {
+
+template <int TX, int TY>
+struct Lds {
+    static constexpr int SX = TX + 2;
+    static constexpr int SN = SX * (TY + 2);
+    __device__ static inline int at(int a, int y, int x) { return a * SN + y * SX + x; }
+};
+
+__device__ __forceinline__ double ld(const double* base, unsigned off)
+{
+    return *(const double*)((const char*)base + off);
+}
+
+__device__ __forceinline__ void st(double* base, unsigned off, double v)
+{
+    *(double*)((char*)base + off) = v;
+}
+
+__device__ __forceinline__ int ldi(const int* base, unsigned off4)
+{
+    return *(const int*)((const char*)base + off4);
+}
+
+// The halo ring of a TX x TY tile without its corners (a cross stencil never
+// reads them): 2*TX + 2*TY cells, cell h at LDS row ly, column lx and at
+// (di, dj) from the tile origin.  Rows y-1 and y+TY first, then columns x-1
+// and x+TX.
+template <int TX, int TY>
+struct Ring {
+    static constexpr int N = 2 * TX + 2 * TY;
+    __device__ static inline void cell(int h, int& ly, int& lx, int& di, int& dj)
+    {
+        if (h < TX) {
+            ly = 0, lx = h + 1, di = h, dj = -1;
+        } else if (h < 2 * TX) {
+            ly = TY + 1, lx = h - TX + 1, di = h - TX, dj = TY;
+        } else if (h < 2 * TX + TY) {
+            ly = h - 2 * TX + 1, lx = 0, di = -1, dj = ly - 1;
+        } else {
+            ly = h - 2 * TX - TY + 1, lx = TX + 1, di = TX, dj = ly - 1;
+        }
+    }
+};
+
+// Thread (tx, ty)'s ring cell, when it has one inside the grid: its LDS
+// slot (array 0) and its column's byte offset.
+template <int TX, int TY>
+__device__ inline bool ring_slot(int tx, int ty, int i0, int j0, int nx, int ny, unsigned sy,
+                                 int& slot, unsigned& hcol)
+{
+    using R = Ring<TX, TY>;
+    static_assert(R::N <= TX * TY, "one ring cell per thread at most");
+    const int h = tx + TX * ty;
+    slot = 0;
+    hcol = 0;
+    if (h >= R::N)
+        return false;
+    int ly, lx, di, dj;
+    R::cell(h, ly, lx, di, dj);
+    const int gi = i0 + di, gj = j0 + dj;
+    if (gi < 0 || gi >= nx || gj < 0 || gj >= ny)
+        return false;
+    slot = Lds<TX, TY>::at(0, ly, lx);
+    hcol = 8u * gi + sy * gj;
+    return true;
+}
+
+}  // namespace zm
+}  // namespace filt

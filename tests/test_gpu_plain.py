@@ -307,3 +307,73 @@ def test_stage_parity_multirank_slabs(hip, orc, rank, size, stage):
             assert elem_rel(out[n], want[n]) <= 1e-13, (n, stage)
         else:
             assert np.array_equal(out[n], want[n], equal_nan=True), (n, stage)
+
+
+def staged_step(d, t, with_u):
+    """time_step_function as five separate stage calls (main.cpp:83-94)."""
+    d.stage(1)
+    if with_u:
+        d.stage(2)
+    d.stage(3, t=t)
+    d.stage(4)
+    d.stage(5)
+
+
+def fused_vs_staged(hip, p, g, steps, with_u, perm=False, poro=False):
+    a = hip.grid(p, perm=perm, poro=poro)
+    b = hip.grid(p, perm=perm, poro=poro)
+    try:
+        a.upload(g)
+        b.upload(g)
+        for s in range(1, steps + 1):
+            a.time_step(s * p.dt, with_u)
+            staged_step(b, s * p.dt, with_u)
+        ga, gb = g.copy(), g.copy()
+        a.download(ga)
+        b.download(gb)
+    finally:
+        a.free()
+        b.free()
+    for n in FIELD_NAMES_ALL:
+        assert np.array_equal(ga[n], gb[n], equal_nan=True), n
+    return ga
+
+
+@pytest.mark.parametrize("params", ["ref", "bl"])
+@pytest.mark.parametrize("dims", [(37, 19, 23), (64, 16, 9), (5, 6, 7), (130, 9, 4), (3, 3, 3)])
+@pytest.mark.parametrize("with_u", [1, 0])
+def test_fused_step_equals_stage_calls(hip, params, dims, with_u):
+    """filt_time_step runs u, roS and the Newton as one z-march (reading P2
+    for P1): every array after three steps equals the five separate stage
+    calls bit for bit."""
+    p = ref_params(*dims, source=1) if params == "ref" else bl_params(*dims)
+    g = random_state(p, np.random.default_rng(dims[0] * 3 + with_u))
+    fused_vs_staged(hip, p, g, 3, with_u)
+
+
+def test_fused_step_per_cell_fields_and_trajectory(hip, orc):
+    """Per-cell K and porosity, 40 steps from the initial state: fused =
+    staged bit for bit, and both = the oracle to the trajectory bar."""
+    p = bl_params(48, 40, 36)
+    rng = np.random.default_rng(9)
+    g = random_state(p, rng, media=False, perm=True, poro=True)
+    fused_vs_staged(hip, p, g, 2, 1, perm=True, poro=True)
+    a = HostGrid(p)
+    orc.initial_data(a, p)
+    out = fused_vs_staged(hip, p, a, 40, 1)
+    orc.run(a, p, 40)
+    for n in ("P1", "S2", "u1x", "u2y", "roS1", "roS2_old"):
+        assert max_rel(out[n], a[n]) <= 1e-10, n
+
+
+@pytest.mark.parametrize("tile", TILES)
+@pytest.mark.parametrize("kz,order", [(1, 0), (5, 1), (16, 0), (64, 1)])
+def test_fused_step_every_tile_config(hip, tile, kz, order):
+    lib = hip.lib
+    p = ref_params(41, 22, 19, source=2)
+    g = random_state(p, np.random.default_rng(kz + tile[1]))
+    try:
+        assert lib.filt_set_tile_config(2, tile[0], tile[1], kz, order) == 0
+        fused_vs_staged(hip, p, g, 2, 1)
+    finally:
+        lib.filt_set_tile_config(2, 0, 0, 0, 0)
