@@ -51,6 +51,72 @@ KERNEL_NAMES = {"ro_P2_Xi": "k_ro_P2_Xi", "u": "k_u_zm", "roS": "k_roS_zm", "new
                 "u_roS_newton": "k_step_zm", "boundary": "k_bc_a+k_bc_b"}
 
 
+def cpu_baseline(args):
+    """Reference CPU path on a bounded sample: two time steps of a
+    256 x 256 x 128 slab of the same grid (the reference's rate is the same
+    per cell at 256^3; its i-outer / k-inner loop order strides whole
+    xy-planes), one thread, about 15-20 s.  Beside it, the oracle port
+    (loop-interchanged, OpenMP on up to 16 host threads, bitwise equal to the
+    reference) on the same sample."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    nx, ny, nz = args.n, args.n, max(3, args.n // 2)
+    p = PKG.abi.default_params_py()
+    p.h1, p.h2, p.h3 = PKG.workloads.INI_H
+    p.dt, p.beta1, p.beta2 = 1e-4, 1e-4, 1e-8
+    p.NX, p.Ny, p.Nz, p.source = nx, ny, nz, 2
+    PKG.abi.finalize_patch_py(p)
+    steps = 2
+    cpu = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+
+    def timed(impl):
+        g = PKG.grid.HostGrid(p)
+        impl.initial_data(g, p)
+        t0 = time.perf_counter()
+        impl.run(g, p, steps)
+        return time.perf_counter() - t0, g
+
+    if oracle.ref_available():
+        impl, kind = oracle.RefCPU(), "reference"
+    else:
+        impl, kind = oracle.Oracle(), "port"
+        impl.set_loop_order(1)
+    dt, g_ref = timed(impl)
+    out = {"value": nx * ny * nz * steps / dt, "unit": "cell-updates/s", "cores": 1,
+           "kind": kind,
+           "sample": f"{steps} steps of a {nx}x{ny}x{nz} slab (media table, same h/dt/beta), "
+                     f"{dt:.1f} s on 1 thread of {cpu}"}
+    port = oracle.Oracle()
+    threads = max(1, min(16, os.cpu_count() or 1))
+    port.set_threads(threads)
+    port.set_loop_order(0)
+    dtp, g_port = timed(port)
+    import numpy as np
+    out["best_effort_port"] = {
+        "value": nx * ny * nz * steps / dtp, "cores": threads,
+        "sample": f"same sample, the oracle port with x-fastest loops and OpenMP, {dtp:.2f} s",
+        "bitwise_equal_to_reference": bool(np.array_equal(g_port["P1"], g_ref["P1"]) and
+                                           np.array_equal(g_port["S2"], g_ref["S2"]))}
+    return out
+
+
+def load_traffic(workload_tag, kernel):
+    path = os.path.join(PROFILES, f"pmc_{kernel}_{workload_tag}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        return json.load(fh).get("bytes_per_launch")
+
+
 def mixed_errors(P64, S64, m64, Pm, Sm, mm):
     """Mixed-form errors against the FP64 form after the same steps."""
     import numpy as np
