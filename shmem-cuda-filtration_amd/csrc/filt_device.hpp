@@ -97,11 +97,17 @@ __device__ __forceinline__ bool same_bits(double a, double b)
 // lands on gives the full iteration's result bit for bit.
 //
 // kLean = true keeps the division and only the fixed-point exit: six fewer
-// live registers, for kernels at the edge of an occupancy step.
+// live registers, for kernels at the edge of an occupancy step.  Params is
+// filt_params, or filt_params in the kernel argument segment (KArg).
+template <bool kLean = false, class Params>
+__device__ inline void newton_cell(double& P, double& S, double roS1, double roS2, double lam,
+                                   double swr, double pd, const Params& p);
+
 // One Newton iteration of cpu.cpp:300-317 from (P, S) with the capillary
 // terms P_k and PkS.
+template <class Params>
 __device__ inline void newton_step(double P, double S, double P_k, double PkS, double roS1,
-                                   double roS2, const filt_params& p, double& Pn, double& Sn)
+                                   double roS2, const Params& p, double& Pn, double& Sn)
 {
     const double ro01 = p.ro01, ro02 = p.ro02, b1 = p.beta1, b2 = p.beta2, Pa = p.P_atm;
     const double F1 = ro01 * (1. + b1 * (P - Pa)) * (1. - S) - roS1;
@@ -115,9 +121,9 @@ __device__ inline void newton_step(double P, double S, double P_k, double PkS, d
     Sn = S - (1 / det) * (F1P * F2 - F2P * F1);
 }
 
-template <bool kLean = false>
+template <bool kLean, class Params>
 __device__ inline void newton_cell(double& P, double& S, double roS1, double roS2, double lam,
-                                   double swr, double pd, const filt_params& p)
+                                   double swr, double pd, const Params& p)
 {
     const bool no_cap = (pd == 0.0) && (lam >= 0.1);
     const double d = 1 - swr;

@@ -15,15 +15,20 @@
 //
 // Structure as in filt_stencil.hip: TX x TY tiles of columns march over a
 // chunk of planes; per plane the five stencil operands (P2, Xi1, Xi2, ro1,
-// ro2) of the tile and its one-cell ring go to one of two LDS tiles, the own
-// column's k-1, k, k+1 stay in registers, the ring and the point operands
-// of plane k+1 and the own column of plane k+2 are in flight while plane k
-// is computed, and one barrier orders each plane.  u is computed on every
-// plane of the grid; roS and the Newton on the interior.
+// ro2) of the tile and its one-cell ring go to one of two LDS tiles, the ring
+// and the point operands of plane k+1 and the own column of plane k+2 are in
+// flight while plane k is computed, and one barrier orders each plane.  The
+// own column's k and k+1 stay in registers; its k-1 is read back from this
+// thread's own slot of the other LDS tile, which no other thread writes.
+// Array pointers and parameters are read from the kernel argument segment
+// where they are used (filt_kernarg.hpp).  With both, the kernel fits 128
+// VGPRs and runs 4 waves per SIMD.  u is computed on every plane of the
+// grid; roS and the Newton on the interior.
 #include <hip/hip_runtime.h>
 
 #include "../../include/filtration_hip.h"
 #include "filt_device.hpp"
+#include "filt_kernarg.hpp"
 #include "filt_runtime.hpp"
 #include "filt_tiles.hpp"
 #include "filt_zmarch.hpp"
@@ -37,9 +42,13 @@ using namespace zm;
 constexpr int NA = 5;  // stencil operands: P2, Xi1, Xi2, ro1, ro2
 enum { AP = 0, AX1 = 1, AX2 = 2, AR1 = 3, AR2 = 4 };
 
+// Offsets of the kernel's first two arguments in the argument segment.
+constexpr unsigned long kArgF = 0;
+constexpr unsigned long kArgP = next_arg_offset<filt_fields, filt_params>(kArgF);
+
 template <int TX, int TY, bool WITH_U>
-__global__ __launch_bounds__(TX* TY) void k_step_zm(filt_fields f, filt_params p, int nx,
-                                                    TileMap map, int kz)
+__global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(4))) void k_step_zm(
+    filt_fields f, filt_params p, int nx, TileMap map, int kz)
 {
     using L = Lds<TX, TY>;
     __shared__ double shb[2][NA * L::SN];
@@ -65,15 +74,17 @@ __global__ __launch_bounds__(TX* TY) void k_step_zm(filt_fields f, filt_params p
     const bool per_cell_media = f.media && p.n_media > 1;
     // Own column at k-1, k, k+1 and (in flight) k+2; the ring cell and the
     // point operands (S2, porosity, medium) of the next plane.
-    double zm[NA] = {}, zc[NA] = {}, zp[NA] = {}, zq[NA] = {}, hn[NA] = {};
+    double zc[NA] = {}, zp[NA] = {}, zq[NA] = {}, hn[NA] = {};
     double Sc = 0, Sn = 0, phic = 0, phin = 0;
     int mdc = 0, mdn = 0;
     if (valid) {
         const unsigned c = col + kb * sz;
 #pragma unroll
         for (int a = 0; a < NA; a++) {
+            // plane kb-1 goes to the own slot of the tile that plane kb
+            // reads as its predecessor
             if (kb >= 1)
-                zm[a] = ld(A[a], c - sz);
+                shb[1][L::at(a, ty + 1, tx + 1)] = ld(A[a], c - sz);
             zc[a] = ld(A[a], c);
             if (kb + 1 < nz)
                 zp[a] = ld(A[a], c + sz);
@@ -99,111 +110,117 @@ __global__ __launch_bounds__(TX* TY) void k_step_zm(filt_fields f, filt_params p
 #pragma unroll
             for (int a = 0; a < NA; a++) sh[hslot + a * L::SN] = hn[a];
         if (k + 1 < ke) {  // the next plane is computed: fetch its operands
+            KArg<filt_fields>* F = kernarg<filt_fields>(kArgF);
+            const double* const B[NA] = {F->P2, F->Xi1, F->Xi2, F->ro1, F->ro2};
             if (valid) {
                 if (k + 2 < nz)
 #pragma unroll
-                    for (int a = 0; a < NA; a++) zq[a] = ld(A[a], c + 2 * sz);
-                Sn = ld(f.S2, c + sz);
-                if (f.poro)
-                    phin = ld(f.poro, c + sz);
+                    for (int a = 0; a < NA; a++) zq[a] = ld(B[a], c + 2 * sz);
+                Sn = ld(F->S2, c + sz);
+                if (F->poro)
+                    phin = ld(F->poro, c + sz);
                 if (per_cell_media)
-                    mdn = ldi(f.media, (c + sz) >> 1);
+                    mdn = ldi(F->media, (c + sz) >> 1);
             }
             if (hv)
 #pragma unroll
-                for (int a = 0; a < NA; a++) hn[a] = ld(A[a], hcol + (k + 1) * sz);
+                for (int a = 0; a < NA; a++) hn[a] = ld(B[a], hcol + (k + 1) * sz);
         }
         __syncthreads();
+        const double* const shm = shb[(k - kb + 1) & 1];  // own slot: plane k-1
         const double Pc = zc[AP];
         if (WITH_U && valid) {
             // u_calculation, cpu.cpp:77-160 (P1 = P2)
+            KArg<filt_fields>* F = kernarg<filt_fields>(kArgF);
+            KArg<filt_params>* Q = kernarg<filt_params>(kArgP);
             const double X1 = zc[AX1], X2 = zc[AX2];
             double u1, u2;
             if (i == 0) {
-                u1 = X1 * (sh[L::at(AP, cy, cx + 1)] - Pc) / p.h1;
-                u2 = X2 * (sh[L::at(AP, cy, cx + 1)] - Pc) / p.h1;
+                u1 = X1 * (sh[L::at(AP, cy, cx + 1)] - Pc) / Q->h1;
+                u2 = X2 * (sh[L::at(AP, cy, cx + 1)] - Pc) / Q->h1;
             } else if (i == nx - 1) {
-                u1 = X1 * (Pc - sh[L::at(AP, cy, cx - 1)]) / p.h1;
-                u2 = X2 * (Pc - sh[L::at(AP, cy, cx - 1)]) / p.h1;
+                u1 = X1 * (Pc - sh[L::at(AP, cy, cx - 1)]) / Q->h1;
+                u2 = X2 * (Pc - sh[L::at(AP, cy, cx - 1)]) / Q->h1;
             } else {
-                u1 = X1 * ((sh[L::at(AP, cy, cx + 1)] - sh[L::at(AP, cy, cx - 1)]) / (2. * p.h1));
-                u2 = X2 * ((sh[L::at(AP, cy, cx + 1)] - sh[L::at(AP, cy, cx - 1)]) / (2. * p.h1));
+                u1 = X1 * ((sh[L::at(AP, cy, cx + 1)] - sh[L::at(AP, cy, cx - 1)]) / (2. * Q->h1));
+                u2 = X2 * ((sh[L::at(AP, cy, cx + 1)] - sh[L::at(AP, cy, cx - 1)]) / (2. * Q->h1));
             }
-            st(f.u1x, c, u1);
-            st(f.u2x, c, u2);
+            st(F->u1x, c, u1);
+            st(F->u2x, c, u2);
             const double r1 = zc[AR1], r2 = zc[AR2];
             if (j == 0) {
-                u1 = X1 * ((sh[L::at(AP, cy + 1, cx)] - Pc) / p.h2 - r1 * g);
-                u2 = X2 * ((sh[L::at(AP, cy + 1, cx)] - Pc) / p.h2 - r2 * g);
+                u1 = X1 * ((sh[L::at(AP, cy + 1, cx)] - Pc) / Q->h2 - r1 * g);
+                u2 = X2 * ((sh[L::at(AP, cy + 1, cx)] - Pc) / Q->h2 - r2 * g);
             } else if (j == ny - 1) {
-                u1 = X1 * ((Pc - sh[L::at(AP, cy - 1, cx)]) / p.h2 - r1 * g);
-                u2 = X2 * ((Pc - sh[L::at(AP, cy - 1, cx)]) / p.h2 - r2 * g);
+                u1 = X1 * ((Pc - sh[L::at(AP, cy - 1, cx)]) / Q->h2 - r1 * g);
+                u2 = X2 * ((Pc - sh[L::at(AP, cy - 1, cx)]) / Q->h2 - r2 * g);
             } else {
-                u1 = X1 * ((sh[L::at(AP, cy + 1, cx)] - sh[L::at(AP, cy - 1, cx)]) / (2. * p.h2) -
+                u1 = X1 * ((sh[L::at(AP, cy + 1, cx)] - sh[L::at(AP, cy - 1, cx)]) / (2. * Q->h2) -
                            r1 * g);
-                u2 = X2 * ((sh[L::at(AP, cy + 1, cx)] - sh[L::at(AP, cy - 1, cx)]) / (2. * p.h2) -
+                u2 = X2 * ((sh[L::at(AP, cy + 1, cx)] - sh[L::at(AP, cy - 1, cx)]) / (2. * Q->h2) -
                            r2 * g);
             }
-            st(f.u1y, c, u1);
-            st(f.u2y, c, u2);
+            st(F->u1y, c, u1);
+            st(F->u2y, c, u2);
             if (k == 0) {
-                u1 = X1 * (zp[AP] - Pc) / p.h3;
-                u2 = X2 * (zp[AP] - Pc) / p.h3;
+                u1 = X1 * (zp[AP] - Pc) / Q->h3;
+                u2 = X2 * (zp[AP] - Pc) / Q->h3;
             } else if (k == nz - 1) {
-                u1 = X1 * (Pc - zm[AP]) / p.h3;
-                u2 = X2 * (Pc - zm[AP]) / p.h3;
+                u1 = X1 * (Pc - shm[L::at(AP, cy, cx)]) / Q->h3;
+                u2 = X2 * (Pc - shm[L::at(AP, cy, cx)]) / Q->h3;
             } else {
-                u1 = X1 * ((zp[AP] - zm[AP]) / (2. * p.h3));
-                u2 = X2 * ((zp[AP] - zm[AP]) / (2. * p.h3));
+                u1 = X1 * ((zp[AP] - shm[L::at(AP, cy, cx)]) / (2. * Q->h3));
+                u2 = X2 * ((zp[AP] - shm[L::at(AP, cy, cx)]) / (2. * Q->h3));
             }
-            st(f.u1z, c, u1);
-            st(f.u2z, c, u2);
+            st(F->u1z, c, u1);
+            st(F->u2z, c, u2);
         }
         if (valid && inner && k >= 1 && k <= nz - 2) {
             // roS_calculation, cpu.cpp:218-291 (P1 = P2)
+            KArg<filt_fields>* F = kernarg<filt_fields>(kArgF);
+            KArg<filt_params>* Q = kernarg<filt_params>(kArgP);
             const double X1c = zc[AX1], X2c = zc[AX2], r1c = zc[AR1], r2c = zc[AR2];
             const double S = Sc;
             const double roS1 = r1c * (1 - S);
             const double roS2 = r2c * S;
             double a1, a2;
-            a2 = -(zp[AP] - Pc) / p.h3;
-            a1 = -(Pc - zm[AP]) / p.h3;
-            const double f3 = upwind(a2, a1, X1c, r1c, zm[AX1], zm[AR1], zp[AX1], zp[AR1], p.h3);
-            const double g3 = upwind(a2, a1, X2c, r2c, zm[AX2], zm[AR2], zp[AX2], zp[AR2], p.h3);
+            a2 = -(zp[AP] - Pc) / Q->h3;
+            a1 = -(Pc - shm[L::at(AP, cy, cx)]) / Q->h3;
+            const double f3 = upwind(a2, a1, X1c, r1c, shm[L::at(AX1, cy, cx)], shm[L::at(AR1, cy, cx)], zp[AX1], zp[AR1], Q->h3);
+            const double g3 = upwind(a2, a1, X2c, r2c, shm[L::at(AX2, cy, cx)], shm[L::at(AR2, cy, cx)], zp[AX2], zp[AR2], Q->h3);
             const int xp = cx + 1, xm = cx - 1, yp = cy + 1, ym = cy - 1;
-            a2 = -(sh[L::at(AP, cy, xp)] - Pc) / p.h1;
-            a1 = -(Pc - sh[L::at(AP, cy, xm)]) / p.h1;
+            a2 = -(sh[L::at(AP, cy, xp)] - Pc) / Q->h1;
+            a1 = -(Pc - sh[L::at(AP, cy, xm)]) / Q->h1;
             const double f1 = upwind(a2, a1, X1c, r1c, sh[L::at(AX1, cy, xm)],
                                      sh[L::at(AR1, cy, xm)], sh[L::at(AX1, cy, xp)],
-                                     sh[L::at(AR1, cy, xp)], p.h1);
+                                     sh[L::at(AR1, cy, xp)], Q->h1);
             const double g1 = upwind(a2, a1, X2c, r2c, sh[L::at(AX2, cy, xm)],
                                      sh[L::at(AR2, cy, xm)], sh[L::at(AX2, cy, xp)],
-                                     sh[L::at(AR2, cy, xp)], p.h1);
-            const double dy2 = -(sh[L::at(AP, yp, cx)] - Pc) / p.h2;
-            const double dy1 = -(Pc - sh[L::at(AP, ym, cx)]) / p.h2;
+                                     sh[L::at(AR2, cy, xp)], Q->h1);
+            const double dy2 = -(sh[L::at(AP, yp, cx)] - Pc) / Q->h2;
+            const double dy1 = -(Pc - sh[L::at(AP, ym, cx)]) / Q->h2;
             const double f2 = upwind(dy2 + r1c * g, dy1 + r1c * g, X1c, r1c,
                                      sh[L::at(AX1, ym, cx)], sh[L::at(AR1, ym, cx)],
-                                     sh[L::at(AX1, yp, cx)], sh[L::at(AR1, yp, cx)], p.h2);
+                                     sh[L::at(AX1, yp, cx)], sh[L::at(AR1, yp, cx)], Q->h2);
             const double g2 = upwind(dy2 + r2c * g, dy1 + r2c * g, X2c, r2c,
                                      sh[L::at(AX2, ym, cx)], sh[L::at(AR2, ym, cx)],
-                                     sh[L::at(AX2, yp, cx)], sh[L::at(AR2, yp, cx)], p.h2);
+                                     sh[L::at(AX2, yp, cx)], sh[L::at(AR2, yp, cx)], Q->h2);
             const int md = mdc;
-            const double phi = f.poro ? phic : p.m[md];
-            const double A1 = roS1 - (p.dt / phi) * (f1 + f2 + f3);
-            const double A2 = roS2 - (p.dt / phi) * (g1 + g2 + g3);
-            st(f.roS1_old, c, roS1);
-            st(f.roS2_old, c, roS2);
-            st(f.roS1, c, A1);
-            st(f.roS2, c, A2);
+            const double phi = F->poro ? phic : Q->m[md];
+            const double A1 = roS1 - (Q->dt / phi) * (f1 + f2 + f3);
+            const double A2 = roS2 - (Q->dt / phi) * (g1 + g2 + g3);
+            st(F->roS1_old, c, roS1);
+            st(F->roS2_old, c, roS2);
+            st(F->roS1, c, A1);
+            st(F->roS2, c, A2);
             // P1_S2_calculation, cpu.cpp:293-320, on this cell's registers
             double P = Pc, Sv = S;
-            newton_cell(P, Sv, A1, A2, p.lambda[md], p.S_wr[md], p.P_d[md], p);
-            st(f.P1, c, P);
-            st(f.S2, c, Sv);
+            newton_cell(P, Sv, A1, A2, Q->lambda[md], Q->S_wr[md], Q->P_d[md], *Q);
+            st(F->P1, c, P);
+            st(F->S2, c, Sv);
         }
 #pragma unroll
         for (int a = 0; a < NA; a++) {
-            zm[a] = zc[a];
             zc[a] = zp[a];
             zp[a] = zq[a];
         }
