@@ -70,6 +70,72 @@ __device__ inline double upwind(double a2, double a1, double Xc, double rc, doub
             (a2 - fabs(a2)) / 2.0 * (-1) * Xp * rp) / h;
 }
 
+// Division by a grid spacing, bit for bit as the IEEE division n / d.
+//
+// hipcc expands an FP64 division into: v_div_scale of d and of n, y0 =
+// v_rcp_f64(d), two Newton steps y <- fma(y, fma(-d, y, 1), y), q = n*y,
+// r = fma(-d, q, n), v_div_fmas(r, y, q) = fma(r, y, q) (times 2^+-64 when
+// v_div_scale scaled), then v_div_fixup, which handles zeros, infinities,
+// NaN, overflow and underflow and otherwise returns its input with the
+// quotient's sign.  v_div_scale scales only for a denormal d or 1/d, a
+// denormal quotient, exponent(n) - exponent(d) >= 768, or a biased
+// exponent(n) <= 53.  For 2^-100 <= d <= 2^101 and 2^-600 <= |n| < 2^601 none
+// of that happens, so the expansion is exactly the sequence below with the
+// reciprocal computed once per divisor: three FP64 operations per division
+// instead of a v_rcp_f64, eight more FP64 operations and the two scale steps.
+// Zeros (q = n*y keeps n's sign, d > 0) take q; any other numerator outside
+// the range takes the division itself.  A probe over 4.3e9 (numerator,
+// divisor) pairs, specials and significands of all ones included, found no
+// difference on MI355X.
+__device__ __forceinline__ double refined_reciprocal(double d)
+{
+    double y = __builtin_amdgcn_rcp(d);
+    double e = fma(-d, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-d, y, 1.0);
+    return fma(y, e, y);
+}
+
+// The rare numerators (infinities, NaN, |n| >= 2^601, 0 < |n| < 2^-600):
+// out of line, so the registers of the full division sequence are not
+// reserved across the kernels that use DivBy.
+__device__ __attribute__((noinline)) double div_rare(double n, double d);
+
+struct DivBy {
+    double d, y;  // divisor in [2^-100, 2^101], y = refined_reciprocal(d)
+
+    __device__ __forceinline__ double operator()(double n) const
+    {
+        const double q = n * y;
+        const double r = fma(-d, q, n);
+        double f = fma(r, y, q);
+        const unsigned e = ((unsigned)__double2hiint(n) >> 20) & 0x7ffu;
+        const bool zero = n == 0.0;
+        f = zero ? q : f;
+        if (!(e - 423u <= 1200u || zero))  // |n| outside [2^-600, 2^601)
+            f = div_rare(n, d);
+        return f;
+    }
+};
+
+__device__ __attribute__((noinline)) inline double div_rare(double n, double d) { return n / d; }
+
+// The divisors DivBy accepts.
+__host__ __device__ inline bool divby_ok(double d)
+{
+    return d >= 0x1p-100 && d <= 0x1p101;
+}
+
+// upwind() with the division by h done by div.
+template <class Div>
+__device__ inline double upwind_div(double a2, double a1, double Xc, double rc, double Xm,
+                                    double rm, double Xp, double rp, const Div& div)
+{
+    return div(((a2 + fabs(a2)) / 2.0 - (a1 - fabs(a1)) / 2.0) * (-1) * Xc * rc -
+               (a1 + fabs(a1)) / 2.0 * (-1) * Xm * rm +
+               (a2 - fabs(a2)) / 2.0 * (-1) * Xp * rp);
+}
+
 // Bit-pattern equality (also for NaN, and telling -0 from +0).
 __device__ __forceinline__ bool same_bits(double a, double b)
 {

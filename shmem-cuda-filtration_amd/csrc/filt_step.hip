@@ -21,9 +21,11 @@
 // own column's k and k+1 stay in registers; its k-1 is read back from this
 // thread's own slot of the other LDS tile, which no other thread writes.
 // Array pointers and parameters are read from the kernel argument segment
-// where they are used (filt_kernarg.hpp).  With both, the kernel fits 128
-// VGPRs and runs 4 waves per SIMD.  u is computed on every plane of the
-// grid; roS and the Newton on the interior.
+// where they are used (filt_kernarg.hpp).  The 15 divisions by h1, h2, h3
+// (or their doubles) per cell go through DivBy (filt_device.hpp): three FP64
+// operations each, bit for bit the IEEE division.  The kernel fits 107 VGPRs
+// and runs 4 waves per SIMD.  u is computed on every plane of the grid; roS
+// and the Newton on the interior.
 #include <hip/hip_runtime.h>
 
 #include "../../include/filtration_hip.h"
@@ -100,6 +102,17 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(4))) voi
         for (int a = 0; a < NA; a++) hn[a] = ld(A[a], hcol + kb * sz);
     const int cx = tx + 1, cy = ty + 1;
     const double g = p.g_const;
+    // reciprocals of h1, h2, h3, 2h1, 2h2, 2h3 for DivBy; read after the
+    // first barrier of the loop
+    __shared__ double rh[6];
+    if (tx == 0 && ty == 0) {
+        rh[0] = refined_reciprocal(p.h1);
+        rh[1] = refined_reciprocal(p.h2);
+        rh[2] = refined_reciprocal(p.h3);
+        rh[3] = refined_reciprocal(2. * p.h1);
+        rh[4] = refined_reciprocal(2. * p.h2);
+        rh[5] = refined_reciprocal(2. * p.h3);
+    }
     for (int k = kb; k < ke; k++) {
         double* const sh = shb[(k - kb) & 1];
         const unsigned c = col + k * sz;
@@ -133,44 +146,50 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(4))) voi
             // u_calculation, cpu.cpp:77-160 (P1 = P2)
             KArg<filt_fields>* F = kernarg<filt_fields>(kArgF);
             KArg<filt_params>* Q = kernarg<filt_params>(kArgP);
+            // division by h and 2h, bit for bit as n / h (filt_device.hpp)
+            const DivBy dx{Q->h1, rh[0]}, dy{Q->h2, rh[1]}, dz{Q->h3, rh[2]};
+            const DivBy dx2{2. * Q->h1, rh[3]}, dy2{2. * Q->h2, rh[4]}, dz2{2. * Q->h3, rh[5]};
             const double X1 = zc[AX1], X2 = zc[AX2];
             double u1, u2;
             if (i == 0) {
-                u1 = X1 * (sh[L::at(AP, cy, cx + 1)] - Pc) / Q->h1;
-                u2 = X2 * (sh[L::at(AP, cy, cx + 1)] - Pc) / Q->h1;
+                u1 = dx(X1 * (sh[L::at(AP, cy, cx + 1)] - Pc));
+                u2 = dx(X2 * (sh[L::at(AP, cy, cx + 1)] - Pc));
             } else if (i == nx - 1) {
-                u1 = X1 * (Pc - sh[L::at(AP, cy, cx - 1)]) / Q->h1;
-                u2 = X2 * (Pc - sh[L::at(AP, cy, cx - 1)]) / Q->h1;
+                u1 = dx(X1 * (Pc - sh[L::at(AP, cy, cx - 1)]));
+                u2 = dx(X2 * (Pc - sh[L::at(AP, cy, cx - 1)]));
             } else {
-                u1 = X1 * ((sh[L::at(AP, cy, cx + 1)] - sh[L::at(AP, cy, cx - 1)]) / (2. * Q->h1));
-                u2 = X2 * ((sh[L::at(AP, cy, cx + 1)] - sh[L::at(AP, cy, cx - 1)]) / (2. * Q->h1));
+                const double gx = dx2(sh[L::at(AP, cy, cx + 1)] - sh[L::at(AP, cy, cx - 1)]);
+                u1 = X1 * gx;
+                u2 = X2 * gx;
             }
             st(F->u1x, c, u1);
             st(F->u2x, c, u2);
             const double r1 = zc[AR1], r2 = zc[AR2];
             if (j == 0) {
-                u1 = X1 * ((sh[L::at(AP, cy + 1, cx)] - Pc) / Q->h2 - r1 * g);
-                u2 = X2 * ((sh[L::at(AP, cy + 1, cx)] - Pc) / Q->h2 - r2 * g);
+                const double gy = dy(sh[L::at(AP, cy + 1, cx)] - Pc);
+                u1 = X1 * (gy - r1 * g);
+                u2 = X2 * (gy - r2 * g);
             } else if (j == ny - 1) {
-                u1 = X1 * ((Pc - sh[L::at(AP, cy - 1, cx)]) / Q->h2 - r1 * g);
-                u2 = X2 * ((Pc - sh[L::at(AP, cy - 1, cx)]) / Q->h2 - r2 * g);
+                const double gy = dy(Pc - sh[L::at(AP, cy - 1, cx)]);
+                u1 = X1 * (gy - r1 * g);
+                u2 = X2 * (gy - r2 * g);
             } else {
-                u1 = X1 * ((sh[L::at(AP, cy + 1, cx)] - sh[L::at(AP, cy - 1, cx)]) / (2. * Q->h2) -
-                           r1 * g);
-                u2 = X2 * ((sh[L::at(AP, cy + 1, cx)] - sh[L::at(AP, cy - 1, cx)]) / (2. * Q->h2) -
-                           r2 * g);
+                const double gy = dy2(sh[L::at(AP, cy + 1, cx)] - sh[L::at(AP, cy - 1, cx)]);
+                u1 = X1 * (gy - r1 * g);
+                u2 = X2 * (gy - r2 * g);
             }
             st(F->u1y, c, u1);
             st(F->u2y, c, u2);
             if (k == 0) {
-                u1 = X1 * (zp[AP] - Pc) / Q->h3;
-                u2 = X2 * (zp[AP] - Pc) / Q->h3;
+                u1 = dz(X1 * (zp[AP] - Pc));
+                u2 = dz(X2 * (zp[AP] - Pc));
             } else if (k == nz - 1) {
-                u1 = X1 * (Pc - shm[L::at(AP, cy, cx)]) / Q->h3;
-                u2 = X2 * (Pc - shm[L::at(AP, cy, cx)]) / Q->h3;
+                u1 = dz(X1 * (Pc - shm[L::at(AP, cy, cx)]));
+                u2 = dz(X2 * (Pc - shm[L::at(AP, cy, cx)]));
             } else {
-                u1 = X1 * ((zp[AP] - shm[L::at(AP, cy, cx)]) / (2. * Q->h3));
-                u2 = X2 * ((zp[AP] - shm[L::at(AP, cy, cx)]) / (2. * Q->h3));
+                const double gz = dz2(zp[AP] - shm[L::at(AP, cy, cx)]);
+                u1 = X1 * gz;
+                u2 = X2 * gz;
             }
             st(F->u1z, c, u1);
             st(F->u2z, c, u2);
@@ -179,32 +198,35 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(4))) voi
             // roS_calculation, cpu.cpp:218-291 (P1 = P2)
             KArg<filt_fields>* F = kernarg<filt_fields>(kArgF);
             KArg<filt_params>* Q = kernarg<filt_params>(kArgP);
+            const DivBy dx{Q->h1, rh[0]}, dy{Q->h2, rh[1]}, dz{Q->h3, rh[2]};
             const double X1c = zc[AX1], X2c = zc[AX2], r1c = zc[AR1], r2c = zc[AR2];
             const double S = Sc;
             const double roS1 = r1c * (1 - S);
             const double roS2 = r2c * S;
             double a1, a2;
-            a2 = -(zp[AP] - Pc) / Q->h3;
-            a1 = -(Pc - shm[L::at(AP, cy, cx)]) / Q->h3;
-            const double f3 = upwind(a2, a1, X1c, r1c, shm[L::at(AX1, cy, cx)], shm[L::at(AR1, cy, cx)], zp[AX1], zp[AR1], Q->h3);
-            const double g3 = upwind(a2, a1, X2c, r2c, shm[L::at(AX2, cy, cx)], shm[L::at(AR2, cy, cx)], zp[AX2], zp[AR2], Q->h3);
+            a2 = dz(-(zp[AP] - Pc));
+            a1 = dz(-(Pc - shm[L::at(AP, cy, cx)]));
+            const double f3 = upwind_div(a2, a1, X1c, r1c, shm[L::at(AX1, cy, cx)],
+                                         shm[L::at(AR1, cy, cx)], zp[AX1], zp[AR1], dz);
+            const double g3 = upwind_div(a2, a1, X2c, r2c, shm[L::at(AX2, cy, cx)],
+                                         shm[L::at(AR2, cy, cx)], zp[AX2], zp[AR2], dz);
             const int xp = cx + 1, xm = cx - 1, yp = cy + 1, ym = cy - 1;
-            a2 = -(sh[L::at(AP, cy, xp)] - Pc) / Q->h1;
-            a1 = -(Pc - sh[L::at(AP, cy, xm)]) / Q->h1;
-            const double f1 = upwind(a2, a1, X1c, r1c, sh[L::at(AX1, cy, xm)],
-                                     sh[L::at(AR1, cy, xm)], sh[L::at(AX1, cy, xp)],
-                                     sh[L::at(AR1, cy, xp)], Q->h1);
-            const double g1 = upwind(a2, a1, X2c, r2c, sh[L::at(AX2, cy, xm)],
-                                     sh[L::at(AR2, cy, xm)], sh[L::at(AX2, cy, xp)],
-                                     sh[L::at(AR2, cy, xp)], Q->h1);
-            const double dy2 = -(sh[L::at(AP, yp, cx)] - Pc) / Q->h2;
-            const double dy1 = -(Pc - sh[L::at(AP, ym, cx)]) / Q->h2;
-            const double f2 = upwind(dy2 + r1c * g, dy1 + r1c * g, X1c, r1c,
-                                     sh[L::at(AX1, ym, cx)], sh[L::at(AR1, ym, cx)],
-                                     sh[L::at(AX1, yp, cx)], sh[L::at(AR1, yp, cx)], Q->h2);
-            const double g2 = upwind(dy2 + r2c * g, dy1 + r2c * g, X2c, r2c,
-                                     sh[L::at(AX2, ym, cx)], sh[L::at(AR2, ym, cx)],
-                                     sh[L::at(AX2, yp, cx)], sh[L::at(AR2, yp, cx)], Q->h2);
+            a2 = dx(-(sh[L::at(AP, cy, xp)] - Pc));
+            a1 = dx(-(Pc - sh[L::at(AP, cy, xm)]));
+            const double f1 = upwind_div(a2, a1, X1c, r1c, sh[L::at(AX1, cy, xm)],
+                                         sh[L::at(AR1, cy, xm)], sh[L::at(AX1, cy, xp)],
+                                         sh[L::at(AR1, cy, xp)], dx);
+            const double g1 = upwind_div(a2, a1, X2c, r2c, sh[L::at(AX2, cy, xm)],
+                                         sh[L::at(AR2, cy, xm)], sh[L::at(AX2, cy, xp)],
+                                         sh[L::at(AR2, cy, xp)], dx);
+            const double dy2 = dy(-(sh[L::at(AP, yp, cx)] - Pc));
+            const double dy1 = dy(-(Pc - sh[L::at(AP, ym, cx)]));
+            const double f2 = upwind_div(dy2 + r1c * g, dy1 + r1c * g, X1c, r1c,
+                                         sh[L::at(AX1, ym, cx)], sh[L::at(AR1, ym, cx)],
+                                         sh[L::at(AX1, yp, cx)], sh[L::at(AR1, yp, cx)], dy);
+            const double g2 = upwind_div(dy2 + r2c * g, dy1 + r2c * g, X2c, r2c,
+                                         sh[L::at(AX2, ym, cx)], sh[L::at(AR2, ym, cx)],
+                                         sh[L::at(AX2, yp, cx)], sh[L::at(AR2, yp, cx)], dy);
             const int md = mdc;
             const double phi = F->poro ? phic : Q->m[md];
             const double A1 = roS1 - (Q->dt / phi) * (f1 + f2 + f3);
@@ -249,6 +271,7 @@ bool step_fusable(const filt_params& p, int nx, int rank, int size)
 {
     return size == 1 && nx == p.NX && nx >= 3 && p.Ny >= 3 && p.Nz >= 3 &&
            tiled_fits(nx, p.Ny, p.Nz) && p.roS_scheme == FILT_SCHEME_UPWIND &&
+           divby_ok(p.h1) && divby_ok(p.h2) && divby_ok(p.h3) &&
            tiled_enabled(2);
 }
 

@@ -6,7 +6,9 @@ bit-exact; stages that call pow (ro_P2_Xi, Newton with P_d != 0) agree to
 1e-13 relative per element (ROCm's pow vs glibc's, ulp level); trajectories
 agree to relative L-inf <= 1e-10 on P1 and S2 and to 1e-12 relative on total
 water / NAPL mass."""
+import json
 import os
+import subprocess
 
 import numpy as np
 import pytest
@@ -14,6 +16,9 @@ import pytest
 from cases import FIELD_NAMES_ALL, HostGrid, c1_params, max_rel, random_state, ref_params
 
 pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG_DIR = "shmem-cuda-filtration_amd"
 
 STAGE_OUT = {1: ("P2", "Xi1", "Xi2", "ro1", "ro2"),
              2: ("u1x", "u1y", "u1z", "u2x", "u2y", "u2z"),
@@ -349,6 +354,40 @@ def test_fused_step_equals_stage_calls(hip, params, dims, with_u):
     p = ref_params(*dims, source=1) if params == "ref" else bl_params(*dims)
     g = random_state(p, np.random.default_rng(dims[0] * 3 + with_u))
     fused_vs_staged(hip, p, g, 3, with_u)
+
+
+def test_divby_equals_ieee_division():
+    """The fused step's division by a grid spacing (DivBy, filt_device.hpp)
+    against n / d, bit for bit, on 2^24 numerators x 64 divisors."""
+    exe = os.path.join(ROOT, PKG_DIR, "lib", "divby_check")
+    assert os.path.exists(exe), "build the package first (make -C shmem-cuda-filtration_amd)"
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    line = r.stdout.strip().splitlines()[-1]
+    res = json.loads(line)
+    assert r.returncode == 0 and res["mismatches"] == 0, line
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_fused_step_extreme_magnitudes_and_specials(hip, seed):
+    """The fused step divides by h through a reciprocal for numerators in
+    [2^-600, 2^601) and by the division itself for zeros, tiny, huge and
+    non-finite ones.  States whose pressure differences and fluxes land in
+    every one of those classes: one step equals the stage calls bit for bit."""
+    p = ref_params(29, 17, 13, source=1)
+    rng = np.random.default_rng(50 + seed)
+    g = random_state(p, rng)
+    P = g["P1"]
+    cls = rng.integers(0, 8, size=P.shape)
+    mag = rng.uniform(1.0, 2.0, size=P.shape)
+    P[cls == 0] = 0.0
+    P[cls == 1] = (mag * 1e-300)[cls == 1]            # differences below 2^-600
+    P[cls == 2] = (mag * 5e-324 * 7)[cls == 2]        # subnormal pressures
+    P[cls == 3] = (mag * 1e200)[cls == 3]             # differences above 2^601
+    P[cls == 4] = -(mag * 1e180)[cls == 4]
+    special = rng.choice([np.inf, -np.inf, np.nan, -0.0], size=P.shape)
+    P[cls == 5] = special[cls == 5]
+    g["P1"] = P
+    fused_vs_staged(hip, p, g, 1, 1)
 
 
 def test_fused_step_per_cell_fields_and_trajectory(hip, orc):
