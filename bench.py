@@ -40,10 +40,11 @@ PROFILES = os.path.join(ROOT, "profiles")
 #   u:        Xi1, Xi2, ro1, ro2, P1, P2 in (48); six u out (48)        = 96
 #   roS:      S2, P1, P2, Xi1, Xi2, ro1, ro2, poro in (64); 4 out (32)  = 96
 #   Newton:   S2, P1, roS1, roS2 in (32); P1, S2 out (16)               = 48
-#   u + roS + Newton in one z-march (filt_u_roS_P1_S2_calculation):
-#             P2, Xi1, Xi2, ro1, ro2, S2, poro in (56); six u, four roS,
-#             P1, S2 out (96)                                           = 152
-PLAIN_BYTES = {"ro_P2_Xi": 64, "u": 96, "roS": 96, "newton": 48, "u_roS_newton": 152,
+#   u + roS + Newton in one z-march (filt_u_roS_P1_S2_calculation;
+#             ro1, ro2 follow from P2 as stage 1 computed them):
+#             P2, Xi1, Xi2, S2, poro in (40); six u, four roS,
+#             P1, S2 out (96)                                           = 136
+PLAIN_BYTES = {"ro_P2_Xi": 64, "u": 96, "roS": 96, "newton": 48, "u_roS_newton": 136,
                "boundary": 0}
 STEP_STAGES = ("ro_P2_Xi", "u_roS_newton", "boundary")     # time_step_function
 SPLIT_STAGES = ("ro_P2_Xi", "u", "roS", "newton", "boundary")  # the five stage calls

@@ -241,8 +241,9 @@ int filt_boundary_conditions(const filt_fields* dev, const filt_params* p,
  * been applied to (so P2 holds P1's bits, P2 = P1 + 0, cpu.cpp:63).  On one
  * rank whose grid has every extent >= 3 and arrays below 4 GiB, with the
  * upwind scheme, one z-marching kernel computes all three: it reads P2 where
- * the stages read P1 or P2, and writes every array the three stages write,
- * with the same bits, moving 152 B per cell instead of 240.  Elsewhere (or
+ * the stages read P1 or P2, evaluates ro1, ro2 from P2 as stage 1 did, and
+ * writes every array the three stages write, with the same bits, moving
+ * 136 B per cell instead of 240.  Elsewhere (or
  * after filt_set_tile_config(2, -1, ...)) the three stages run in turn. */
 int filt_u_roS_P1_S2_calculation(const filt_fields* dev, const filt_params* p,
                                  double t, int localNx, int rank, int size, int with_u);

@@ -44,6 +44,21 @@ struct Stage1 {
     double P2, Xi1, Xi2, ro1, ro2;
 };
 
+// Stage 1's densities (cpu.cpp:72-73).  ro2 is evaluated at P2 = P1 + 0,
+// which has P1's bits but for P1 = -0; the difference P - P_atm, and so the
+// density, is the same for -0 and +0.  So both follow from P2 alone.
+template <class Params>
+__device__ __forceinline__ double ro1_of(double P, const Params& p)
+{
+    return p.ro01 * (1. + p.beta1 * (P - p.P_atm));
+}
+
+template <class Params>
+__device__ __forceinline__ double ro2_of(double P, const Params& p)
+{
+    return p.ro02 * (1. + p.beta2 * (P - p.P_atm));
+}
+
 __device__ inline Stage1 stage1_cell(double P1, double S2, double K, double lam, double swr,
                                      const filt_params& p)
 {
@@ -55,8 +70,8 @@ __device__ inline Stage1 stage1_cell(double P1, double S2, double K, double lam,
     r.P2 = P1 + P_k;
     r.Xi1 = -1. * K * k1 / p.mu1;
     r.Xi2 = -1. * K * k2 / p.mu2;
-    r.ro1 = p.ro01 * (1. + p.beta1 * (P1 - p.P_atm));
-    r.ro2 = p.ro02 * (1. + p.beta2 * (r.P2 - p.P_atm));
+    r.ro1 = ro1_of(P1, p);
+    r.ro2 = ro2_of(r.P2, p);
     return r;
 }
 

@@ -7,23 +7,27 @@
 // bits (for every P1 but -0.0, which no pressure takes).  The kernel reads
 // P2 wherever the stages read P1 or P2; the Newton then rewrites P1 and S2
 // of its own cell in place without any neighbour needing their old values
-// (the stencils read P2, Xi and ro; S2 only at the centre).  Every array is
-// written with the values the three stages write one after another, so the
-// state after the step is the same bit for bit, while HBM sees 152 B per
-// cell instead of 240: the stencil operands are read once for u and roS,
-// roS is not read back by the Newton, and P1, S2 are not read twice.
+// (the stencils read P2, Xi and ro; S2 only at the centre).  ro1 and ro2
+// are not read either: stage 1 computed them from P1 and P2 alone, and the
+// same expressions on P2 give the same bits (ro1_of, ro2_of).  Every array
+// is written with the values the three stages write one after another, so
+// the state after the step is the same bit for bit, while HBM sees 136 B per
+// cell instead of 240: the stencil operands are read once for u and roS, the
+// densities not at all, roS is not read back by the Newton, and P1, S2 are
+// not read twice.
 //
 // Structure as in filt_stencil.hip: TX x TY tiles of columns march over a
 // chunk of planes; per plane the five stencil operands (P2, Xi1, Xi2, ro1,
 // ro2) of the tile and its one-cell ring go to one of two LDS tiles, the ring
 // and the point operands of plane k+1 and the own column of plane k+2 are in
 // flight while plane k is computed, and one barrier orders each plane.  The
-// own column's k and k+1 stay in registers; its k-1 is read back from this
+// own column's k and k+1 stay in registers (P2, Xi1, Xi2; the densities
+// are evaluated from P2 as the plane is staged); its k-1 is read back from this
 // thread's own slot of the other LDS tile, which no other thread writes.
 // Array pointers and parameters are read from the kernel argument segment
 // where they are used (filt_kernarg.hpp).  The 15 divisions by h1, h2, h3
 // (or their doubles) per cell go through DivBy (filt_device.hpp): three FP64
-// operations each, bit for bit the IEEE division.  The kernel fits 107 VGPRs
+// operations each, bit for bit the IEEE division.  The kernel fits 97 VGPRs
 // and runs 4 waves per SIMD.  u is computed on every plane of the grid; roS
 // and the Newton on the interior.
 #include <hip/hip_runtime.h>
@@ -41,7 +45,8 @@ namespace {
 
 using namespace zm;
 
-constexpr int NA = 5;  // stencil operands: P2, Xi1, Xi2, ro1, ro2
+constexpr int NA = 5;  // stencil operands in LDS: P2, Xi1, Xi2, ro1, ro2
+constexpr int NL = 3;  // of which loaded: P2, Xi1, Xi2 (ro1, ro2 follow from P2)
 enum { AP = 0, AX1 = 1, AX2 = 2, AR1 = 3, AR2 = 4 };
 
 // Offsets of the kernel's first two arguments in the argument segment.
@@ -68,7 +73,7 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(4))) voi
     const int ke = min(bz * kz + kz, WITH_U ? nz : nz - 1);
     if (kb >= ke)
         return;  // uniform over the block
-    const double* const A[NA] = {f.P2, f.Xi1, f.Xi2, f.ro1, f.ro2};
+    const double* const A[NL] = {f.P2, f.Xi1, f.Xi2};
     const unsigned col = 8u * i + sy * j;
     int hslot;
     unsigned hcol;
@@ -76,13 +81,13 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(4))) voi
     const bool per_cell_media = f.media && p.n_media > 1;
     // Own column at k-1, k, k+1 and (in flight) k+2; the ring cell and the
     // point operands (S2, porosity, medium) of the next plane.
-    double zc[NA] = {}, zp[NA] = {}, zq[NA] = {}, hn[NA] = {};
+    double zc[NL] = {}, zp[NL] = {}, zq[NL] = {}, hn[NL] = {};
     double Sc = 0, Sn = 0, phic = 0, phin = 0;
     int mdc = 0, mdn = 0;
     if (valid) {
         const unsigned c = col + kb * sz;
 #pragma unroll
-        for (int a = 0; a < NA; a++) {
+        for (int a = 0; a < NL; a++) {
             // plane kb-1 goes to the own slot of the tile that plane kb
             // reads as its predecessor
             if (kb >= 1)
@@ -90,6 +95,11 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(4))) voi
             zc[a] = ld(A[a], c);
             if (kb + 1 < nz)
                 zp[a] = ld(A[a], c + sz);
+        }
+        if (kb >= 1) {
+            const double Pm = shb[1][L::at(AP, ty + 1, tx + 1)];
+            shb[1][L::at(AR1, ty + 1, tx + 1)] = ro1_of(Pm, p);
+            shb[1][L::at(AR2, ty + 1, tx + 1)] = ro2_of(Pm, p);
         }
         Sc = ld(f.S2, c);
         if (f.poro)
@@ -99,7 +109,7 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(4))) voi
     }
     if (hv)
 #pragma unroll
-        for (int a = 0; a < NA; a++) hn[a] = ld(A[a], hcol + kb * sz);
+        for (int a = 0; a < NL; a++) hn[a] = ld(A[a], hcol + kb * sz);
     const int cx = tx + 1, cy = ty + 1;
     const double g = p.g_const;
     // reciprocals of h1, h2, h3, 2h1, 2h2, 2h3 for DivBy; read after the
@@ -116,19 +126,32 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(4))) voi
     for (int k = kb; k < ke; k++) {
         double* const sh = shb[(k - kb) & 1];
         const unsigned c = col + k * sz;
-        if (valid)
+        // densities of the own cell and of the ring cell, from P2
+        double r1c, r2c;
+        {
+            KArg<filt_params>* Q = kernarg<filt_params>(kArgP);
+            r1c = ro1_of(zc[AP], *Q);
+            r2c = ro2_of(zc[AP], *Q);
+            if (valid) {
 #pragma unroll
-            for (int a = 0; a < NA; a++) sh[L::at(a, cy, cx)] = zc[a];
-        if (hv)
+                for (int a = 0; a < NL; a++) sh[L::at(a, cy, cx)] = zc[a];
+                sh[L::at(AR1, cy, cx)] = r1c;
+                sh[L::at(AR2, cy, cx)] = r2c;
+            }
+            if (hv) {
 #pragma unroll
-            for (int a = 0; a < NA; a++) sh[hslot + a * L::SN] = hn[a];
+                for (int a = 0; a < NL; a++) sh[hslot + a * L::SN] = hn[a];
+                sh[hslot + AR1 * L::SN] = ro1_of(hn[AP], *Q);
+                sh[hslot + AR2 * L::SN] = ro2_of(hn[AP], *Q);
+            }
+        }
         if (k + 1 < ke) {  // the next plane is computed: fetch its operands
             KArg<filt_fields>* F = kernarg<filt_fields>(kArgF);
-            const double* const B[NA] = {F->P2, F->Xi1, F->Xi2, F->ro1, F->ro2};
+            const double* const B[NL] = {F->P2, F->Xi1, F->Xi2};
             if (valid) {
                 if (k + 2 < nz)
 #pragma unroll
-                    for (int a = 0; a < NA; a++) zq[a] = ld(B[a], c + 2 * sz);
+                    for (int a = 0; a < NL; a++) zq[a] = ld(B[a], c + 2 * sz);
                 Sn = ld(F->S2, c + sz);
                 if (F->poro)
                     phin = ld(F->poro, c + sz);
@@ -137,7 +160,7 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(4))) voi
             }
             if (hv)
 #pragma unroll
-                for (int a = 0; a < NA; a++) hn[a] = ld(B[a], hcol + (k + 1) * sz);
+                for (int a = 0; a < NL; a++) hn[a] = ld(B[a], hcol + (k + 1) * sz);
         }
         __syncthreads();
         const double* const shm = shb[(k - kb + 1) & 1];  // own slot: plane k-1
@@ -164,7 +187,7 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(4))) voi
             }
             st(F->u1x, c, u1);
             st(F->u2x, c, u2);
-            const double r1 = zc[AR1], r2 = zc[AR2];
+            const double r1 = r1c, r2 = r2c;
             if (j == 0) {
                 const double gy = dy(sh[L::at(AP, cy + 1, cx)] - Pc);
                 u1 = X1 * (gy - r1 * g);
@@ -199,17 +222,18 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(4))) voi
             KArg<filt_fields>* F = kernarg<filt_fields>(kArgF);
             KArg<filt_params>* Q = kernarg<filt_params>(kArgP);
             const DivBy dx{Q->h1, rh[0]}, dy{Q->h2, rh[1]}, dz{Q->h3, rh[2]};
-            const double X1c = zc[AX1], X2c = zc[AX2], r1c = zc[AR1], r2c = zc[AR2];
+            const double X1c = zc[AX1], X2c = zc[AX2];
             const double S = Sc;
             const double roS1 = r1c * (1 - S);
             const double roS2 = r2c * S;
             double a1, a2;
             a2 = dz(-(zp[AP] - Pc));
             a1 = dz(-(Pc - shm[L::at(AP, cy, cx)]));
+            const double r1p = ro1_of(zp[AP], *Q), r2p = ro2_of(zp[AP], *Q);
             const double f3 = upwind_div(a2, a1, X1c, r1c, shm[L::at(AX1, cy, cx)],
-                                         shm[L::at(AR1, cy, cx)], zp[AX1], zp[AR1], dz);
+                                         shm[L::at(AR1, cy, cx)], zp[AX1], r1p, dz);
             const double g3 = upwind_div(a2, a1, X2c, r2c, shm[L::at(AX2, cy, cx)],
-                                         shm[L::at(AR2, cy, cx)], zp[AX2], zp[AR2], dz);
+                                         shm[L::at(AR2, cy, cx)], zp[AX2], r2p, dz);
             const int xp = cx + 1, xm = cx - 1, yp = cy + 1, ym = cy - 1;
             a2 = dx(-(sh[L::at(AP, cy, xp)] - Pc));
             a1 = dx(-(Pc - sh[L::at(AP, cy, xm)]));
@@ -242,7 +266,7 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(4))) voi
             st(F->S2, c, Sv);
         }
 #pragma unroll
-        for (int a = 0; a < NA; a++) {
+        for (int a = 0; a < NL; a++) {
             zc[a] = zp[a];
             zp[a] = zq[a];
         }

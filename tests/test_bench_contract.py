@@ -27,7 +27,7 @@ def test_helpers_and_tables():
     for s in b.STEP_STAGES + b.SPLIT_STAGES:
         assert s in b.PLAIN_BYTES and s in b.KERNEL_NAMES, s
     assert sum(b.PLAIN_BYTES[s] for s in b.SPLIT_STAGES) == 304
-    assert sum(b.PLAIN_BYTES[s] for s in b.STEP_STAGES) == 216
+    assert sum(b.PLAIN_BYTES[s] for s in b.STEP_STAGES) == 200
 
 
 def test_every_global_name_resolves():
