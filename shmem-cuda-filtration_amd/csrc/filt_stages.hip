@@ -58,11 +58,12 @@ __global__ __launch_bounds__(TX* TY) void k_ro_P2_Xi(filt_fields f, filt_params 
     const int md = media_of(f, p, e.c);
     const double K = f.perm ? f.perm[e.c] : p.K[md];
     const Stage1 r = stage1_cell(f.P1[e.c], f.S2[e.c], K, p.lambda[md], p.S_wr[md], p);
-    f.P2[e.c] = r.P2;
-    f.Xi1[e.c] = r.Xi1;
-    f.Xi2[e.c] = r.Xi2;
-    f.ro1[e.c] = r.ro1;
-    f.ro2[e.c] = r.ro2;
+    // non-temporal: the five arrays (40 B/cell) stream out past the caches
+    __builtin_nontemporal_store(r.P2, f.P2 + e.c);
+    __builtin_nontemporal_store(r.Xi1, f.Xi1 + e.c);
+    __builtin_nontemporal_store(r.Xi2, f.Xi2 + e.c);
+    __builtin_nontemporal_store(r.ro1, f.ro1 + e.c);
+    __builtin_nontemporal_store(r.ro2, f.ro2 + e.c);
 }
 
 // ---------------------------------------------------------------- stage 2 --
@@ -286,8 +287,8 @@ __global__ __launch_bounds__(TX* TY) void k_newton(filt_fields f, filt_params p,
     const int md = media_of(f, p, c);
     double P = f.P1[c], S = f.S2[c];
     newton_cell(P, S, f.roS1[c], f.roS2[c], p.lambda[md], p.S_wr[md], p.P_d[md], p);
-    f.P1[c] = P;
-    f.S2[c] = S;
+    __builtin_nontemporal_store(P, f.P1 + c);
+    __builtin_nontemporal_store(S, f.S2 + c);
 }
 
 // ---------------------------------------------------------------- stage 5 --

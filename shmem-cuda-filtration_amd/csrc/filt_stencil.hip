@@ -150,10 +150,10 @@ __global__ __launch_bounds__(TX* TY) void k_roS_zm(filt_fields f, filt_params p,
                                      sh[L::at(3, yp, cx)], sh[L::at(5, yp, cx)], p.h2);
 
             const double phi = f.poro ? phic : p.m[mdc];
-            st(f.roS1_old, c, roS1);
-            st(f.roS2_old, c, roS2);
-            st(f.roS1, c, roS1 - (p.dt / phi) * (f1 + f2 + f3));
-            st(f.roS2, c, roS2 - (p.dt / phi) * (g1 + g2 + g3));
+            stn(f.roS1_old, c, roS1);
+            stn(f.roS2_old, c, roS2);
+            stn(f.roS1, c, roS1 - (p.dt / phi) * (f1 + f2 + f3));
+            stn(f.roS2, c, roS2 - (p.dt / phi) * (g1 + g2 + g3));
         }
 #pragma unroll
         for (int a = 0; a < 6; a++) {
@@ -262,8 +262,8 @@ __global__ __launch_bounds__(TX* TY) void k_u_zm(filt_fields f, filt_params p, i
                 u1 = X1 * ((sh[L::at(0, cy, cx + 1)] - sh[L::at(0, cy, cx - 1)]) / (2. * p.h1));
                 u2 = X2 * ((sh[L::at(1, cy, cx + 1)] - sh[L::at(1, cy, cx - 1)]) / (2. * p.h1));
             }
-            st(f.u1x, c, u1);
-            st(f.u2x, c, u2);
+            stn(f.u1x, c, u1);
+            stn(f.u2x, c, u2);
             // y
             const double r1 = r1c, r2 = r2c;
             if (j == 0) {
@@ -276,8 +276,8 @@ __global__ __launch_bounds__(TX* TY) void k_u_zm(filt_fields f, filt_params p, i
                 u1 = X1 * ((sh[L::at(0, cy + 1, cx)] - sh[L::at(0, cy - 1, cx)]) / (2. * p.h2) - r1 * g);
                 u2 = X2 * ((sh[L::at(1, cy + 1, cx)] - sh[L::at(1, cy - 1, cx)]) / (2. * p.h2) - r2 * g);
             }
-            st(f.u1y, c, u1);
-            st(f.u2y, c, u2);
+            stn(f.u1y, c, u1);
+            stn(f.u2y, c, u2);
             // z
             if (k == 0) {
                 u1 = X1 * (zp0 - P1c) / p.h3;
@@ -289,8 +289,8 @@ __global__ __launch_bounds__(TX* TY) void k_u_zm(filt_fields f, filt_params p, i
                 u1 = X1 * ((zp0 - zm0) / (2. * p.h3));
                 u2 = X2 * ((zp1 - zm1) / (2. * p.h3));
             }
-            st(f.u1z, c, u1);
-            st(f.u2z, c, u2);
+            stn(f.u1z, c, u1);
+            stn(f.u2z, c, u2);
         }
         zm0 = zc0;
         zm1 = zc1;

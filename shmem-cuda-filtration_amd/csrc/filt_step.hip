@@ -185,8 +185,8 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(4))) voi
                 u1 = X1 * gx;
                 u2 = X2 * gx;
             }
-            st(F->u1x, c, u1);
-            st(F->u2x, c, u2);
+            stn(F->u1x, c, u1);
+            stn(F->u2x, c, u2);
             const double r1 = r1c, r2 = r2c;
             if (j == 0) {
                 const double gy = dy(sh[L::at(AP, cy + 1, cx)] - Pc);
@@ -201,8 +201,8 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(4))) voi
                 u1 = X1 * (gy - r1 * g);
                 u2 = X2 * (gy - r2 * g);
             }
-            st(F->u1y, c, u1);
-            st(F->u2y, c, u2);
+            stn(F->u1y, c, u1);
+            stn(F->u2y, c, u2);
             if (k == 0) {
                 u1 = dz(X1 * (zp[AP] - Pc));
                 u2 = dz(X2 * (zp[AP] - Pc));
@@ -214,8 +214,8 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(4))) voi
                 u1 = X1 * gz;
                 u2 = X2 * gz;
             }
-            st(F->u1z, c, u1);
-            st(F->u2z, c, u2);
+            stn(F->u1z, c, u1);
+            stn(F->u2z, c, u2);
         }
         if (valid && inner && k >= 1 && k <= nz - 2) {
             // roS_calculation, cpu.cpp:218-291 (P1 = P2)
@@ -255,15 +255,15 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(4))) voi
             const double phi = F->poro ? phic : Q->m[md];
             const double A1 = roS1 - (Q->dt / phi) * (f1 + f2 + f3);
             const double A2 = roS2 - (Q->dt / phi) * (g1 + g2 + g3);
-            st(F->roS1_old, c, roS1);
-            st(F->roS2_old, c, roS2);
-            st(F->roS1, c, A1);
-            st(F->roS2, c, A2);
+            stn(F->roS1_old, c, roS1);
+            stn(F->roS2_old, c, roS2);
+            stn(F->roS1, c, A1);
+            stn(F->roS2, c, A2);
             // P1_S2_calculation, cpu.cpp:293-320, on this cell's registers
             double P = Pc, Sv = S;
             newton_cell(P, Sv, A1, A2, Q->lambda[md], Q->S_wr[md], Q->P_d[md], *Q);
-            st(F->P1, c, P);
-            st(F->S2, c, Sv);
+            stn(F->P1, c, P);
+            stn(F->S2, c, Sv);
         }
 #pragma unroll
         for (int a = 0; a < NL; a++) {
