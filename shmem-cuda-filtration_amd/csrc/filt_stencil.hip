@@ -18,7 +18,10 @@
 // read once per chunk.  Addresses are 32-bit byte offsets from the
 // (uniform) array bases, which keeps them in one VGPR each.  The arithmetic
 // is expression for expression the one of filt_stages.hip, so both paths
-// give identical bits.
+// give identical bits.  The roS kernel reads its own column's plane k-1 back
+// from its own slot of the other LDS tile and divides by h1, h2, h3 through
+// DivBy (filt_device.hpp, bit for bit the IEEE division), which fits it in
+// 115 VGPRs at 4 waves per SIMD.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -44,8 +47,8 @@ using namespace zm;
 // cpu.cpp:218-291 for interior cells; arrays P1, P2, Xi1, Xi2, ro1, ro2 are
 // the stencil operands (index 0..5 below).
 template <int TX, int TY>
-__global__ __launch_bounds__(TX* TY) void k_roS_zm(filt_fields f, filt_params p, int nx,
-                                                   TileMap map, int kz)
+__global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(4))) void k_roS_zm(
+    filt_fields f, filt_params p, int nx, TileMap map, int kz)
 {
     using L = Lds<TX, TY>;
     __shared__ double shb[2][6 * L::SN];
@@ -70,14 +73,16 @@ __global__ __launch_bounds__(TX* TY) void k_roS_zm(filt_fields f, filt_params p,
     const bool by_media = !f.poro && f.media && p.n_media > 1;
     // Pipeline registers: own column at k-1, k, k+1 and (in flight) k+2;
     // ring cell and point operands of the next plane.
-    double zm[6] = {}, zc[6] = {}, zp[6] = {}, zq[6] = {}, hn[6] = {};
+    double zc[6] = {}, zp[6] = {}, zq[6] = {}, hn[6] = {};
     double Sc = 0, Sn = 0, phic = 0, phin = 0;
     int mdc = 0, mdn = 0;
     if (valid) {
         const unsigned c = col + kb * sz;
 #pragma unroll
         for (int a = 0; a < 6; a++) {
-            zm[a] = ld(A[a], c - sz);
+            // plane kb-1 to the own slot of the tile plane kb reads as its
+            // predecessor (no other thread writes that slot)
+            shb[1][L::at(a, ty + 1, tx + 1)] = ld(A[a], c - sz);
             zc[a] = ld(A[a], c);
             zp[a] = ld(A[a], c + sz);  // kb + 1 <= ke <= nz - 1
         }
@@ -92,8 +97,15 @@ __global__ __launch_bounds__(TX* TY) void k_roS_zm(filt_fields f, filt_params p,
         for (int a = 0; a < 6; a++) hn[a] = ld(A[a], hcol + kb * sz);
     const int cx = tx + 1, cy = ty + 1;
     const double g = p.g_const;
+    __shared__ double rh[3];  // reciprocals of h1, h2, h3 for DivBy
+    if (tx == 0 && ty == 0) {
+        rh[0] = refined_reciprocal(p.h1);
+        rh[1] = refined_reciprocal(p.h2);
+        rh[2] = refined_reciprocal(p.h3);
+    }
     for (int k = kb; k < ke; k++) {
         double* const sh = shb[(k - kb) & 1];
+        const double* const shm = shb[(k - kb + 1) & 1];  // own slot: plane k-1
         const unsigned c = col + k * sz;
         if (valid)
 #pragma unroll
@@ -117,37 +129,45 @@ __global__ __launch_bounds__(TX* TY) void k_roS_zm(filt_fields f, filt_params p,
         }
         __syncthreads();
         if (valid && inner) {
+            const DivBy dx{p.h1, rh[0]}, dy{p.h2, rh[1]}, dz{p.h3, rh[2]};
+            double zm[6];
+#pragma unroll
+            for (int a = 0; a < 6; a++) zm[a] = shm[L::at(a, cy, cx)];
             const double p1 = zc[0], p2 = zc[1];
             const double X1c = zc[2], X2c = zc[3], r1c = zc[4], r2c = zc[5];
             const double S = Sc;
             const double roS1 = r1c * (1 - S);
             const double roS2 = r2c * S;
             double a1, a2;
-            a2 = -(zp[0] - p1) / p.h3;
-            a1 = -(p1 - zm[0]) / p.h3;
-            const double f3 = upwind(a2, a1, X1c, r1c, zm[2], zm[4], zp[2], zp[4], p.h3);
-            a2 = -(zp[1] - p2) / p.h3;
-            a1 = -(p2 - zm[1]) / p.h3;
-            const double g3 = upwind(a2, a1, X2c, r2c, zm[3], zm[5], zp[3], zp[5], p.h3);
+            a2 = dz(-(zp[0] - p1));
+            a1 = dz(-(p1 - zm[0]));
+            const double f3 = upwind_div(a2, a1, X1c, r1c, zm[2], zm[4], zp[2], zp[4], dz);
+            a2 = dz(-(zp[1] - p2));
+            a1 = dz(-(p2 - zm[1]));
+            const double g3 = upwind_div(a2, a1, X2c, r2c, zm[3], zm[5], zp[3], zp[5], dz);
 
             const int xp = cx + 1, xm = cx - 1, yp = cy + 1, ym = cy - 1;
-            a2 = -(sh[L::at(0, cy, xp)] - p1) / p.h1;
-            a1 = -(p1 - sh[L::at(0, cy, xm)]) / p.h1;
-            const double f1 = upwind(a2, a1, X1c, r1c, sh[L::at(2, cy, xm)], sh[L::at(4, cy, xm)],
-                                     sh[L::at(2, cy, xp)], sh[L::at(4, cy, xp)], p.h1);
-            a2 = -(sh[L::at(0, yp, cx)] - p1) / p.h2 + r1c * g;
-            a1 = -(p1 - sh[L::at(0, ym, cx)]) / p.h2 + r1c * g;
-            const double f2 = upwind(a2, a1, X1c, r1c, sh[L::at(2, ym, cx)], sh[L::at(4, ym, cx)],
-                                     sh[L::at(2, yp, cx)], sh[L::at(4, yp, cx)], p.h2);
+            a2 = dx(-(sh[L::at(0, cy, xp)] - p1));
+            a1 = dx(-(p1 - sh[L::at(0, cy, xm)]));
+            const double f1 = upwind_div(a2, a1, X1c, r1c, sh[L::at(2, cy, xm)],
+                                         sh[L::at(4, cy, xm)], sh[L::at(2, cy, xp)],
+                                         sh[L::at(4, cy, xp)], dx);
+            a2 = dy(-(sh[L::at(0, yp, cx)] - p1)) + r1c * g;
+            a1 = dy(-(p1 - sh[L::at(0, ym, cx)])) + r1c * g;
+            const double f2 = upwind_div(a2, a1, X1c, r1c, sh[L::at(2, ym, cx)],
+                                         sh[L::at(4, ym, cx)], sh[L::at(2, yp, cx)],
+                                         sh[L::at(4, yp, cx)], dy);
 
-            a2 = -(sh[L::at(1, cy, xp)] - p2) / p.h1;
-            a1 = -(p2 - sh[L::at(1, cy, xm)]) / p.h1;
-            const double g1 = upwind(a2, a1, X2c, r2c, sh[L::at(3, cy, xm)], sh[L::at(5, cy, xm)],
-                                     sh[L::at(3, cy, xp)], sh[L::at(5, cy, xp)], p.h1);
-            a2 = -(sh[L::at(1, yp, cx)] - p2) / p.h2 + r2c * g;
-            a1 = -(p2 - sh[L::at(1, ym, cx)]) / p.h2 + r2c * g;
-            const double g2 = upwind(a2, a1, X2c, r2c, sh[L::at(3, ym, cx)], sh[L::at(5, ym, cx)],
-                                     sh[L::at(3, yp, cx)], sh[L::at(5, yp, cx)], p.h2);
+            a2 = dx(-(sh[L::at(1, cy, xp)] - p2));
+            a1 = dx(-(p2 - sh[L::at(1, cy, xm)]));
+            const double g1 = upwind_div(a2, a1, X2c, r2c, sh[L::at(3, cy, xm)],
+                                         sh[L::at(5, cy, xm)], sh[L::at(3, cy, xp)],
+                                         sh[L::at(5, cy, xp)], dx);
+            a2 = dy(-(sh[L::at(1, yp, cx)] - p2)) + r2c * g;
+            a1 = dy(-(p2 - sh[L::at(1, ym, cx)])) + r2c * g;
+            const double g2 = upwind_div(a2, a1, X2c, r2c, sh[L::at(3, ym, cx)],
+                                         sh[L::at(5, ym, cx)], sh[L::at(3, yp, cx)],
+                                         sh[L::at(5, yp, cx)], dy);
 
             const double phi = f.poro ? phic : p.m[mdc];
             stn(f.roS1_old, c, roS1);
@@ -157,7 +177,6 @@ __global__ __launch_bounds__(TX* TY) void k_roS_zm(filt_fields f, filt_params p,
         }
 #pragma unroll
         for (int a = 0; a < 6; a++) {
-            zm[a] = zc[a];
             zc[a] = zp[a];
             zp[a] = zq[a];
         }
@@ -329,10 +348,10 @@ UFn pick_u(int tx, int ty)
 
 // Defaults: the fastest configurations of a sweep over every shape, kz in
 // {16, 64} (the fused step: {8, 16, 32, 64}) and both orders on a 256^3 grid
-// on MI355X.  The fused step's 64x8 tile runs two blocks of 8 waves per CU at
+// on MI355X (roS at 4 waves per SIMD: 64x8, kz 64).  The fused step's 64x8 tile runs two blocks of 8 waves per CU at
 // 4 waves per SIMD; kz 64 timed within 1% of kz 16, which keeps more blocks
 // for smaller grids.
-constexpr TileConfig kRoSDefault = {64, 4, 16, 1};
+constexpr TileConfig kRoSDefault = {64, 8, 64, 1};
 constexpr TileConfig kUDefault = {64, 8, 64, 1};
 constexpr TileConfig kStepDefault = {64, 8, 16, 1};
 TileConfig g_roS_cfg = kRoSDefault;
