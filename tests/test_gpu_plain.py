@@ -286,6 +286,38 @@ def test_untiled_stencils_3d_bitwise(hip, orc, params, dims):
             lib.filt_set_tile_config(kern, 0, 0, 0, 0)
 
 
+def extreme_state(p, rng):
+    """A random state whose pressures (P1 and P2) put the stencils' numerators
+    in every class DivBy treats apart: zero, below 2^-600, subnormal, above
+    2^601, infinite, NaN, and -0."""
+    g = random_state(p, rng)
+    for name in ("P1", "P2"):
+        P = g[name]
+        cls = rng.integers(0, 8, size=P.shape)
+        mag = rng.uniform(1.0, 2.0, size=P.shape)
+        P[cls == 0] = 0.0
+        P[cls == 1] = (mag * 1e-300)[cls == 1]
+        P[cls == 2] = (mag * 5e-324 * 7)[cls == 2]
+        P[cls == 3] = (mag * 1e200)[cls == 3]
+        P[cls == 4] = -(mag * 1e180)[cls == 4]
+        special = rng.choice([np.inf, -np.inf, np.nan, -0.0], size=P.shape)
+        P[cls == 5] = special[cls == 5]
+        g[name] = P
+    return g
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_roS_tiled_extreme_values_equal_oracle(hip, orc, seed):
+    """The tiled roS kernel divides by h through DivBy; on states with zero,
+    tiny, huge and non-finite pressure differences it still gives the
+    oracle's bits (NaN where the oracle has NaN)."""
+    p = ref_params(37, 21, 13, source=1)
+    g = extreme_state(p, np.random.default_rng(70 + seed))
+    out, want = run_stage_pair(hip, orc, p, g, 3)
+    for n in FIELD_NAMES_ALL:
+        assert np.array_equal(out[n], want[n], equal_nan=True), n
+
+
 @pytest.mark.parametrize("rank,size", [(0, 2), (1, 3), (2, 3)])
 @pytest.mark.parametrize("stage", [1, 2, 3, 4, 5])
 def test_stage_parity_multirank_slabs(hip, orc, rank, size, stage):
@@ -374,19 +406,7 @@ def test_fused_step_extreme_magnitudes_and_specials(hip, seed):
     non-finite ones.  States whose pressure differences and fluxes land in
     every one of those classes: one step equals the stage calls bit for bit."""
     p = ref_params(29, 17, 13, source=1)
-    rng = np.random.default_rng(50 + seed)
-    g = random_state(p, rng)
-    P = g["P1"]
-    cls = rng.integers(0, 8, size=P.shape)
-    mag = rng.uniform(1.0, 2.0, size=P.shape)
-    P[cls == 0] = 0.0
-    P[cls == 1] = (mag * 1e-300)[cls == 1]            # differences below 2^-600
-    P[cls == 2] = (mag * 5e-324 * 7)[cls == 2]        # subnormal pressures
-    P[cls == 3] = (mag * 1e200)[cls == 3]             # differences above 2^601
-    P[cls == 4] = -(mag * 1e180)[cls == 4]
-    special = rng.choice([np.inf, -np.inf, np.nan, -0.0], size=P.shape)
-    P[cls == 5] = special[cls == 5]
-    g["P1"] = P
+    g = extreme_state(p, np.random.default_rng(50 + seed))
     fused_vs_staged(hip, p, g, 1, 1)
 
 
