@@ -12,8 +12,8 @@
 // value the compiler cannot see through, so a member read through it is a
 // scalar load (a scalar-cache hit) at the point of use, and nothing stays
 // live across the loop.  Explicit kernel arguments start at offset 0 of the
-// segment, in declaration order, each at its natural alignment;
-// next_arg_offset gives the offset of the argument after one of type Prev.
+// segment, so a kernel that takes one struct argument reads it through
+// kernarg<Struct>(0) and its members at their C++ offsets.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -29,12 +29,6 @@ __device__ __forceinline__ KArg<T>* kernarg(unsigned long offset)
     KArg<T>* q = (KArg<T>*)((KArg<char>*)__builtin_amdgcn_kernarg_segment_ptr() + offset);
     asm volatile("" : "+s"(q));
     return q;
-}
-
-template <class Prev, class Next>
-constexpr unsigned long next_arg_offset(unsigned long prev_offset)
-{
-    return (prev_offset + sizeof(Prev) + alignof(Next) - 1) / alignof(Next) * alignof(Next);
 }
 
 }  // namespace filt

@@ -49,14 +49,26 @@ constexpr int NA = 5;  // stencil operands in LDS: P2, Xi1, Xi2, ro1, ro2
 constexpr int NL = 3;  // of which loaded: P2, Xi1, Xi2 (ro1, ro2 follow from P2)
 enum { AP = 0, AX1 = 1, AX2 = 2, AR1 = 3, AR2 = 4 };
 
-// Offsets of the kernel's first two arguments in the argument segment.
-constexpr unsigned long kArgF = 0;
-constexpr unsigned long kArgP = next_arg_offset<filt_fields, filt_params>(kArgF);
+// The kernel's one argument: at offset 0 of the argument segment, so that
+// the loop reads its members through kernarg<StepArgs>(0).
+struct StepArgs {
+    filt_fields f;
+    filt_params p;
+    int nx, kz;
+    TileMap map;
+};
+
+__device__ __forceinline__ KArg<filt_fields>* fields_arg() { return &kernarg<StepArgs>(0)->f; }
+__device__ __forceinline__ KArg<filt_params>* params_arg() { return &kernarg<StepArgs>(0)->p; }
 
 template <int TX, int TY, bool WITH_U>
 __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(4))) void k_step_zm(
-    filt_fields f, filt_params p, int nx, TileMap map, int kz)
+    StepArgs args)
 {
+    const filt_fields& f = args.f;
+    const filt_params& p = args.p;
+    const int nx = args.nx, kz = args.kz;
+    const TileMap& map = args.map;
     using L = Lds<TX, TY>;
     __shared__ double shb[2][NA * L::SN];
     int bx, by, bz;
@@ -129,7 +141,7 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(4))) voi
         // densities of the own cell and of the ring cell, from P2
         double r1c, r2c;
         {
-            KArg<filt_params>* Q = kernarg<filt_params>(kArgP);
+            KArg<filt_params>* Q = params_arg();
             r1c = ro1_of(zc[AP], *Q);
             r2c = ro2_of(zc[AP], *Q);
             if (valid) {
@@ -146,7 +158,7 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(4))) voi
             }
         }
         if (k + 1 < ke) {  // the next plane is computed: fetch its operands
-            KArg<filt_fields>* F = kernarg<filt_fields>(kArgF);
+            KArg<filt_fields>* F = fields_arg();
             const double* const B[NL] = {F->P2, F->Xi1, F->Xi2};
             if (valid) {
                 if (k + 2 < nz)
@@ -167,8 +179,8 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(4))) voi
         const double Pc = zc[AP];
         if (WITH_U && valid) {
             // u_calculation, cpu.cpp:77-160 (P1 = P2)
-            KArg<filt_fields>* F = kernarg<filt_fields>(kArgF);
-            KArg<filt_params>* Q = kernarg<filt_params>(kArgP);
+            KArg<filt_fields>* F = fields_arg();
+            KArg<filt_params>* Q = params_arg();
             // division by h and 2h, bit for bit as n / h (filt_device.hpp)
             const DivBy dx{Q->h1, rh[0]}, dy{Q->h2, rh[1]}, dz{Q->h3, rh[2]};
             const DivBy dx2{2. * Q->h1, rh[3]}, dy2{2. * Q->h2, rh[4]}, dz2{2. * Q->h3, rh[5]};
@@ -219,8 +231,8 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(4))) voi
         }
         if (valid && inner && k >= 1 && k <= nz - 2) {
             // roS_calculation, cpu.cpp:218-291 (P1 = P2)
-            KArg<filt_fields>* F = kernarg<filt_fields>(kArgF);
-            KArg<filt_params>* Q = kernarg<filt_params>(kArgP);
+            KArg<filt_fields>* F = fields_arg();
+            KArg<filt_params>* Q = params_arg();
             const DivBy dx{Q->h1, rh[0]}, dy{Q->h2, rh[1]}, dz{Q->h3, rh[2]};
             const double X1c = zc[AX1], X2c = zc[AX2];
             const double S = Sc;
@@ -277,7 +289,7 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(4))) voi
 }
 
 template <bool WITH_U>
-using StepFn = void (*)(filt_fields, filt_params, int, TileMap, int);
+using StepFn = void (*)(StepArgs);
 
 template <bool WITH_U>
 StepFn<WITH_U> pick_step(int tx, int ty)
@@ -302,13 +314,19 @@ bool step_fusable(const filt_params& p, int nx, int rank, int size)
 int launch_step_zm(const filt_fields& f, const filt_params& p, int nx, bool with_u)
 {
     const TileConfig t = step_tile_config(nx, p.Ny, p.Nz);
-    const TileMap m = make_map(nx, p.Ny, p.Nz, t);
+    StepArgs a;
+    a.f = f;
+    a.p = p;
+    a.nx = nx;
+    a.kz = t.kz;
+    a.map = make_map(nx, p.Ny, p.Nz, t);
+    const unsigned blocks = (unsigned)a.map.count();
     if (with_u) {
-        hipLaunchKernelGGL(pick_step<true>(t.tx, t.ty), dim3((unsigned)m.count()),
-                           dim3(t.tx, t.ty), 0, stream(), f, p, nx, m, t.kz);
+        hipLaunchKernelGGL(pick_step<true>(t.tx, t.ty), dim3(blocks), dim3(t.tx, t.ty), 0,
+                           stream(), a);
     } else {
-        hipLaunchKernelGGL(pick_step<false>(t.tx, t.ty), dim3((unsigned)m.count()),
-                           dim3(t.tx, t.ty), 0, stream(), f, p, nx, m, t.kz);
+        hipLaunchKernelGGL(pick_step<false>(t.tx, t.ty), dim3(blocks), dim3(t.tx, t.ty), 0,
+                           stream(), a);
     }
     FILT_LAUNCH_CHECK("k_step_zm");
     return FILT_OK;
