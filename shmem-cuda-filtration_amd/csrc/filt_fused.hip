@@ -62,6 +62,16 @@ __device__ __forceinline__ int media_at(const FusedGrid& G, long long c)
     return (G.media && G.p.n_media > 1) ? G.media[c] : 0;
 }
 
+// The media table in LDS, one row of FILT_MAX_MEDIA per quantity: the tile
+// loop looks entries up by each cell's medium, and a per-lane load from the
+// descriptor in global memory would make its wait also wait for the loads
+// in flight for the next plane.
+enum { M_LAM = 0, M_SWR = 1, M_PD = 2, M_M = 3, M_K = 4, M_INV1 = 5, M_ROWS = 6 };
+__device__ __forceinline__ double mtab(const double* med, int row, int md)
+{
+    return med[row * FILT_MAX_MEDIA + md];
+}
+
 // ------------------------------------------------------------ FP64 form --
 // Stage-1 values of one cell as the flux needs them (P2 = P1 + 0 = P1).
 struct C1 {
@@ -75,21 +85,22 @@ struct RawD {
     int md;
 };
 
-__device__ inline RawD fp64_raw(const FusedGrid& G, const double* Pin, const double* Sin,
-                                long long c, bool centre)
+__device__ inline RawD fp64_raw(const FusedGrid& G, const double* med, const double* Pin,
+                                const double* Sin, long long c, bool centre)
 {
     RawD r;
     r.md = media_at(G, c);
     r.P = Pin[c];
     r.S = Sin[c];
-    r.K = G.perm ? G.perm[c] : G.p.K[r.md];
-    r.phi = centre ? (G.poro ? G.poro[c] : G.p.m[r.md]) : 0.0;
+    r.K = G.perm ? G.perm[c] : mtab(med, M_K, r.md);
+    r.phi = centre ? (G.poro ? G.poro[c] : mtab(med, M_M, r.md)) : 0.0;
     return r;
 }
 
-__device__ inline C1 fp64_eval(const FusedGrid& G, const RawD& q)
+__device__ inline C1 fp64_eval(const FusedGrid& G, const double* med, const RawD& q)
 {
-    const Stage1 r = stage1_cell(q.P, q.S, q.K, G.p.lambda[q.md], G.p.S_wr[q.md], G.p);
+    const Stage1 r = stage1_cell(q.P, q.S, q.K, mtab(med, M_LAM, q.md), mtab(med, M_SWR, q.md),
+                                 G.p);
     return C1{q.P, r.Xi1, r.Xi2, r.ro1, r.ro2};
 }
 
@@ -113,9 +124,9 @@ struct Fp64Lds {
 // roS_calculation (cpu.cpp:218-291) then Newton (cpu.cpp:293-320) for an
 // interior cell, expression for expression as in filt_stages.hip; neighbours
 // in x / y from LDS slots, in z from the registers zm / zp.
-__device__ inline void fp64_update(const FusedGrid& G, const C1& cc, const C1& zm,
-                                   const C1& zp, const Fp64Lds& L, int s, const RawD& q,
-                                   double& nP, double& nS)
+__device__ inline void fp64_update(const FusedGrid& G, const double* med, const C1& cc,
+                                   const C1& zm, const C1& zp, const Fp64Lds& L, int s,
+                                   const RawD& q, double& nP, double& nS)
 {
     const filt_params& p = G.p;
     const int xm = s - 1, xp = s + 1, ym = s - SX, yp = s + SX;
@@ -138,7 +149,8 @@ __device__ inline void fp64_update(const FusedGrid& G, const C1& cc, const C1& z
     const double A2 = cc.r2 * S - (p.dt / phi) * (g1 + g2 + g3);
     nP = cc.P;
     nS = S;
-    newton_cell<false>(nP, nS, A1, A2, p.lambda[md], p.S_wr[md], p.P_d[md], p);
+    newton_cell<false>(nP, nS, A1, A2, mtab(med, M_LAM, md), mtab(med, M_SWR, md),
+                       mtab(med, M_PD, md), p);
 }
 
 // ----------------------------------------------------------- mixed form --
@@ -159,34 +171,35 @@ struct RawM {
     int md;
 };
 
-__device__ inline RawM mixed_raw(const FusedGrid& G, const double* Pin, const double* Sin,
-                                 long long c, bool centre)
+__device__ inline RawM mixed_raw(const FusedGrid& G, const double* med, const double* Pin,
+                                 const double* Sin, long long c, bool centre)
 {
     RawM r;
     r.md = media_at(G, c);
     r.P = Pin[c];
     r.S = Sin[c];
-    r.K = G.perm32 ? G.perm32[c] : (float)G.p.K[r.md];
+    r.K = G.perm32 ? G.perm32[c] : (float)mtab(med, M_K, r.md);
     r.phi = (centre && G.poro32) ? G.poro32[c] : 0.f;
     return r;
 }
 
-__device__ inline M1 mixed_eval(const FusedGrid& G, const RawM& q)
+__device__ inline M1 mixed_eval(const FusedGrid& G, const double* med, const RawM& q)
 {
     const int md = q.md;
     const float K = q.K;
     const double P = q.P;
     // 1 - S_e = S/(1-S_wr) formed directly: at small NAPL saturation the FP32
     // difference 1 - S_e would cancel to a few significant bits.
-    const float oms = (float)(q.S * G.inv_1mswr[md]);
+    const float oms = (float)(q.S * mtab(med, M_INV1, md));
     const float se = 1.f - oms;
     float k1, k2;
-    if (G.p.lambda[md] == 2.0) {  // Brooks-Corey exponents 4 and 2
+    const double lamd = mtab(med, M_LAM, md);
+    if (lamd == 2.0) {  // Brooks-Corey exponents 4 and 2
         const float s2 = se * se;
         k1 = s2 * s2;
         k2 = oms * oms * (oms * (2.f - oms));  // (1-S_e)^2 (1-S_e^2)
     } else {
-        const float lam = (float)G.p.lambda[md];
+        const float lam = (float)lamd;
         k1 = powf(se, (2.f + 3.f * lam) / lam);
         // 1 - S_e^e2 = -expm1(e2*log1p(-oms)), accurate for S_e near 1
         k2 = oms * oms * -expm1f((2.f + lam) / lam * log1pf(-oms));
@@ -218,9 +231,9 @@ __device__ __forceinline__ float uflux(float a2, float a1, float Lc, float Lm, f
     return (fmaxf(a2, 0.f) - fminf(a1, 0.f)) * Lc - fmaxf(a1, 0.f) * Lm + fminf(a2, 0.f) * Lp;
 }
 
-__device__ inline void mixed_update(const FusedGrid& G, const M1& cc, const M1& zm,
-                                    const M1& zp, const MixedLds& L, int s, const RawM& q,
-                                    double& nP, double& nS)
+__device__ inline void mixed_update(const FusedGrid& G, const double* med, const M1& cc,
+                                    const M1& zm, const M1& zp, const MixedLds& L, int s,
+                                    const RawM& q, double& nP, double& nS)
 {
     const double S = q.S;
     const filt_params& p = G.p;
@@ -244,7 +257,7 @@ __device__ inline void mixed_update(const FusedGrid& G, const M1& cc, const M1& 
                      uflux(az2, az1, cc.L2, zm.L2, zp.L2) * fz;
     const int md = q.md;
     // dt/phi in FP64: an FP32 factor would bias every increment alike
-    const double phi = G.poro32 ? (double)q.phi : p.m[md];
+    const double phi = G.poro32 ? (double)q.phi : mtab(med, M_M, md);
     double rphi = __builtin_amdgcn_rcp(phi);
     rphi = fma(fma(-phi, rphi, 1.0), rphi, rphi);
     const double dtphi = p.dt * rphi;
@@ -252,8 +265,8 @@ __device__ inline void mixed_update(const FusedGrid& G, const M1& cc, const M1& 
     const double A2 = r2c * S - dtphi * (double)F2;
     // Newton of cpu.cpp:293-320 in FP64 with reciprocals
     const double ro01 = p.ro01, ro02 = p.ro02, b1 = p.beta1, b2 = p.beta2, Pa = p.P_atm;
-    const double swr = p.S_wr[md], pd = p.P_d[md], lam = p.lambda[md];
-    const double inv1 = G.inv_1mswr[md];
+    const double swr = mtab(med, M_SWR, md), pd = mtab(med, M_PD, md);
+    const double lam = mtab(med, M_LAM, md), inv1 = mtab(med, M_INV1, md);
     const bool no_cap = (pd == 0.0) && (lam >= 0.1);
     // Without capillary pressure (P_k = 0) the two equations
     //   ro01 (1 + b1 x)(1 - S) = A1,   ro02 (1 + b2 x) S = A2,   x = P - P_atm
@@ -404,7 +417,8 @@ __device__ inline void shell_cell(const FusedGrid& G, const double* Pin, const d
 // orders both the writes before the reads and the reads of plane k before
 // the overwrite at plane k+2.
 template <bool MIXED>
-__device__ void interior_tile(const FusedGrid& G, long long lb, int parity, char* lds)
+__device__ void interior_tile(const FusedGrid& G, long long lb, int parity, char* lds,
+                              const double* med)
 {
     using V = typename std::conditional<MIXED, M1, C1>::type;
     using Raw = typename std::conditional<MIXED, RawM, RawD>::type;
@@ -461,15 +475,15 @@ __device__ void interior_tile(const FusedGrid& G, long long lb, int parity, char
     };
     auto raw = [&](long long c, bool centre) -> Raw {
         if constexpr (MIXED)
-            return mixed_raw(G, Pin, Sin, c, centre);
+            return mixed_raw(G, med, Pin, Sin, c, centre);
         else
-            return fp64_raw(G, Pin, Sin, c, centre);
+            return fp64_raw(G, med, Pin, Sin, c, centre);
     };
     auto eval = [&](const Raw& q) -> V {
         if constexpr (MIXED)
-            return mixed_eval(G, q);
+            return mixed_eval(G, med, q);
         else
-            return fp64_eval(G, q);
+            return fp64_eval(G, med, q);
     };
 
     V zm{}, zc{}, zp{};
@@ -498,7 +512,7 @@ __device__ void interior_tile(const FusedGrid& G, long long lb, int parity, char
             if (in_grid && inner) {
                 const long long c = col + k * sz;
                 double nP, nS;
-                fp64_update(G, get(L, s), get(Lm, s), zp, L, s, raw(c, true), nP, nS);
+                fp64_update(G, med, get(L, s), get(Lm, s), zp, L, s, raw(c, true), nP, nS);
                 write_cell(G, Po, So, Pin, i, j, k, nP, nS);
             }
             __syncthreads();
@@ -533,7 +547,7 @@ __device__ void interior_tile(const FusedGrid& G, long long lb, int parity, char
         if (in_grid && inner) {
             double nP, nS;
             if constexpr (MIXED)
-                mixed_update(G, zc, zm, zp, L, s, qc, nP, nS);
+                mixed_update(G, med, zc, zm, zp, L, s, qc, nP, nS);
             write_cell(G, Po, So, Pin, i, j, k, nP, nS);
         }
         zm = zc;
@@ -550,13 +564,22 @@ __global__ __launch_bounds__(FT) void k_fused(const FusedGrid* __restrict__ grid
                                               long long total_blocks, int parity, int order)
 {
     __shared__ __attribute__((aligned(16))) char lds[MIXED ? 2 * SN * 16 : LDS_BYTES];
+    __shared__ double med[M_ROWS * FILT_MAX_MEDIA];
     long long b = blockIdx.x;
     if (order == 0 && total_blocks % 8 == 0)
         b = (b % 8) * (total_blocks / 8) + b / 8;
     const FusedGrid& G = grids[block_grid[b]];
     const long long lb = b - G.first_block;
-    if (lb < G.n_interior_blocks) {
-        interior_tile<MIXED>(G, lb, parity, lds);
+    if (lb < G.n_interior_blocks) {  // uniform over the block
+        const int t = threadIdx.y * FTX + threadIdx.x;
+        if (t < M_ROWS * FILT_MAX_MEDIA) {
+            const int row = t / FILT_MAX_MEDIA, q = t % FILT_MAX_MEDIA;
+            const filt_params& p = G.p;
+            med[t] = row == M_LAM ? p.lambda[q] : row == M_SWR ? p.S_wr[q] : row == M_PD ? p.P_d[q]
+                   : row == M_M ? p.m[q] : row == M_K ? p.K[q] : G.inv_1mswr[q];
+        }
+        __syncthreads();
+        interior_tile<MIXED>(G, lb, parity, lds, med);
         return;
     }
     const long long q = (lb - G.n_interior_blocks) * FT + threadIdx.y * FTX + threadIdx.x;

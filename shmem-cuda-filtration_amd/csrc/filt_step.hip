@@ -25,7 +25,10 @@
 // are evaluated from P2 as the plane is staged); its k-1 is read back from this
 // thread's own slot of the other LDS tile, which no other thread writes.
 // Array pointers and parameters are read from the kernel argument segment
-// where they are used (filt_kernarg.hpp).  The 15 divisions by h1, h2, h3
+// where they are used (filt_kernarg.hpp); the media table (lambda, S_wr,
+// P_d, m) is copied to LDS, so its per-cell lookups are LDS reads rather
+// than per-lane loads from the argument segment, whose waits would also wait
+// for the loads in flight for the next plane.  The 15 divisions by h1, h2, h3
 // (or their doubles) per cell go through DivBy (filt_device.hpp): three FP64
 // operations each, bit for bit the IEEE division.  The kernel fits 97 VGPRs
 // and runs 4 waves per SIMD.  u is computed on every plane of the grid; roS
@@ -127,6 +130,16 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(4))) voi
     // reciprocals of h1, h2, h3, 2h1, 2h2, 2h3 for DivBy; read after the
     // first barrier of the loop
     __shared__ double rh[6];
+    // the media table (lambda, S_wr, P_d, m) for per-cell lookups: from LDS,
+    // not from per-lane loads of the argument segment
+    __shared__ double med[4][FILT_MAX_MEDIA];
+    {
+        const int t = tx + TX * ty;
+        if (t < 4 * FILT_MAX_MEDIA) {
+            const int q = t % FILT_MAX_MEDIA, w = t / FILT_MAX_MEDIA;
+            med[w][q] = w == 0 ? p.lambda[q] : w == 1 ? p.S_wr[q] : w == 2 ? p.P_d[q] : p.m[q];
+        }
+    }
     if (tx == 0 && ty == 0) {
         rh[0] = refined_reciprocal(p.h1);
         rh[1] = refined_reciprocal(p.h2);
@@ -264,7 +277,7 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(4))) voi
                                          sh[L::at(AX2, ym, cx)], sh[L::at(AR2, ym, cx)],
                                          sh[L::at(AX2, yp, cx)], sh[L::at(AR2, yp, cx)], dy);
             const int md = mdc;
-            const double phi = F->poro ? phic : Q->m[md];
+            const double phi = F->poro ? phic : med[3][md];
             const double A1 = roS1 - (Q->dt / phi) * (f1 + f2 + f3);
             const double A2 = roS2 - (Q->dt / phi) * (g1 + g2 + g3);
             stn(F->roS1_old, c, roS1);
@@ -273,7 +286,7 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(4))) voi
             stn(F->roS2, c, A2);
             // P1_S2_calculation, cpu.cpp:293-320, on this cell's registers
             double P = Pc, Sv = S;
-            newton_cell(P, Sv, A1, A2, Q->lambda[md], Q->S_wr[md], Q->P_d[md], *Q);
+            newton_cell(P, Sv, A1, A2, med[0][md], med[1][md], med[2][md], *Q);
             stn(F->P1, c, P);
             stn(F->S2, c, Sv);
         }
