@@ -57,9 +57,25 @@ constexpr int SN = SX * (FTY + 2);
 
 __device__ __forceinline__ int slot(int x, int y) { return (y + 1) * SX + (x + 1); }
 
+// The descriptor's device pointers are generic.  Accessed through gmem(),
+// loads and stores are global_* instructions, counted by vmcnt alone; as
+// flat_* instructions they would also count in lgkmcnt, so every wait for an
+// LDS access in the tile loop would also wait for the HBM loads in flight.
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* gmem(const T* p)
+{
+    return (const __attribute__((address_space(1))) T*)p;
+}
+
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gmem(T* p)
+{
+    return (__attribute__((address_space(1))) T*)p;
+}
+
 __device__ __forceinline__ int media_at(const FusedGrid& G, long long c)
 {
-    return (G.media && G.p.n_media > 1) ? G.media[c] : 0;
+    return (G.media && G.p.n_media > 1) ? gmem(G.media)[c] : 0;
 }
 
 // The media table in LDS, one row of FILT_MAX_MEDIA per quantity: the tile
@@ -90,10 +106,10 @@ __device__ inline RawD fp64_raw(const FusedGrid& G, const double* med, const dou
 {
     RawD r;
     r.md = media_at(G, c);
-    r.P = Pin[c];
-    r.S = Sin[c];
-    r.K = G.perm ? G.perm[c] : mtab(med, M_K, r.md);
-    r.phi = centre ? (G.poro ? G.poro[c] : mtab(med, M_M, r.md)) : 0.0;
+    r.P = gmem(Pin)[c];
+    r.S = gmem(Sin)[c];
+    r.K = G.perm ? gmem(G.perm)[c] : mtab(med, M_K, r.md);
+    r.phi = centre ? (G.poro ? gmem(G.poro)[c] : mtab(med, M_M, r.md)) : 0.0;
     return r;
 }
 
@@ -176,10 +192,10 @@ __device__ inline RawM mixed_raw(const FusedGrid& G, const double* med, const do
 {
     RawM r;
     r.md = media_at(G, c);
-    r.P = Pin[c];
-    r.S = Sin[c];
-    r.K = G.perm32 ? G.perm32[c] : (float)mtab(med, M_K, r.md);
-    r.phi = (centre && G.poro32) ? G.poro32[c] : 0.f;
+    r.P = gmem(Pin)[c];
+    r.S = gmem(Sin)[c];
+    r.K = G.perm32 ? gmem(G.perm32)[c] : (float)mtab(med, M_K, r.md);
+    r.phi = (centre && G.poro32) ? gmem(G.poro32)[c] : 0.f;
     return r;
 }
 
@@ -325,9 +341,11 @@ __device__ inline void mixed_update(const FusedGrid& G, const double* med, const
 // -------------------------------------------------------------- writing --
 
 // Output of interior cell (i,j,k) and the shell cells that copy it.
-__device__ inline void write_cell(const FusedGrid& G, double* Po, double* So, const double* Pin,
-                                  int i, int j, int k, double nP, double nS)
+__device__ inline void write_cell(const FusedGrid& G, double* Po_, double* So_,
+                                  const double* Pin_, int i, int j, int k, double nP, double nS)
 {
+    const auto Po = gmem(Po_), So = gmem(So_);
+    const auto Pin = gmem(Pin_);
     const int nx = G.nx, ny = G.ny, nz = G.nz;
     const long long sy = nx, sz = (long long)nx * ny;
     const long long c = i + j * sy + k * sz;
@@ -377,9 +395,11 @@ __device__ inline void write_cell(const FusedGrid& G, double* Po, double* So, co
 }
 
 // Shell cells that do not depend on this step's interior values.
-__device__ inline void shell_cell(const FusedGrid& G, const double* Pin, const double* Sin,
-                                  double* Po, double* So, long long q)
+__device__ inline void shell_cell(const FusedGrid& G, const double* Pin_, const double* Sin_,
+                                  double* Po_, double* So_, long long q)
 {
+    const auto Pin = gmem(Pin_), Sin = gmem(Sin_);
+    const auto Po = gmem(Po_), So = gmem(So_);
     const int nx = G.nx, ny = G.ny, nz = G.nz;
     const long long sy = nx, sz = (long long)nx * ny;
     const long long ringA = 2LL * ny + 2LL * (nz - 2);
