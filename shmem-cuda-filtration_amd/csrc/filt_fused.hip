@@ -194,7 +194,10 @@ __device__ inline RawM mixed_raw(const FusedGrid& G, const double* med, const do
     r.md = media_at(G, c);
     r.P = gmem(Pin)[c];
     r.S = gmem(Sin)[c];
-    r.K = G.perm32 ? gmem(G.perm32)[c] : (float)mtab(med, M_K, r.md);
+    // K by medium is looked up when the cell is evaluated, not here: a table
+    // read indexed by the medium just loaded would wait for that load, and
+    // with it for every load in flight
+    r.K = G.perm32 ? gmem(G.perm32)[c] : 0.f;
     r.phi = (centre && G.poro32) ? gmem(G.poro32)[c] : 0.f;
     return r;
 }
@@ -202,7 +205,7 @@ __device__ inline RawM mixed_raw(const FusedGrid& G, const double* med, const do
 __device__ inline M1 mixed_eval(const FusedGrid& G, const double* med, const RawM& q)
 {
     const int md = q.md;
-    const float K = q.K;
+    const float K = G.perm32 ? q.K : (float)mtab(med, M_K, md);
     const double P = q.P;
     // 1 - S_e = S/(1-S_wr) formed directly: at small NAPL saturation the FP32
     // difference 1 - S_e would cancel to a few significant bits.
