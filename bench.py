@@ -231,7 +231,7 @@ def run_plain(args, p, perm, poro, split=False, steps=None):
     return p, wall, stage_ms, r, finite, tuned
 
 
-FUSED_BYTES = {0: 48, 1: 40}  # P, S in + out (32) + perm, poro (FP64 16 / FP32 8)
+FUSED_BYTES = {0: 48, 1: 48}  # P, S in + out (32) + perm, poro in FP64 (16)
 
 
 def time_ensemble(e, steps, hiprt, lib):

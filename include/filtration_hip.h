@@ -301,9 +301,11 @@ int filt_autotune_stencil(const filt_fields* dev, const filt_params* p, int loca
  * each with its own parameters, optional per-cell permeability / porosity /
  * media and its state, all resident in HBM.  Results of a member do not
  * depend on the other members.  FILT_PREC_FP64 computes what the stage path
- * computes (same expressions); FILT_PREC_MIXED evaluates the stencil fluxes
- * and the ro*S increment in FP32 (FP32 property fields), keeping pressure
- * differences, the ro*S base, the Newton solve and mass sums in FP64.
+ * computes (same expressions); FILT_PREC_MIXED evaluates the water phase's
+ * stencil fluxes in FP32 and keeps the NAPL phase's fluxes, pressure
+ * differences, the ro*S update, the Newton solve, the property fields and
+ * the mass sums in FP64 (water, NAPL and total mass within 1e-9 relative of
+ * the FP64 form).
  * Use an ensemble from one host thread at a time.
  */
 #define FILT_PREC_FP64 0

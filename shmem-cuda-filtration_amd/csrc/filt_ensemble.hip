@@ -185,9 +185,9 @@ size_t filt_ensemble_member_bytes(const filt_params* p, int precision, int with_
     if (!p)
         return 0;
     const size_t n = (size_t)p->NX * p->Ny * p->Nz;
-    const size_t stat = precision == FILT_PREC_MIXED ? sizeof(float) : sizeof(double);
-    return 4 * round256(n * 8) + (with_perm ? round256(n * stat) : 0) +
-           (with_poro ? round256(n * stat) : 0) + (with_media ? round256(n * 4) : 0);
+    (void)precision;  // both forms keep K and porosity in FP64
+    return 4 * round256(n * 8) + (with_perm ? round256(n * 8) : 0) +
+           (with_poro ? round256(n * 8) : 0) + (with_media ? round256(n * 4) : 0);
 }
 
 int filt_ensemble_add(filt_ensemble* e, const filt_params* p, const double* P1, const double* S2,
@@ -207,7 +207,6 @@ int filt_ensemble_add(filt_ensemble* e, const filt_params* p, const double* P1, 
         if (p->n_media < 0 || p->n_media > FILT_MAX_MEDIA || p->newton_iterations < 0)
             return set_error(FILT_ERR_INVALID, "bad n_media / newton_iterations");
         const size_t n = (size_t)p->NX * p->Ny * p->Nz;
-        const bool mixed = e->precision == FILT_PREC_MIXED;
         const size_t bytes = filt_ensemble_member_bytes(p, e->precision, perm != nullptr,
                                                         poro != nullptr, media != nullptr);
         char* base = nullptr;
@@ -236,25 +235,14 @@ int filt_ensemble_add(filt_ensemble* e, const filt_params* p, const double* P1, 
         };
         put(g.P[e->parity], P1, n * 8);
         put(g.S[e->parity], S2, n * 8);
-        std::vector<float> tmp;
         for (int which = 0; which < 2; which++) {
             const double* src = which == 0 ? perm : poro;
             if (!src)
                 continue;
-            if (mixed) {
-                tmp.resize(n);
-                for (size_t c = 0; c < n; c++)
-                    tmp[c] = (float)src[c];
-                float* d = (float*)(base + off);
-                put(d, tmp.data(), n * 4);
-                (which == 0 ? g.perm32 : g.poro32) = d;
-                off += round256(n * 4);
-            } else {
-                double* d = (double*)(base + off);
-                put(d, src, n * 8);
-                (which == 0 ? g.perm : g.poro) = d;
-                off += round256(n * 8);
-            }
+            double* d = (double*)(base + off);
+            put(d, src, n * 8);
+            (which == 0 ? g.perm : g.poro) = d;
+            off += round256(n * 8);
         }
         if (media) {
             int* d = (int*)(base + off);

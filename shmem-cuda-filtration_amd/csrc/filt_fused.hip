@@ -170,20 +170,23 @@ __device__ inline void fp64_update(const FusedGrid& G, const double* med, const 
 }
 
 // ----------------------------------------------------------- mixed form --
-// Stencil fluxes in FP32 on FP32 mobility-density products L = -Xi*ro >= 0;
-// pressure differences (the y face force -dP/h + ro*g cancels toward
-// hydrostatic balance), the ro(P)*S base, dt/phi times the flux, the Newton
-// and the mass sums in FP64.  Divisions by the uniform spacings, the
-// viscosities and 1 - S_wr are multiplications by reciprocals; 1/det is an
-// FP64 reciprocal refined by one Newton-Raphson step.  Statics are FP32.
+// The water phase's stencil fluxes in FP32 on FP32 mobility-density products
+// L1 = -Xi1*ro1 >= 0.  The NAPL phase's in FP64: its mass is small and dilute,
+// and FP32 flux differences showed in it at 1e-9 to 1e-8 relative.  Pressure
+// differences (the y face force -dP/h + ro*g cancels toward hydrostatic
+// balance), the ro(P)*S base, dt/phi times the flux, the Newton and the mass
+// sums in FP64.  Divisions by the uniform spacings, the viscosities and
+// 1 - S_wr are multiplications by reciprocals; 1/det is an FP64 reciprocal
+// refined by one Newton-Raphson step.  K and porosity stay FP64 (rounded to
+// FP32 they moved the NAPL mass by 2e-9 relative).
 struct M1 {
     double P;
-    float L1, L2;
+    double L2;
+    float L1;
 };
 
 struct RawM {
-    double P, S;
-    float K, phi;
+    double P, S, K, phi;
     int md;
 };
 
@@ -197,45 +200,47 @@ __device__ inline RawM mixed_raw(const FusedGrid& G, const double* med, const do
     // K by medium is looked up when the cell is evaluated, not here: a table
     // read indexed by the medium just loaded would wait for that load, and
     // with it for every load in flight
-    r.K = G.perm32 ? gmem(G.perm32)[c] : 0.f;
-    r.phi = (centre && G.poro32) ? gmem(G.poro32)[c] : 0.f;
+    r.K = G.perm ? gmem(G.perm)[c] : 0.0;
+    r.phi = (centre && G.poro) ? gmem(G.poro)[c] : 0.0;
     return r;
 }
 
 __device__ inline M1 mixed_eval(const FusedGrid& G, const double* med, const RawM& q)
 {
     const int md = q.md;
-    const float K = G.perm32 ? q.K : (float)mtab(med, M_K, md);
+    const double K = G.perm ? q.K : mtab(med, M_K, md);
     const double P = q.P;
     // 1 - S_e = S/(1-S_wr) formed directly: at small NAPL saturation the FP32
     // difference 1 - S_e would cancel to a few significant bits.
     const float oms = (float)(q.S * mtab(med, M_INV1, md));
     const float se = 1.f - oms;
-    float k1, k2;
+    float k1;
+    double k2d;
     const double lamd = mtab(med, M_LAM, md);
+    const double omsd = q.S * mtab(med, M_INV1, md);
     if (lamd == 2.0) {  // Brooks-Corey exponents 4 and 2
         const float s2 = se * se;
         k1 = s2 * s2;
-        k2 = oms * oms * (oms * (2.f - oms));  // (1-S_e)^2 (1-S_e^2)
+        k2d = omsd * omsd * (omsd * (2. - omsd));  // (1-S_e)^2 (1-S_e^2)
     } else {
         const float lam = (float)lamd;
         k1 = powf(se, (2.f + 3.f * lam) / lam);
         // 1 - S_e^e2 = -expm1(e2*log1p(-oms)), accurate for S_e near 1
-        k2 = oms * oms * -expm1f((2.f + lam) / lam * log1pf(-oms));
+        k2d = (double)(oms * oms * -expm1f((2.f + lam) / lam * log1pf(-oms)));
     }
     const float r1 = (float)(G.p.ro01 * (1. + G.p.beta1 * (P - G.p.P_atm)));
-    const float r2 = (float)(G.p.ro02 * (1. + G.p.beta2 * (P - G.p.P_atm)));
+    const double r2 = G.p.ro02 * (1. + G.p.beta2 * (P - G.p.P_atm));
     M1 v;
     v.P = P;
-    v.L1 = K * k1 * (float)G.inv_mu1 * r1;
-    v.L2 = K * k2 * (float)G.inv_mu2 * r2;
+    v.L1 = (float)K * k1 * (float)G.inv_mu1 * r1;
+    v.L2 = K * k2d * G.inv_mu2 * r2;
     return v;
 }
 
 struct MixedLds {
     double* P;
     float* L1;
-    float* L2;
+    double* L2;
     __device__ void put(int s, const M1& v) const
     {
         P[s] = v.P;
@@ -250,6 +255,11 @@ __device__ __forceinline__ float uflux(float a2, float a1, float Lc, float Lm, f
     return (fmaxf(a2, 0.f) - fminf(a1, 0.f)) * Lc - fmaxf(a1, 0.f) * Lm + fminf(a2, 0.f) * Lp;
 }
 
+__device__ __forceinline__ double uflux64(double a2, double a1, double Lc, double Lm, double Lp)
+{
+    return (fmax(a2, 0.) - fmin(a1, 0.)) * Lc - fmax(a1, 0.) * Lm + fmin(a2, 0.) * Lp;
+}
+
 __device__ inline void mixed_update(const FusedGrid& G, const double* med, const M1& cc,
                                     const M1& zm, const M1& zp, const MixedLds& L, int s,
                                     const RawM& q, double& nP, double& nS)
@@ -261,8 +271,10 @@ __device__ inline void mixed_update(const FusedGrid& G, const double* med, const
     const double r1c = p.ro01 * (1. + p.beta1 * (p1 - p.P_atm));
     const double r2c = p.ro02 * (1. + p.beta2 * (p1 - p.P_atm));
     const double ihx = G.inv_h[0], ihy = G.inv_h[1], ihz = G.inv_h[2];
-    const float ax2 = (float)((p1 - L.P[xp]) * ihx), ax1 = (float)((L.P[xm] - p1) * ihx);
-    const float az2 = (float)((p1 - zp.P) * ihz), az1 = (float)((zm.P - p1) * ihz);
+    const double ax2d = (p1 - L.P[xp]) * ihx, ax1d = (L.P[xm] - p1) * ihx;
+    const double az2d = (p1 - zp.P) * ihz, az1d = (zm.P - p1) * ihz;
+    const float ax2 = (float)ax2d, ax1 = (float)ax1d;
+    const float az2 = (float)az2d, az1 = (float)az1d;
     const double dy2 = (p1 - L.P[yp]) * ihy, dy1 = (L.P[ym] - p1) * ihy;
     const double g = p.g_const;
     const float fx = (float)ihx, fy = (float)ihy, fz = (float)ihz;
@@ -270,18 +282,17 @@ __device__ inline void mixed_update(const FusedGrid& G, const double* med, const
                      uflux((float)(dy2 + r1c * g), (float)(dy1 + r1c * g), cc.L1, L.L1[ym],
                            L.L1[yp]) * fy +
                      uflux(az2, az1, cc.L1, zm.L1, zp.L1) * fz;
-    const float F2 = uflux(ax2, ax1, cc.L2, L.L2[xm], L.L2[xp]) * fx +
-                     uflux((float)(dy2 + r2c * g), (float)(dy1 + r2c * g), cc.L2, L.L2[ym],
-                           L.L2[yp]) * fy +
-                     uflux(az2, az1, cc.L2, zm.L2, zp.L2) * fz;
+    const double F2 = uflux64(ax2d, ax1d, cc.L2, L.L2[xm], L.L2[xp]) * ihx +
+                      uflux64(dy2 + r2c * g, dy1 + r2c * g, cc.L2, L.L2[ym], L.L2[yp]) * ihy +
+                      uflux64(az2d, az1d, cc.L2, zm.L2, zp.L2) * ihz;
     const int md = q.md;
     // dt/phi in FP64: an FP32 factor would bias every increment alike
-    const double phi = G.poro32 ? (double)q.phi : mtab(med, M_M, md);
+    const double phi = G.poro ? q.phi : mtab(med, M_M, md);
     double rphi = __builtin_amdgcn_rcp(phi);
     rphi = fma(fma(-phi, rphi, 1.0), rphi, rphi);
     const double dtphi = p.dt * rphi;
     const double A1 = r1c * (1 - S) - dtphi * (double)F1;
-    const double A2 = r2c * S - dtphi * (double)F2;
+    const double A2 = r2c * S - dtphi * F2;
     // Newton of cpu.cpp:293-320 in FP64 with reciprocals
     const double ro01 = p.ro01, ro02 = p.ro02, b1 = p.beta1, b2 = p.beta2, Pa = p.P_atm;
     const double swr = mtab(med, M_SWR, md), pd = mtab(med, M_PD, md);
@@ -446,7 +457,7 @@ __device__ void interior_tile(const FusedGrid& G, long long lb, int parity, char
     using V = typename std::conditional<MIXED, M1, C1>::type;
     using Raw = typename std::conditional<MIXED, RawM, RawD>::type;
     using Lds = typename std::conditional<MIXED, MixedLds, Fp64Lds>::type;
-    constexpr int BUF = MIXED ? SN * 16 : 5 * SN * 8;  // bytes of one LDS tile
+    constexpr int BUF = MIXED ? SN * 20 : 5 * SN * 8;  // bytes of one LDS tile
     const double* Pin = G.P[parity];
     const double* Sin = G.S[parity];
     double* Po = G.P[parity ^ 1];
@@ -487,8 +498,8 @@ __device__ void interior_tile(const FusedGrid& G, long long lb, int parity, char
         Lds L;
         if constexpr (MIXED) {
             L.P = (double*)base;
-            L.L1 = (float*)(base + SN * sizeof(double));
-            L.L2 = L.L1 + SN;
+            L.L2 = (double*)base + SN;
+            L.L1 = (float*)(base + 2 * SN * sizeof(double));
         } else {
             double* d = (double*)base;
             L.P = d;
@@ -589,7 +600,7 @@ __global__ __launch_bounds__(FT) void k_fused(const FusedGrid* __restrict__ grid
                                               const int* __restrict__ block_grid,
                                               long long total_blocks, int parity, int order)
 {
-    __shared__ __attribute__((aligned(16))) char lds[MIXED ? 2 * SN * 16 : LDS_BYTES];
+    __shared__ __attribute__((aligned(16))) char lds[MIXED ? 2 * SN * 20 : LDS_BYTES];
     __shared__ double med[M_ROWS * FILT_MAX_MEDIA];
     long long b = blockIdx.x;
     if (order == 0 && total_blocks % 8 == 0)
@@ -637,7 +648,7 @@ __global__ __launch_bounds__(256) void k_mass_partial(const FusedGrid* __restric
         if (!is_interior(i, j, k, G.nx, G.ny, G.nz))
             continue;
         const int md = media_at(G, c);
-        const double phi = G.poro ? G.poro[c] : (G.poro32 ? (double)G.poro32[c] : G.p.m[md]);
+        const double phi = G.poro ? G.poro[c] : G.p.m[md];
         const double P = G.P[parity][c], S = G.S[parity][c];
         const double r1 = G.p.ro01 * (1. + G.p.beta1 * (P - G.p.P_atm));
         const double r2 = G.p.ro02 * (1. + G.p.beta2 * (P - G.p.P_atm));

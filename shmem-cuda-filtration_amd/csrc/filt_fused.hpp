@@ -10,10 +10,8 @@ namespace filt {
 struct FusedGrid {
     double* P[2];         // ping-pong water pressure
     double* S[2];         // ping-pong NAPL saturation
-    const double* perm;   // per-cell K, FP64 form (null: K[media])
-    const double* poro;   // per-cell porosity, FP64 form (null: m[media])
-    const float* perm32;  // per-cell K, mixed form
-    const float* poro32;  // per-cell porosity, mixed form
+    const double* perm;   // per-cell K (null: K[media])
+    const double* poro;   // per-cell porosity (null: m[media])
     const int* media;     // null: medium 0
     int nx, ny, nz;
     int nbx, nby, nbz, kz;       // interior tiles
