@@ -286,8 +286,10 @@ int filt_get_tile_config(int kernel, int nx, int ny, int nz, int* tx, int* ty, i
  * kernels on the given device grid (after stage 1 has filled P2, Xi, ro),
  * reps launches each (median), and keeps the fastest per kernel for grids of
  * this shape.  Rewrites only roS*, u* with the values one call would give.
- * best_ms[0] / best_ms[1] receive the roS / u times when non-null.
- * filt_set_tile_config clears tuned entries. */
+ * best_ms[0] / best_ms[1] receive the roS / u times when non-null.  The
+ * tiled roS kernel needs h1, h2, h3 in [2^-100, 2^100] (it divides through a
+ * refined reciprocal); outside that range roS runs untiled, is not tuned,
+ * and best_ms[0] is 0.  filt_set_tile_config clears tuned entries. */
 int filt_autotune_stencil(const filt_fields* dev, const filt_params* p, int localNx, int reps,
                           double* best_ms);
 

@@ -497,8 +497,10 @@ dim3 cell_grid(const filt_params* p, int nx)
 // smaller grids take the one-cell-per-thread kernels above.
 bool use_tiled(const filt_params* p, int nx, int kernel)
 {
+    // the tiled roS kernel divides by h1, h2, h3 through DivBy
+    const bool div_ok = kernel != 0 || (divby_ok(p->h1) && divby_ok(p->h2) && divby_ok(p->h3));
     return nx >= 3 && p->Ny >= 3 && p->Nz >= 3 && tiled_fits(nx, p->Ny, p->Nz) &&
-           tiled_enabled(kernel);
+           tiled_enabled(kernel) && div_ok;
 }
 
 }  // namespace

@@ -476,9 +476,12 @@ extern "C" int filt_autotune_stencil(const filt_fields* dev, const filt_params* 
         hipEvent_t e0, e1;
         FILT_HIP_TRY(hipEventCreate(&e0));
         FILT_HIP_TRY(hipEventCreate(&e1));
-        TileConfig best[2];
+        TileConfig best[2] = {cfg_for(0, nx, ny, nz), cfg_for(1, nx, ny, nz)};
         double best_t[2] = {1e30, 1e30};
-        for (int kern = 0; kern < 2; kern++)
+        // the tiled roS kernel divides through DivBy: outside its divisor
+        // range the stage runs untiled and only u is tuned
+        const bool roS_ok = divby_ok(p->h1) && divby_ok(p->h2) && divby_ok(p->h3);
+        for (int kern = roS_ok ? 0 : 1; kern < 2; kern++)
             for (auto& s : shapes)
                 for (int kz : kzs)
                     for (int order = 0; order < 2; order++) {
@@ -516,7 +519,7 @@ extern "C" int filt_autotune_stencil(const filt_fields* dev, const filt_params* 
             g_tuned[std::make_tuple(nx, ny, nz)] = std::make_pair(best[0], best[1]);
         }
         if (best_ms) {
-            best_ms[0] = best_t[0];
+            best_ms[0] = roS_ok ? best_t[0] : 0.0;
             best_ms[1] = best_t[1];
         }
         return FILT_OK;

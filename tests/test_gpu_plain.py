@@ -306,6 +306,19 @@ def extreme_state(p, rng):
     return g
 
 
+def test_spacing_outside_divby_range_runs_untiled(hip, orc):
+    """With h1 below 2^-100 the tiled roS kernel and the fused step (whose
+    divisions by h go through a refined reciprocal) step aside: the stage
+    path and the time step still give the oracle's bits."""
+    p = ref_params(21, 13, 11, source=1)
+    p.h1 = 2.0 ** -105
+    g = random_state(p, np.random.default_rng(3))
+    out, want = run_stage_pair(hip, orc, p, g, 3)
+    for n in FIELD_NAMES_ALL:
+        assert np.array_equal(out[n], want[n], equal_nan=True), n
+    fused_vs_staged(hip, p, g, 1, 1)
+
+
 @pytest.mark.parametrize("seed", [0, 1])
 def test_roS_tiled_extreme_values_equal_oracle(hip, orc, seed):
     """The tiled roS kernel divides by h through DivBy; on states with zero,
