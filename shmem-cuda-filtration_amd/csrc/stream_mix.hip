@@ -1,0 +1,153 @@
+// stream_mix.hip -- HBM bandwidth of pure streaming kernels with the same
+// array mix as each plain-path kernel.
+//
+// A kernel that reads I FP64 arrays and writes O FP64 arrays, one cell per
+// thread, with no arithmetic beyond a sum, is the fastest any kernel with that
+// access mix can be: it is the roofline of the mix on this GPU.  Mixes with
+// many output streams move fewer bytes per second than a copy, so each
+// plain-path kernel is compared with the stream of its own mix:
+//
+//   copy          1 in,  1 out   (2^27 cells: 1 GiB each way)
+//   k_ro_P2_Xi    3 in,  5 out   S2, P1, perm -> P2, Xi1, Xi2, ro1, ro2
+//   k_u_zm        6 in,  6 out   Xi1, Xi2, ro1, ro2, P1, P2 -> six u
+//   k_roS_zm      8 in,  4 out   S2, P1, P2, Xi1, Xi2, ro1, ro2, poro -> four roS
+//   k_newton      4 in,  2 out   S2, P1, roS1, roS2 -> P1, S2
+//   k_step_zm     5 in, 12 out   P2, Xi1, Xi2, S2, poro -> six u, four roS, P1, S2
+//
+// Every mix runs with plain and with non-temporal stores.  Prints one JSON
+// line per mix and store kind.  Usage: stream_mix [cells] (default 256^3).
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+
+#define CHECK(x)                                                                       \
+    do {                                                                               \
+        hipError_t e_ = (x);                                                           \
+        if (e_ != hipSuccess) {                                                        \
+            fprintf(stderr, "%s:%d %s\n", __FILE__, __LINE__, hipGetErrorString(e_)); \
+            exit(1);                                                                   \
+        }                                                                              \
+    } while (0)
+
+namespace {
+
+constexpr int kMaxIn = 8, kMaxOut = 12;
+
+struct Streams {
+    const double* in[kMaxIn];
+    double* out[kMaxOut];
+};
+
+template <int I, int O, bool NT>
+__global__ __launch_bounds__(256) void k_mix(Streams s, size_t n)
+{
+    const size_t c = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (c >= n)
+        return;
+    double v = 0.0;
+#pragma unroll
+    for (int a = 0; a < I; a++)
+        v += s.in[a][c];
+#pragma unroll
+    for (int b = 0; b < O; b++) {
+        if (NT)
+            __builtin_nontemporal_store(v + b, s.out[b] + c);
+        else
+            s.out[b][c] = v + b;
+    }
+}
+
+__global__ void k_fill(double* a, size_t n, double base)
+{
+    const size_t c = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (c < n)
+        a[c] = base + (double)(c & 1023);
+}
+
+unsigned blocks_for(size_t n) { return (unsigned)((n + 255) / 256); }
+
+template <int I, int O, bool NT>
+void run(const char* name, const Streams& s, size_t n)
+{
+    hipEvent_t a, b;
+    CHECK(hipEventCreate(&a));
+    CHECK(hipEventCreate(&b));
+    const int reps = 20;
+    for (int w = 0; w < 3; w++)
+        hipLaunchKernelGGL((k_mix<I, O, NT>), dim3(blocks_for(n)), dim3(256), 0, 0, s, n);
+    CHECK(hipGetLastError());
+    CHECK(hipDeviceSynchronize());
+    float best = 1e30f;
+    for (int t = 0; t < 3; t++) {
+        CHECK(hipEventRecord(a));
+        for (int r = 0; r < reps; r++)
+            hipLaunchKernelGGL((k_mix<I, O, NT>), dim3(blocks_for(n)), dim3(256), 0, 0, s, n);
+        CHECK(hipEventRecord(b));
+        CHECK(hipEventSynchronize(b));
+        float ms;
+        CHECK(hipEventElapsedTime(&ms, a, b));
+        best = ms / reps < best ? ms / reps : best;
+    }
+    CHECK(hipGetLastError());
+    const double bytes = 8.0 * (I + O) * (double)n;
+    printf("{\"mix\": \"%s\", \"in\": %d, \"out\": %d, \"nontemporal_stores\": %d, "
+           "\"cells\": %zu, \"bytes_per_cell\": %d, \"ms\": %.4f, \"TBps\": %.3f}\n",
+           name, I, O, NT ? 1 : 0, n, 8 * (I + O), best, bytes / (best * 1e-3) / 1e12);
+    fflush(stdout);
+    CHECK(hipEventDestroy(a));
+    CHECK(hipEventDestroy(b));
+}
+
+template <int I, int O>
+void both(const char* name, const Streams& s, size_t n)
+{
+    run<I, O, false>(name, s, n);
+    run<I, O, true>(name, s, n);
+}
+
+}  // namespace
+
+int main(int argc, char** argv)
+{
+    const size_t n = argc > 1 ? (size_t)atoll(argv[1]) : (size_t)256 * 256 * 256;
+    if (n == 0 || n > ((size_t)1 << 31)) {
+        fprintf(stderr, "cells must be in [1, 2^31]\n");
+        return 2;
+    }
+    Streams s;
+    for (int a = 0; a < kMaxIn; a++) {
+        double* p;
+        CHECK(hipMalloc(&p, n * sizeof(double)));
+        hipLaunchKernelGGL(k_fill, dim3(blocks_for(n)), dim3(256), 0, 0, p, n, 1.0 + a);
+        s.in[a] = p;
+    }
+    for (int b = 0; b < kMaxOut; b++)
+        CHECK(hipMalloc(&s.out[b], n * sizeof(double)));
+    CHECK(hipDeviceSynchronize());
+
+    {   // the copy over 1 GiB each way, past the 256 MB last-level cache
+        const size_t nc = (size_t)1 << 27;
+        Streams c = s;
+        double *src, *dst;
+        CHECK(hipMalloc(&src, nc * sizeof(double)));
+        CHECK(hipMalloc(&dst, nc * sizeof(double)));
+        hipLaunchKernelGGL(k_fill, dim3(blocks_for(nc)), dim3(256), 0, 0, src, nc, 1.0);
+        c.in[0] = src;
+        c.out[0] = dst;
+        both<1, 1>("copy", c, nc);
+        CHECK(hipFree(src));
+        CHECK(hipFree(dst));
+    }
+    both<3, 5>("k_ro_P2_Xi", s, n);
+    both<6, 6>("k_u_zm", s, n);
+    both<8, 4>("k_roS_zm", s, n);
+    both<4, 2>("k_newton", s, n);
+    both<5, 12>("k_step_zm", s, n);
+
+    for (int a = 0; a < kMaxIn; a++)
+        CHECK(hipFree((void*)s.in[a]));
+    for (int b = 0; b < kMaxOut; b++)
+        CHECK(hipFree(s.out[b]));
+    return 0;
+}

@@ -265,6 +265,36 @@ def test_exchange_pack_unpack_match_reference(hip, ref, rank, size):
         d.free()
 
 
+def test_device_copy_ragged_sizes(hip):
+    """filt_device_copy (the streaming copy kernel with non-temporal loads and
+    stores) copies exactly n_bytes for sizes that end inside a block and
+    inside a thread's last vector, leaves the bytes past n_bytes alone, and
+    rejects sizes that are not a multiple of 16."""
+    import ctypes
+    lib = hip.lib
+    cap = 16 * 1024 * 70 + 4096
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    assert lib.filt_device_malloc(ctypes.byref(src), cap) == 0
+    assert lib.filt_device_malloc(ctypes.byref(dst), cap) == 0
+    try:
+        rng = np.random.default_rng(7)
+        data = rng.integers(0, 256, cap, dtype=np.uint8)
+        fill = np.full(cap, 0xA5, np.uint8)
+        assert lib.filt_memcpy(src, data.ctypes.data, cap, 0) == 0
+        for n in (0, 16, 4096 - 16, 16 * 1024, 16 * 1024 * 70 + 48, cap):
+            assert lib.filt_memcpy(dst, fill.ctypes.data, cap, 0) == 0
+            assert lib.filt_device_copy(dst, src, n) == 0
+            got = np.zeros(cap, np.uint8)
+            assert lib.filt_memcpy(got.ctypes.data, dst, cap, 1) == 0
+            assert np.array_equal(got[:n], data[:n]), n
+            assert np.all(got[n:] == 0xA5), n
+        assert lib.filt_device_copy(dst, src, 24) != 0
+        assert lib.filt_device_copy(None, src, 16) != 0
+    finally:
+        lib.filt_device_free_ptr(src)
+        lib.filt_device_free_ptr(dst)
+
+
 @pytest.mark.parametrize("params", ["ref", "bl"])
 @pytest.mark.parametrize("dims", [(33, 9, 7), (70, 19, 6)])
 def test_untiled_stencils_3d_bitwise(hip, orc, params, dims):
