@@ -231,6 +231,56 @@ def run_plain(args, p, perm, poro, split=False, steps=None):
     return p, wall, stage_ms, r, finite, tuned
 
 
+def run_shim(args, p, perm, poro, steps, fuse):
+    """The reference driver's backend calls through the defines.h shims
+    (lib/libfiltration_defines.so) in time_step_function's order
+    (main.cpp:86-92), each complete on return as in gpu.cu.  fuse: the shims
+    defer u and roS and run stages 2-4 as one launch (their default);
+    otherwise FILT_FUSE_STAGES=0 and five launches.  The defines.h interface
+    has no per-cell K or porosity: the same initial state runs on the first
+    medium of the table, so the kernels read the media array instead."""
+    from importlib import import_module
+    import numpy as np
+    shim_mod = import_module(PKG.__name__ + ".defines_shim")
+    shim = shim_mod.Shim()
+    PKG.device.Backend()
+    old = os.environ.get("FILT_FUSE_STAGES")
+    os.environ["FILT_FUSE_STAGES"] = "1" if fuse else "0"
+    g = PKG.workloads.initial_grid(p, perm, poro)
+    c = shim_mod.consts_from_params(p)
+    nx = p.NX
+    dev, buf = shim_mod.PtrArrays(), ctypes.c_void_p()
+    shim.device_memory_alloc(ctypes.byref(dev), ctypes.byref(buf), nx, c)
+    if not dev.P1:
+        raise SystemExit("defines_shim: device_memory_alloc failed")
+    try:
+        for n in PKG.abi.FIELD_NAMES:
+            a = np.ascontiguousarray(g.a[n])
+            shim.load_data_to_device(a.ctypes.data, getattr(dev, n), nx, c)
+        media = np.zeros((p.Nz, p.Ny, nx), np.int32)
+        shim.load_data_to_device_int(media.ctypes.data, dev.media, nx, c)
+        for s in range(1, args.warmup + 1):
+            shim.time_step(dev, c, s * p.dt, nx)
+        t0 = time.perf_counter()
+        for s in range(steps):
+            shim.time_step(dev, c, (args.warmup + s + 1) * p.dt, nx)
+        wall = time.perf_counter() - t0
+        P1 = np.empty((p.Nz, p.Ny, nx))
+        S2 = np.empty_like(P1)
+        shim.load_data_to_host(P1.ctypes.data, dev.P1, nx, c)
+        shim.load_data_to_host(S2.ctypes.data, dev.S2, nx, c)
+    finally:
+        shim.device_memory_free(dev, buf)
+        if old is None:
+            os.environ.pop("FILT_FUSE_STAGES", None)
+        else:
+            os.environ["FILT_FUSE_STAGES"] = old
+    cells = p.NX * p.Ny * p.Nz
+    return {"value": cells * steps / wall, "ms_per_step": wall * 1e3 / steps,
+            "steps": steps, "stages_2_4_fused": fuse,
+            "state_finite": bool(np.isfinite(P1).all() and np.isfinite(S2).all())}
+
+
 FUSED_BYTES = {0: 48, 1: 48}  # P, S in + out (32) + perm, poro in FP64 (16)
 
 
@@ -478,6 +528,10 @@ def main():
              "stages_ms": stage5, "state_finite": finite5,
              "bytes_per_cell_update": sum(PLAIN_BYTES[s] for s in SPLIT_STAGES),
              "hbm_frac": split_bytes * max(1, args.steps // 2) / wall5 / HBM_PEAK}
+    # the same calls through the defines.h shims, as the reference driver makes them
+    shim_steps = max(1, args.steps // 2)
+    shims = {"fused": run_shim(args, p, perm, poro, shim_steps, True),
+             "separate": run_shim(args, p, perm, poro, shim_steps, False)}
     line = {
         "metric": "cell-updates/s",
         "value": value,
@@ -522,6 +576,7 @@ def main():
                                "frac": split_bytes * args.steps / wall / HBM_PEAK},
         },
         "stage_calls": split,
+        "defines_shims": shims,
         "forms": forms,
     }
     if not args.no_cpu:

@@ -11,6 +11,21 @@
 // returns (gpu.cu synchronises after every launch, gpu.cu:7-27); errors are
 // printed and execution continues.  Set FILT_ABORT_ON_ERROR=1 to abort
 // instead.
+//
+// One difference, invisible through this interface: time_step_function
+// (main.cpp:86-91) calls u_calculation, roS_calculation and P1_S2_calculation
+// right after ro_P2_Xi_calculation on the same arrays (the exchanges between
+// them do nothing on one rank).  The shim defers u and roS and runs the three
+// stages in P1_S2_calculation as one z-marching launch
+// (filt_u_roS_P1_S2_calculation), which writes every array the three stages
+// write with the same bits and moves 136 instead of 240 bytes per cell.  The
+// fused kernel relies on stage 1 having just run (P2 then holds P1's bits),
+// so the shim only defers u when the previous call was ro_P2_Xi_calculation
+// with the same arrays and parameters.  Every other entry point first runs
+// whatever is deferred, so data read back through load_data_to_host or the
+// exchange functions is what the separate calls would have left.  Device
+// memory read by other means between u_calculation and P1_S2_calculation
+// would not hold u yet.  FILT_FUSE_STAGES=0 turns the deferral off.
 #ifdef FILT_DEFINES_HEADER
 #include FILT_DEFINES_HEADER
 #else
@@ -19,6 +34,7 @@
 
 #include <stdio.h>
 #include <stdlib.h>
+#include <string.h>
 
 // The header's compile-time constants (defines.h:8-15) are macros whose names
 // collide with filt_params members: take their values, then drop the macros.
@@ -133,10 +149,54 @@ void finish(int rc, const char* where)
         report(filt_synchronize(), where);
 }
 
+// The stages 2-4 chain of time_step_function: what has run or been deferred
+// since the last stage-1 call, and that call's arguments.
+struct Chain {
+    enum { kNone, kStage1, kU, kURoS } state = kNone;
+    ptr_Arrays dev;
+    consts def;
+    int localNx, rank, size;
+    double t;
+};
+Chain g_chain;
+
+bool fuse_enabled()
+{
+    const char* e = getenv("FILT_FUSE_STAGES");
+    return !(e && e[0] == '0');
+}
+
+// ptr_Arrays (pointers only) and consts (20 doubles, 6 ints) have no padding,
+// so equal bytes mean equal arguments.
+bool same_call(const ptr_Arrays& dev, const consts& def, int localNx, int rank, int size)
+{
+    return memcmp(&dev, &g_chain.dev, sizeof dev) == 0 &&
+           memcmp(&def, &g_chain.def, sizeof def) == 0 && localNx == g_chain.localNx &&
+           rank == g_chain.rank && size == g_chain.size;
+}
+
+// Runs the deferred stages, in the order the caller made the calls.
+void flush()
+{
+    const int s = g_chain.state;
+    g_chain.state = Chain::kNone;
+    if (s < Chain::kU)
+        return;
+    filt_params p = to_params(g_chain.def);
+    filt_fields f = to_fields(g_chain.dev);
+    finish(filt_u_calculation(&f, &p, g_chain.localNx, g_chain.rank, g_chain.size),
+           "u_calculation");
+    if (s == Chain::kURoS)
+        finish(filt_roS_calculation(&f, &p, g_chain.t, g_chain.localNx, g_chain.rank,
+                                    g_chain.size),
+               "roS_calculation");
+}
+
 }  // namespace
 
 void device_memory_alloc(ptr_Arrays* ArraysPtr, double** DevBuffer, int nX, consts def)
 {
+    flush();
     filt_params p = to_params(def);
     filt_fields f;
     const int rc = filt_device_memory_alloc(&f, DevBuffer, nX, &p);
@@ -168,18 +228,21 @@ void device_memory_alloc(ptr_Arrays* ArraysPtr, double** DevBuffer, int nX, cons
 
 void device_memory_free(ptr_Arrays DevArraysPtr, double* DevBuffer)
 {
+    flush();
     filt_fields f = to_fields(DevArraysPtr);
     report(filt_device_memory_free(&f, DevBuffer), "device_memory_free");
 }
 
 void load_data_to_host(double* HostArrayPtr, double* DevArrayPtr, int localNx, consts def)
 {
+    flush();
     filt_params p = to_params(def);
     report(filt_load_data_to_host(HostArrayPtr, DevArrayPtr, localNx, &p), "load_data_to_host");
 }
 
 void load_data_to_device(double* HostArrayPtr, double* DevArrayPtr, int localNx, consts def)
 {
+    flush();
     filt_params p = to_params(def);
     report(filt_load_data_to_device(HostArrayPtr, DevArrayPtr, localNx, &p),
            "load_data_to_device");
@@ -187,6 +250,7 @@ void load_data_to_device(double* HostArrayPtr, double* DevArrayPtr, int localNx,
 
 void load_data_to_device_int(int* HostArrayPtr, int* DevArrayPtr, int localNx, consts def)
 {
+    flush();
     filt_params p = to_params(def);
     report(filt_load_data_to_device_int(HostArrayPtr, DevArrayPtr, localNx, &p),
            "load_data_to_device_int");
@@ -195,6 +259,7 @@ void load_data_to_device_int(int* HostArrayPtr, int* DevArrayPtr, int localNx, c
 void Device_Initialize(int rank, int* blocksX, int* blocksY, int* blocksZ, int localNx,
                        consts def)
 {
+    flush();
     filt_params p = to_params(def);
     const int rc = filt_device_initialize(rank, localNx, &p, blocksX, blocksY, blocksZ);
     if (rc == FILT_OK) {
@@ -207,14 +272,29 @@ void Device_Initialize(int rank, int* blocksX, int* blocksY, int* blocksZ, int l
 void ro_P2_Xi_calculation(ptr_Arrays HostArraysPtr, ptr_Arrays DevArraysPtr, consts def,
                           int localNx, int rank, int size, int blocksX, int blocksY, int blocksZ)
 {
+    flush();
     filt_params p = to_params(def);
     filt_fields f = to_fields(DevArraysPtr);
-    finish(filt_ro_P2_Xi_calculation(&f, &p, localNx, rank, size), "ro_P2_Xi_calculation");
+    const int rc = filt_ro_P2_Xi_calculation(&f, &p, localNx, rank, size);
+    finish(rc, "ro_P2_Xi_calculation");
+    if (rc == FILT_OK && fuse_enabled()) {
+        g_chain.state = Chain::kStage1;
+        g_chain.dev = DevArraysPtr;
+        g_chain.def = def;
+        g_chain.localNx = localNx;
+        g_chain.rank = rank;
+        g_chain.size = size;
+    }
 }
 
 void u_calculation(ptr_Arrays HostArraysPtr, ptr_Arrays DevArraysPtr, int localNx, int rank,
                    int size, int blocksX, int blocksY, int blocksZ, consts def)
 {
+    if (g_chain.state == Chain::kStage1 && same_call(DevArraysPtr, def, localNx, rank, size)) {
+        g_chain.state = Chain::kU;  // runs in P1_S2_calculation or at the next flush
+        return;
+    }
+    flush();
     filt_params p = to_params(def);
     filt_fields f = to_fields(DevArraysPtr);
     finish(filt_u_calculation(&f, &p, localNx, rank, size), "u_calculation");
@@ -223,6 +303,12 @@ void u_calculation(ptr_Arrays HostArraysPtr, ptr_Arrays DevArraysPtr, int localN
 void roS_calculation(ptr_Arrays HostArraysPtr, ptr_Arrays DevArraysPtr, consts def, double t,
                      int localNx, int rank, int size, int blocksX, int blocksY, int blocksZ)
 {
+    if (g_chain.state == Chain::kU && same_call(DevArraysPtr, def, localNx, rank, size)) {
+        g_chain.state = Chain::kURoS;
+        g_chain.t = t;
+        return;
+    }
+    flush();
     filt_params p = to_params(def);
     filt_fields f = to_fields(DevArraysPtr);
     finish(filt_roS_calculation(&f, &p, t, localNx, rank, size), "roS_calculation");
@@ -233,12 +319,20 @@ void P1_S2_calculation(ptr_Arrays HostArraysPtr, ptr_Arrays DevArraysPtr, consts
 {
     filt_params p = to_params(def);
     filt_fields f = to_fields(DevArraysPtr);
+    if (g_chain.state == Chain::kURoS && same_call(DevArraysPtr, def, localNx, rank, size)) {
+        g_chain.state = Chain::kNone;
+        finish(filt_u_roS_P1_S2_calculation(&f, &p, g_chain.t, localNx, rank, size, 1),
+               "P1_S2_calculation");
+        return;
+    }
+    flush();
     finish(filt_P1_S2_calculation(&f, &p, localNx, rank, size), "P1_S2_calculation");
 }
 
 void boundary_conditions(ptr_Arrays HostArraysPtr, ptr_Arrays DevArraysPtr, int localNx,
                          int rank, int size, int blocksX, int blocksY, int blocksZ, consts def)
 {
+    flush();
     filt_params p = to_params(def);
     filt_fields f = to_fields(DevArraysPtr);
     finish(filt_boundary_conditions(&f, &p, localNx, rank, size), "boundary_conditions");
@@ -248,6 +342,7 @@ void load_exchange_data(double* HostArrayPtr, double* DevArrayPtr, double* HostB
                         double* DevBuffer, int localNx, int blocksY, int blocksZ, int rank,
                         int size, consts def)
 {
+    flush();
     filt_params p = to_params(def);
     report(filt_load_exchange_data(DevArrayPtr, DevBuffer, HostBuffer, localNx, &p),
            "load_exchange_data");
@@ -257,6 +352,7 @@ void save_exchange_data(double* HostArrayPtr, double* DevArrayPtr, double* HostB
                         double* DevBuffer, int localNx, int blocksY, int blocksZ, int rank,
                         int size, consts def)
 {
+    flush();
     filt_params p = to_params(def);
     report(filt_save_exchange_data(DevArrayPtr, DevBuffer, HostBuffer, localNx, rank, size, &p),
            "save_exchange_data");

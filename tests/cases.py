@@ -6,6 +6,7 @@ import numpy as np
 pkg = importlib.import_module("shmem-cuda-filtration_amd")
 abi = pkg.abi
 HostGrid = pkg.grid.HostGrid
+FIELD_NAMES = abi.FIELD_NAMES
 FIELD_NAMES_ALL = abi.FIELD_NAMES + ("media",)
 
 # defines.ini of the reference (defines.ini:5-49): the ini-controlled members.

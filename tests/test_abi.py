@@ -100,3 +100,17 @@ def test_no_cpu_fallback_in_product_sources():
             if fn.endswith((".py", ".hip", ".cpp", ".hpp", ".h")) or fn == "Makefile":
                 text = open(os.path.join(root, fn)).read()
                 assert not banned.search(text), fn
+
+
+def test_defines_shim_module_binds_every_entry_point():
+    """defines_shim binds the defines.h functions of libfiltration_defines.so
+    under the reference's mangled names, with the reference's struct sizes
+    (ptr_Arrays: 21 pointers; consts: 20 doubles and 6 ints, no padding)."""
+    import ctypes
+    import importlib
+    m = importlib.import_module("shmem-cuda-filtration_amd.defines_shim")
+    assert ctypes.sizeof(m.PtrArrays) == 21 * 8
+    assert ctypes.sizeof(m.Consts) == 20 * 8 + 6 * 4
+    s = m.Shim()
+    for name in m._SIGNATURES:
+        assert callable(getattr(s, name))

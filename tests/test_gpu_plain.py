@@ -13,7 +13,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from cases import FIELD_NAMES_ALL, HostGrid, c1_params, max_rel, random_state, ref_params
+from cases import FIELD_NAMES, FIELD_NAMES_ALL, HostGrid, c1_params, max_rel, random_state, ref_params
 
 pytestmark = pytest.mark.gpu
 
@@ -233,6 +233,22 @@ def test_reference_driver_links_and_runs_on_hip_backend(dims, source, steps):
     assert np.any(Sc[:, 1, :] > 0)  # NAPL has entered the domain
 
 
+def test_reference_driver_fused_shims_equal_separate_stages(monkeypatch):
+    """The reference driver on the HIP backend gives the same bits whether
+    the shims fuse stages 2-4 (default) or run them one by one
+    (FILT_FUSE_STAGES=0)."""
+    import oracle
+    if not (os.path.exists(oracle.REF_HIP_LIB) and oracle.ref_available()):
+        pytest.skip("oracle/_ref not built (reference tree absent at build time)")
+    p = ref_params(24, 20, 16, source=2)
+    gpu = oracle.RefCPU(oracle.REF_HIP_LIB)
+    monkeypatch.setenv("FILT_FUSE_STAGES", "1")
+    Pf, Sf = gpu.drive(p, 30)
+    monkeypatch.setenv("FILT_FUSE_STAGES", "0")
+    Ps, Ss = gpu.drive(p, 30)
+    assert np.array_equal(Pf, Ps) and np.array_equal(Sf, Ss)
+
+
 @pytest.mark.parametrize("rank,size", [(0, 2), (1, 3), (2, 3)])
 def test_exchange_pack_unpack_match_reference(hip, ref, rank, size):
     """load_exchange_data / save_exchange_data (gpu.cu:613-653): the HIP
@@ -263,6 +279,106 @@ def test_exchange_pack_unpack_match_reference(hip, ref, rank, size):
         assert np.array_equal(got, want)
     finally:
         d.free()
+
+
+def _shim_case(dims, seed):
+    import importlib
+    shim_mod = importlib.import_module("shmem-cuda-filtration_amd.defines_shim")
+    p = ref_params(*dims, source=1)
+    g = random_state(p, np.random.default_rng(seed))
+    return shim_mod, shim_mod.Shim(), p, g, shim_mod.consts_from_params(p)
+
+
+def _shim_upload(shim, dev, g, c):
+    for n in FIELD_NAMES:
+        a = np.ascontiguousarray(g.a[n])
+        shim.load_data_to_device(a.ctypes.data, getattr(dev, n), g.nx, c)
+    m = np.ascontiguousarray(g.a["media"])
+    shim.load_data_to_device_int(m.ctypes.data, dev.media, g.nx, c)
+
+
+def _shim_get(shim, dev, name, g, c):
+    """An FP64 array through load_data_to_host (media is int32: not through here)."""
+    out = np.empty_like(g.a[name])
+    shim.load_data_to_host(out.ctypes.data, getattr(dev, name), g.nx, c)
+    return out
+
+
+@pytest.mark.parametrize("fuse", ["1", "0"])
+@pytest.mark.parametrize("dims", [(33, 20, 17), (5, 4, 3), (9, 2, 6)])
+def test_defines_shim_time_step_equals_stage_calls(hip, dims, fuse, monkeypatch):
+    """The defines.h shims called in time_step_function's order
+    (main.cpp:86-92) defer u and roS and run stages 2-4 as one launch in
+    P1_S2_calculation; with FILT_FUSE_STAGES=0 they run one by one.  Either
+    way, two steps leave every array with the bits of the five C-ABI stage
+    calls.  (9, 2, 6) has an extent below 3: the fused entry point runs the
+    stages in turn."""
+    import ctypes
+    monkeypatch.setenv("FILT_FUSE_STAGES", fuse)
+    shim_mod, shim, p, g, c = _shim_case(dims, 21)
+    d = hip.grid(p)
+    try:
+        d.upload(g)
+        for s in range(2):
+            for n in (1, 2, 3, 4, 5):
+                d.stage(n, t=0.25 * (s + 1))
+        d.sync()
+        want = {n: d.get(n) for n in FIELD_NAMES}
+    finally:
+        d.free()
+    dev, buf = shim_mod.PtrArrays(), ctypes.c_void_p()
+    shim.device_memory_alloc(ctypes.byref(dev), ctypes.byref(buf), g.nx, c)
+    assert dev.P1
+    try:
+        _shim_upload(shim, dev, g, c)
+        for s in range(2):
+            shim.time_step(dev, c, 0.25 * (s + 1), g.nx)
+        for n in FIELD_NAMES:
+            assert np.array_equal(_shim_get(shim, dev, n, g, c), want[n]), n
+    finally:
+        shim.device_memory_free(dev, buf)
+
+
+@pytest.mark.parametrize("read_after", [2, 3])
+def test_defines_shim_reads_between_stages_see_every_stage(hip, read_after):
+    """A read through load_data_to_host after a deferred u_calculation (or
+    roS_calculation) runs what was deferred first: it returns the arrays the
+    separate stage calls leave, and the rest of the step, now unfused, still
+    ends with the five stage calls' bits."""
+    import ctypes
+    shim_mod, shim, p, g, c = _shim_case((24, 18, 11), 5)
+    d = hip.grid(p)
+    try:
+        d.upload(g)
+        for n in range(1, read_after + 1):
+            d.stage(n, t=0.5)
+        d.sync()
+        mid = {n: d.get(n) for n in STAGE_OUT[read_after]}
+        for n in range(read_after + 1, 6):
+            d.stage(n, t=0.5)
+        d.sync()
+        want = {n: d.get(n) for n in FIELD_NAMES}
+    finally:
+        d.free()
+    dev, buf = shim_mod.PtrArrays(), ctypes.c_void_p()
+    shim.device_memory_alloc(ctypes.byref(dev), ctypes.byref(buf), g.nx, c)
+    try:
+        _shim_upload(shim, dev, g, c)
+        host = shim_mod.PtrArrays()
+        shim.ro_P2_Xi_calculation(host, dev, c, g.nx, 0, 1, 0, 0, 0)
+        shim.u_calculation(host, dev, g.nx, 0, 1, 0, 0, 0, c)
+        if read_after == 3:
+            shim.roS_calculation(host, dev, c, 0.5, g.nx, 0, 1, 0, 0, 0)
+        for n, a in mid.items():
+            assert np.array_equal(_shim_get(shim, dev, n, g, c), a), n
+        if read_after == 2:
+            shim.roS_calculation(host, dev, c, 0.5, g.nx, 0, 1, 0, 0, 0)
+        shim.P1_S2_calculation(host, dev, c, g.nx, 0, 1, 0, 0, 0)
+        shim.boundary_conditions(host, dev, g.nx, 0, 1, 0, 0, 0, c)
+        for n in FIELD_NAMES:
+            assert np.array_equal(_shim_get(shim, dev, n, g, c), want[n]), n
+    finally:
+        shim.device_memory_free(dev, buf)
 
 
 def test_device_copy_ragged_sizes(hip):
