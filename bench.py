@@ -437,6 +437,26 @@ def run_serving(args):
     return out
 
 
+def stream_mix_ceiling(kernel, cells):
+    """Time of a pure stream with the kernel's mix of FP64 arrays (lib/stream_mix,
+    built from csrc/stream_mix.hip): the fastest any kernel reading and
+    writing those arrays once can run on this GPU.  Best of plain and
+    non-temporal stores; None when the tool is missing or has no such mix."""
+    import subprocess
+    exe = os.path.join(PKG.abi.LIB_DIR, "stream_mix")
+    if not os.path.exists(exe):
+        return None
+    out = subprocess.run([exe, str(cells), kernel], capture_output=True, text=True,
+                         timeout=120, check=True).stdout
+    rows = [json.loads(x) for x in out.splitlines() if x.startswith("{")]
+    if not rows:
+        return None
+    best = min(rows, key=lambda r: r["ms"])
+    return {"mix": f"{best['in']} in / {best['out']} out FP64 arrays",
+            "bytes_per_cell": best["bytes_per_cell"], "ms": best["ms"],
+            "GBps": best["TBps"] * 1e3, "nontemporal_stores": best["nontemporal_stores"]}
+
+
 def copy_calibration(n_bytes=1 << 30, reps=5):
     """The library's streaming copy kernel over n_bytes: the achievable HBM
     ceiling, and the reference point that scales the PMC byte counters."""
@@ -518,6 +538,9 @@ def main():
     achieved = dom_bytes / (stage_ms[dom] * 1e-3)
     tag = f"plain_{args.n}cube"
     traffic = load_traffic(tag, KERNEL_NAMES[dom])
+    mix_ceiling = stream_mix_ceiling(KERNEL_NAMES[dom], cells)
+    if mix_ceiling:
+        mix_ceiling["kernel_frac_of_mix"] = mix_ceiling["ms"] / stage_ms[dom]
     step_bytes = sum(PLAIN_BYTES[s] for s in STEP_STAGES) * cells
     split_bytes = sum(PLAIN_BYTES[s] for s in SPLIT_STAGES) * cells
     # the five stage functions one by one (the reference driver's calls)
@@ -566,6 +589,7 @@ def main():
             "frac": achieved / HBM_PEAK,
             "traffic": traffic,
             "algorithmic_bytes_per_launch": dom_bytes,
+            "stream_mix_ceiling": mix_ceiling,
         },
         "stages_ms": stage_ms,
         "step_roofline": {

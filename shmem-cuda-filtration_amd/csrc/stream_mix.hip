@@ -15,11 +15,13 @@
 //   k_step_zm     5 in, 12 out   P2, Xi1, Xi2, S2, poro -> six u, four roS, P1, S2
 //
 // Every mix runs with plain and with non-temporal stores.  Prints one JSON
-// line per mix and store kind.  Usage: stream_mix [cells] (default 256^3).
+// line per mix and store kind.  Usage: stream_mix [cells [mix]] (default
+// 256^3 cells, every mix; a mix name runs that mix alone).
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 
 #define CHECK(x)                                                                       \
     do {                                                                               \
@@ -111,6 +113,8 @@ void both(const char* name, const Streams& s, size_t n)
 int main(int argc, char** argv)
 {
     const size_t n = argc > 1 ? (size_t)atoll(argv[1]) : (size_t)256 * 256 * 256;
+    const char* only = argc > 2 ? argv[2] : nullptr;
+    auto want = [only](const char* name) { return !only || strcmp(only, name) == 0; };
     if (n == 0 || n > ((size_t)1 << 31)) {
         fprintf(stderr, "cells must be in [1, 2^31]\n");
         return 2;
@@ -126,7 +130,7 @@ int main(int argc, char** argv)
         CHECK(hipMalloc(&s.out[b], n * sizeof(double)));
     CHECK(hipDeviceSynchronize());
 
-    {   // the copy over 1 GiB each way, past the 256 MB last-level cache
+    if (want("copy")) {  // 1 GiB each way, past the 256 MB last-level cache
         const size_t nc = (size_t)1 << 27;
         Streams c = s;
         double *src, *dst;
@@ -139,11 +143,11 @@ int main(int argc, char** argv)
         CHECK(hipFree(src));
         CHECK(hipFree(dst));
     }
-    both<3, 5>("k_ro_P2_Xi", s, n);
-    both<6, 6>("k_u_zm", s, n);
-    both<8, 4>("k_roS_zm", s, n);
-    both<4, 2>("k_newton", s, n);
-    both<5, 12>("k_step_zm", s, n);
+    if (want("k_ro_P2_Xi")) both<3, 5>("k_ro_P2_Xi", s, n);
+    if (want("k_u_zm")) both<6, 6>("k_u_zm", s, n);
+    if (want("k_roS_zm")) both<8, 4>("k_roS_zm", s, n);
+    if (want("k_newton")) both<4, 2>("k_newton", s, n);
+    if (want("k_step_zm")) both<5, 12>("k_step_zm", s, n);
 
     for (int a = 0; a < kMaxIn; a++)
         CHECK(hipFree((void*)s.in[a]));
