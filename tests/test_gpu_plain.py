@@ -281,6 +281,65 @@ def test_exchange_pack_unpack_match_reference(hip, ref, rank, size):
         d.free()
 
 
+def test_arrays_beyond_4gib_take_64bit_kernels_equal_oracle(hip, orc):
+    """A grid whose arrays exceed 4 GiB (1024 x 1024 x 514 cells, 4.02 GiB per
+    array, 84 GiB on the device) is past the tiled kernels' 32-bit byte
+    offsets: stages 1-4 run on the one-cell-per-thread kernels with 64-bit
+    indices.  The last six z-planes (all beyond the first 4 GiB of every
+    array) are compared with the oracle run on those six planes alone, in the
+    planes 2-5 whose stencils stay inside the six: stage 1 to 1e-13 per
+    element (pow at ulp level), and stages 2-4, given the GPU's stage-1
+    arrays, bit for bit (lambda = 2, P_d = 0: exact shortcuts)."""
+    import ctypes
+    nx, ny, nz = 1024, 1024, 514
+    assert nx * ny * nz * 8 > 1 << 32
+    p = bl_params(nx, ny, nz)
+    bx, by, bz = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    if hip.lib.filt_device_initialize(0, nx, ctypes.byref(p), ctypes.byref(bx),
+                                      ctypes.byref(by), ctypes.byref(bz)) != 0:
+        pytest.skip("not enough device memory for 21 arrays of 4 GiB")
+    rng = np.random.default_rng(4)
+    P1 = rng.uniform(1e5, 1.3e5, (nz, ny, nx))
+    # S2 >= 0.05 keeps 1 - S_e^2 >= 0.1: Xi2's factor 1 - pow(S_e, 2) would
+    # otherwise turn glibc pow's occasional one-ulp error (it is not
+    # correctly rounded) into relative errors above 1e-13 among 2^22 cells
+    S2 = rng.uniform(0.05, 0.6, (nz, ny, nx))
+    plane = nx * ny
+    k0 = nz - 6
+    stage1 = ("P2", "Xi1", "Xi2", "ro1", "ro2")
+    later = ("u1x", "u1y", "u1z", "u2x", "u2y", "u2z", "roS1", "roS2", "roS1_old",
+             "roS2_old", "P1", "S2")
+    d = hip.grid(p)
+    try:
+        lib = hip.lib
+        assert lib.filt_memcpy(d.f.P1, P1.ctypes.data, P1.nbytes, 0) == 0
+        assert lib.filt_memcpy(d.f.S2, S2.ctypes.data, S2.nbytes, 0) == 0
+        for n in (1, 2, 3, 4):
+            d.stage(n, t=0.25)
+        d.sync()
+        got = {}
+        for n in stage1 + later:
+            a = np.empty((6, ny, nx))
+            assert lib.filt_memcpy(a.ctypes.data, getattr(d.f, n) + 8 * k0 * plane, a.nbytes,
+                                   1) == 0
+            got[n] = a
+    finally:
+        d.free()
+    p6 = bl_params(nx, ny, 6)
+    g = HostGrid(p6)
+    g["P1"] = P1[k0:]
+    g["S2"] = S2[k0:]
+    del P1, S2
+    orc.stage(1, g, p6)
+    for n in stage1:
+        assert elem_rel(got[n][2:], g[n][2:]) <= 1e-13, n
+        g[n] = got[n]
+    for n in (2, 3, 4):
+        orc.stage(n, g, p6, t=0.25)
+    for n in later:
+        assert np.array_equal(got[n][2:], g[n][2:]), n
+
+
 def _shim_case(dims, seed):
     import importlib
     shim_mod = importlib.import_module("shmem-cuda-filtration_amd.defines_shim")
