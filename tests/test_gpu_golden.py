@@ -5,8 +5,10 @@ Bars as for the oracle comparisons: stages without pow (u, roS, boundary) and
 the boundary known-answer test bit for bit; stages that call pow (ro_P2_Xi,
 the Newton with P_d != 0, which every fixture has) to 1e-13 relative per
 element; trajectories to relative L-inf 1e-10 on the arrays the fixtures hold
-after step 1 and at the end.  The initial state of a trajectory comes from
-the oracle's initial_data, which test_golden pins bit for bit."""
+after step 1 and at the end.  The fused forms run the 3D trajectories: FP64
+to 1e-10, mixed precision to its bars (S2 within 1e-4, P1 within 1e-6).  The
+initial state of a trajectory comes from the oracle's initial_data, which
+test_golden pins bit for bit."""
 import numpy as np
 import pytest
 
@@ -102,3 +104,25 @@ def test_trajectory_on_gpu(hip, orc, name):
                 assert max_rel(got[key[len(prefix):]], d[key]) <= 1e-10, (name, key)
                 checked += 1
     assert checked >= 4
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("name", ["traj_15cube_s3", "traj_16cube_two_media"])
+def test_fused_forms_trajectory_against_golden(hip, orc, name, precision):
+    """The fused one-launch step on the 3D golden trajectories: FP64 to the
+    trajectory bar (1e-10 relative L-inf), mixed precision to its own bars
+    (S2 within 1e-4, P1 within 1e-6 relative)."""
+    from test_gpu_fused import fused_run
+    d, meta = load(name)
+    p = params_of(meta)
+    g = HostGrid(p)
+    orc.initial_data(g, p)
+    if "media" in d.files:
+        g.a["media"][...] = d["media"]
+    P, S, _ = fused_run(hip, p, g, meta["steps"], precision=precision)
+    if precision == 0:
+        assert max_rel(P, d["final_P1"]) <= 1e-10
+        assert max_rel(S, d["final_S2"]) <= 1e-10
+    else:
+        assert float(np.max(np.abs(S - d["final_S2"]))) <= 1e-4
+        assert max_rel(P, d["final_P1"]) <= 1e-6
