@@ -248,6 +248,12 @@ int filt_boundary_conditions(const filt_fields* dev, const filt_params* p,
 int filt_u_roS_P1_S2_calculation(const filt_fields* dev, const filt_params* p,
                                  double t, int localNx, int rank, int size, int with_u);
 
+/* Number of launches of the one-kernel stages 2-4 (k_step_zm) since the
+ * library was loaded, from any entry point.  A diagnostic with no
+ * counterpart in the reference: it lets a caller see which path its calls
+ * took (for example that the defines.h shims fused u, roS and the Newton). */
+long long filt_step_zm_launches(void);
+
 /* One whole time step in the order of time_step_function (main.cpp:83-94):
  * stage 1, filt_u_roS_P1_S2_calculation, boundary conditions; the exchanges
  * are no-ops on one GPU (no_communication.cpp:3-14).  with_u = 0 skips

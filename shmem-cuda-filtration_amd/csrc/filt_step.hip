@@ -35,6 +35,8 @@
 // and the Newton on the interior.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include "../../include/filtration_hip.h"
 #include "filt_device.hpp"
 #include "filt_kernarg.hpp"
@@ -324,6 +326,9 @@ bool step_fusable(const filt_params& p, int nx, int rank, int size)
            tiled_enabled(2);
 }
 
+// Launches of k_step_zm since the library was loaded (filt_step_zm_launches).
+static std::atomic<long long> g_step_launches{0};
+
 int launch_step_zm(const filt_fields& f, const filt_params& p, int nx, bool with_u)
 {
     const TileConfig t = step_tile_config(nx, p.Ny, p.Nz);
@@ -342,7 +347,13 @@ int launch_step_zm(const filt_fields& f, const filt_params& p, int nx, bool with
                            stream(), a);
     }
     FILT_LAUNCH_CHECK("k_step_zm");
+    g_step_launches.fetch_add(1, std::memory_order_relaxed);
     return FILT_OK;
 }
 
 }  // namespace filt
+
+extern "C" long long filt_step_zm_launches(void)
+{
+    return filt::g_step_launches.load(std::memory_order_relaxed);
+}

@@ -240,12 +240,18 @@ def test_reference_driver_fused_shims_equal_separate_stages(monkeypatch):
     import oracle
     if not (os.path.exists(oracle.REF_HIP_LIB) and oracle.ref_available()):
         pytest.skip("oracle/_ref not built (reference tree absent at build time)")
+    import importlib
+    lib = importlib.import_module("shmem-cuda-filtration_amd").abi.load_library()
     p = ref_params(24, 20, 16, source=2)
     gpu = oracle.RefCPU(oracle.REF_HIP_LIB)
     monkeypatch.setenv("FILT_FUSE_STAGES", "1")
+    n0 = lib.filt_step_zm_launches()
     Pf, Sf = gpu.drive(p, 30)
+    n1 = lib.filt_step_zm_launches()
     monkeypatch.setenv("FILT_FUSE_STAGES", "0")
     Ps, Ss = gpu.drive(p, 30)
+    n2 = lib.filt_step_zm_launches()
+    assert (n1 - n0, n2 - n1) == (30, 0)  # the driver's own steps took the fused path
     assert np.array_equal(Pf, Ps) and np.array_equal(Sf, Ss)
 
 
@@ -390,8 +396,12 @@ def test_defines_shim_time_step_equals_stage_calls(hip, dims, fuse, monkeypatch)
     assert dev.P1
     try:
         _shim_upload(shim, dev, g, c)
+        before = hip.lib.filt_step_zm_launches()
         for s in range(2):
             shim.time_step(dev, c, 0.25 * (s + 1), g.nx)
+        fused = hip.lib.filt_step_zm_launches() - before
+        # the one-kernel path ran exactly when deferral is on and the grid allows it
+        assert fused == (2 if fuse == "1" and min(dims) >= 3 else 0)
         for n in FIELD_NAMES:
             assert np.array_equal(_shim_get(shim, dev, n, g, c), want[n]), n
     finally:
