@@ -309,8 +309,9 @@ int filt_autotune_stencil(const filt_fields* dev, const filt_params* p, int loca
  * each with its own parameters, optional per-cell permeability / porosity /
  * media and its state, all resident in HBM.  Results of a member do not
  * depend on the other members.  FILT_PREC_FP64 computes what the stage path
- * computes (same expressions); FILT_PREC_MIXED evaluates the water phase's
- * stencil fluxes in FP32 and keeps the NAPL phase's fluxes, pressure
+ * computes (same expressions); FILT_PREC_MIXED evaluates and stores the
+ * water phase's mobility-density products in FP32 (its fluxes are formed
+ * from them in FP64) and keeps the NAPL phase's mobility and fluxes, pressure
  * differences, the ro*S update, the Newton solve, the property fields and
  * the mass sums in FP64 (water, NAPL and total mass within 1e-9 relative of
  * the FP64 form).

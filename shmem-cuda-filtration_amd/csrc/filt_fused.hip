@@ -170,9 +170,12 @@ __device__ inline void fp64_update(const FusedGrid& G, const double* med, const 
 }
 
 // ----------------------------------------------------------- mixed form --
-// The water phase's stencil fluxes in FP32 on FP32 mobility-density products
-// L1 = -Xi1*ro1 >= 0.  The NAPL phase's in FP64: its mass is small and dilute,
-// and FP32 flux differences showed in it at 1e-9 to 1e-8 relative.  Pressure
+// The water phase's mobility-density products L1 = -Xi1*ro1 >= 0 are
+// evaluated and kept (in LDS) in FP32; its stencil fluxes are formed from them
+// in FP64 (FP32 flux arithmetic doubled the NAPL mass error through the
+// pressure, at no gain in speed).  The NAPL phase's mobility and fluxes are
+// FP64: its mass is small and dilute, and FP32 flux differences showed in it
+// at 1e-9 to 1e-8 relative.  Pressure
 // differences (the y face force -dP/h + ro*g cancels toward hydrostatic
 // balance), the ro(P)*S base, dt/phi times the flux, the Newton and the mass
 // sums in FP64.  Divisions by the uniform spacings, the viscosities and
@@ -250,11 +253,6 @@ struct MixedLds {
 };
 
 // upwind flux with the face forces a2 (+ face) and a1 (- face)
-__device__ __forceinline__ float uflux(float a2, float a1, float Lc, float Lm, float Lp)
-{
-    return (fmaxf(a2, 0.f) - fminf(a1, 0.f)) * Lc - fmaxf(a1, 0.f) * Lm + fminf(a2, 0.f) * Lp;
-}
-
 __device__ __forceinline__ double uflux64(double a2, double a1, double Lc, double Lm, double Lp)
 {
     return (fmax(a2, 0.) - fmin(a1, 0.)) * Lc - fmax(a1, 0.) * Lm + fmin(a2, 0.) * Lp;
@@ -273,15 +271,11 @@ __device__ inline void mixed_update(const FusedGrid& G, const double* med, const
     const double ihx = G.inv_h[0], ihy = G.inv_h[1], ihz = G.inv_h[2];
     const double ax2d = (p1 - L.P[xp]) * ihx, ax1d = (L.P[xm] - p1) * ihx;
     const double az2d = (p1 - zp.P) * ihz, az1d = (zm.P - p1) * ihz;
-    const float ax2 = (float)ax2d, ax1 = (float)ax1d;
-    const float az2 = (float)az2d, az1 = (float)az1d;
     const double dy2 = (p1 - L.P[yp]) * ihy, dy1 = (L.P[ym] - p1) * ihy;
     const double g = p.g_const;
-    const float fx = (float)ihx, fy = (float)ihy, fz = (float)ihz;
-    const float F1 = uflux(ax2, ax1, cc.L1, L.L1[xm], L.L1[xp]) * fx +
-                     uflux((float)(dy2 + r1c * g), (float)(dy1 + r1c * g), cc.L1, L.L1[ym],
-                           L.L1[yp]) * fy +
-                     uflux(az2, az1, cc.L1, zm.L1, zp.L1) * fz;
+    const double F1 = uflux64(ax2d, ax1d, cc.L1, L.L1[xm], L.L1[xp]) * ihx +
+                      uflux64(dy2 + r1c * g, dy1 + r1c * g, cc.L1, L.L1[ym], L.L1[yp]) * ihy +
+                      uflux64(az2d, az1d, cc.L1, zm.L1, zp.L1) * ihz;
     const double F2 = uflux64(ax2d, ax1d, cc.L2, L.L2[xm], L.L2[xp]) * ihx +
                       uflux64(dy2 + r2c * g, dy1 + r2c * g, cc.L2, L.L2[ym], L.L2[yp]) * ihy +
                       uflux64(az2d, az1d, cc.L2, zm.L2, zp.L2) * ihz;
@@ -291,7 +285,7 @@ __device__ inline void mixed_update(const FusedGrid& G, const double* med, const
     double rphi = __builtin_amdgcn_rcp(phi);
     rphi = fma(fma(-phi, rphi, 1.0), rphi, rphi);
     const double dtphi = p.dt * rphi;
-    const double A1 = r1c * (1 - S) - dtphi * (double)F1;
+    const double A1 = r1c * (1 - S) - dtphi * F1;
     const double A2 = r2c * S - dtphi * F2;
     // Newton of cpu.cpp:293-320 in FP64 with reciprocals
     const double ro01 = p.ro01, ro02 = p.ro02, b1 = p.beta1, b2 = p.beta2, Pa = p.P_atm;
