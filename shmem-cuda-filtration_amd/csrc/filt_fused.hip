@@ -170,22 +170,23 @@ __device__ inline void fp64_update(const FusedGrid& G, const double* med, const 
 }
 
 // ----------------------------------------------------------- mixed form --
-// The water phase's mobility-density products L1 = -Xi1*ro1 >= 0 are
-// evaluated and kept (in LDS) in FP32; its stencil fluxes are formed from them
-// in FP64 (FP32 flux arithmetic doubled the NAPL mass error through the
-// pressure, at no gain in speed).  The NAPL phase's mobility and fluxes are
-// FP64: its mass is small and dilute, and FP32 flux differences showed in it
-// at 1e-9 to 1e-8 relative.  Pressure
-// differences (the y face force -dP/h + ro*g cancels toward hydrostatic
-// balance), the ro(P)*S base, dt/phi times the flux, the Newton and the mass
-// sums in FP64.  Divisions by the uniform spacings, the viscosities and
-// 1 - S_wr are multiplications by reciprocals; 1/det is an FP64 reciprocal
-// refined by one Newton-Raphson step.  K and porosity stay FP64 (rounded to
-// FP32 they moved the NAPL mass by 2e-9 relative).
+// The fast form.  Its arithmetic differs from the FP64 form's where that buys
+// speed without moving the results: the Newton in closed form (capillary-free
+// case) or with refined reciprocals, the Brooks-Corey powers for lambda = 2
+// as products, and divisions by the uniform spacings, the viscosities and
+// 1 - S_wr as multiplications by reciprocals.  FP32 is left only in the
+// powers for lambda != 2 (powf, and expm1f / log1pf for 1 - S_e^e2), where
+// FP64 pow costs several times more.  Everything else is FP64: the mobility-
+// density products L1 = -Xi1*ro1 and L2 = -Xi2*ro2 (kept in LDS), the fluxes,
+// the pressure differences, the ro(P)*S base, dt/phi, the Newton, K and
+// porosity, and the mass sums.  On this GPU an unpacked FP32 operation costs
+// the issue slot of an FP64 one: with L1 and the water fluxes in FP32 the
+// form ran at the same speed and its NAPL mass drifted from the FP64 form's
+// by up to 1e-9 relative (through the pressure); with them in FP64 by 4e-13.
 struct M1 {
     double P;
     double L2;
-    float L1;
+    double L1;
 };
 
 struct RawM {
@@ -217,12 +218,13 @@ __device__ inline M1 mixed_eval(const FusedGrid& G, const double* med, const Raw
     // difference 1 - S_e would cancel to a few significant bits.
     const float oms = (float)(q.S * mtab(med, M_INV1, md));
     const float se = 1.f - oms;
-    float k1;
+    double k1;
     double k2d;
     const double lamd = mtab(med, M_LAM, md);
     const double omsd = q.S * mtab(med, M_INV1, md);
     if (lamd == 2.0) {  // Brooks-Corey exponents 4 and 2
-        const float s2 = se * se;
+        const double sed = 1.0 - omsd;
+        const double s2 = sed * sed;
         k1 = s2 * s2;
         k2d = omsd * omsd * (omsd * (2. - omsd));  // (1-S_e)^2 (1-S_e^2)
     } else {
@@ -231,18 +233,18 @@ __device__ inline M1 mixed_eval(const FusedGrid& G, const double* med, const Raw
         // 1 - S_e^e2 = -expm1(e2*log1p(-oms)), accurate for S_e near 1
         k2d = (double)(oms * oms * -expm1f((2.f + lam) / lam * log1pf(-oms)));
     }
-    const float r1 = (float)(G.p.ro01 * (1. + G.p.beta1 * (P - G.p.P_atm)));
+    const double r1 = G.p.ro01 * (1. + G.p.beta1 * (P - G.p.P_atm));
     const double r2 = G.p.ro02 * (1. + G.p.beta2 * (P - G.p.P_atm));
     M1 v;
     v.P = P;
-    v.L1 = (float)K * k1 * (float)G.inv_mu1 * r1;
+    v.L1 = K * k1 * G.inv_mu1 * r1;
     v.L2 = K * k2d * G.inv_mu2 * r2;
     return v;
 }
 
 struct MixedLds {
     double* P;
-    float* L1;
+    double* L1;
     double* L2;
     __device__ void put(int s, const M1& v) const
     {
@@ -451,7 +453,7 @@ __device__ void interior_tile(const FusedGrid& G, long long lb, int parity, char
     using V = typename std::conditional<MIXED, M1, C1>::type;
     using Raw = typename std::conditional<MIXED, RawM, RawD>::type;
     using Lds = typename std::conditional<MIXED, MixedLds, Fp64Lds>::type;
-    constexpr int BUF = MIXED ? SN * 20 : 5 * SN * 8;  // bytes of one LDS tile
+    constexpr int BUF = MIXED ? SN * 24 : 5 * SN * 8;  // bytes of one LDS tile
     const double* Pin = G.P[parity];
     const double* Sin = G.S[parity];
     double* Po = G.P[parity ^ 1];
@@ -493,7 +495,7 @@ __device__ void interior_tile(const FusedGrid& G, long long lb, int parity, char
         if constexpr (MIXED) {
             L.P = (double*)base;
             L.L2 = (double*)base + SN;
-            L.L1 = (float*)(base + 2 * SN * sizeof(double));
+            L.L1 = (double*)(base + 2 * SN * sizeof(double));
         } else {
             double* d = (double*)base;
             L.P = d;
@@ -594,7 +596,7 @@ __global__ __launch_bounds__(FT) void k_fused(const FusedGrid* __restrict__ grid
                                               const int* __restrict__ block_grid,
                                               long long total_blocks, int parity, int order)
 {
-    __shared__ __attribute__((aligned(16))) char lds[MIXED ? 2 * SN * 20 : LDS_BYTES];
+    __shared__ __attribute__((aligned(16))) char lds[MIXED ? 2 * SN * 24 : LDS_BYTES];
     __shared__ double med[M_ROWS * FILT_MAX_MEDIA];
     long long b = blockIdx.x;
     if (order == 0 && total_blocks % 8 == 0)

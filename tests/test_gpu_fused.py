@@ -2,7 +2,8 @@
 
 FP64 fused: same bits as the plain stage path on the GPU (same expressions),
 relative L-inf <= 1e-10 on P1 / S2 and 1e-12 on mass against the oracle.
-Mixed (FP32 water mobility, FP64 fluxes, NAPL phase and Newton): errors against the
+Mixed (the fast form: closed-form Newton, FP32 only in the lambda != 2
+powers): errors against the
 oracle stated per field and held to S2 L-inf <= 1e-4, P1 relative L-inf
 <= 1e-6, and water, NAPL and total mass <= 1e-9 relative.  Batching: a
 member's result is the same alone and in a mix, bit for bit."""
