@@ -135,7 +135,7 @@ def run_slab(args, nx, ny, nz):
     """One large grid, un-batched: a 512 x 512 x 256 slab (sigma 1.5,
     correlation 16 cells, ln K clipped at +-2 sigma), args.slab_steps steps
     through the plain five-stage path, the fused FP64 form and the mixed
-    form, so that the fusion / FP32 gain shows apart from batching."""
+    form, so that the fusion and fast-form gain shows apart from batching."""
     import types
     W = PKG.workloads
     t0 = time.perf_counter()
