@@ -309,12 +309,11 @@ int filt_autotune_stencil(const filt_fields* dev, const filt_params* p, int loca
  * each with its own parameters, optional per-cell permeability / porosity /
  * media and its state, all resident in HBM.  Results of a member do not
  * depend on the other members.  FILT_PREC_FP64 computes what the stage path
- * computes (same expressions); FILT_PREC_MIXED is the fast form: the
- * Newton in closed form where the capillary pressure vanishes, the lambda = 2
- * Brooks-Corey powers as products, divisions by uniform quantities as
- * multiplications by reciprocals, and FP32 only in the Brooks-Corey powers
- * for lambda != 2; mobilities, fluxes, the ro*S update, the Newton, the
- * property fields and the mass sums are FP64 (water, NAPL and total mass
+ * computes (same expressions); FILT_PREC_MIXED (named for FP32 parts it no
+ * longer has) is the fast form, all FP64: the Newton in closed form where the
+ * capillary pressure vanishes, the Brooks-Corey powers as products for
+ * lambda = 2 and as exp / expm1 of one log1p otherwise, divisions by uniform
+ * quantities as multiplications by reciprocals (water, NAPL and total mass
  * within 1e-9 relative of the FP64 form; measured 4e-13 or less).
  * Use an ensemble from one host thread at a time.
  */

@@ -43,6 +43,7 @@ struct filt_ensemble {
     int* d_block_grid = nullptr;
     size_t d_block_cap = 0;
     long long total_blocks = 0;
+    bool general_lambda = false;  // some member has a medium with lambda != 2
     double* d_partial = nullptr;
     size_t d_partial_cap = 0;
     double* d_mass = nullptr;
@@ -89,6 +90,9 @@ int sync_tables(filt_ensemble* e)
         bg.insert(bg.end(), (size_t)nb, (int)q);
     }
     e->total_blocks = first;
+    e->general_lambda = false;
+    for (const FusedGrid& g : gs)
+        e->general_lambda = e->general_lambda || fused_general_lambda(g);
     // launches already enqueued read the current tables: let them finish
     // before the tables are rewritten or reallocated
     FILT_HIP_TRY(hipStreamSynchronize(stream()));
@@ -302,7 +306,7 @@ int filt_ensemble_step(filt_ensemble* e, int nsteps)
             return rc;
         for (int s = 0; s < nsteps; s++) {
             rc = fused_launch(e->d_grids, e->d_block_grid, e->total_blocks, e->parity, e->precision,
-                              e->order, stream());
+                              e->general_lambda, e->order, stream());
             if (rc)
                 return rc;
             e->parity ^= 1;
@@ -328,7 +332,7 @@ int filt_ensemble_capture(filt_ensemble* e, int nsteps)
             int par = start;
             for (int s = 0; s < nsteps && rc == FILT_OK; s++) {
                 rc = fused_launch(e->d_grids, e->d_block_grid, e->total_blocks, par, e->precision,
-                                  e->order, stream());
+                                  e->general_lambda, e->order, stream());
                 par ^= 1;
             }
             hipGraph_t g = nullptr;

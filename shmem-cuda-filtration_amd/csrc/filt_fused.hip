@@ -170,19 +170,17 @@ __device__ inline void fp64_update(const FusedGrid& G, const double* med, const 
 }
 
 // ----------------------------------------------------------- mixed form --
-// The fast form.  Its arithmetic differs from the FP64 form's where that buys
-// speed without moving the results: the Newton in closed form (capillary-free
-// case) or with refined reciprocals, the Brooks-Corey powers for lambda = 2
-// as products, and divisions by the uniform spacings, the viscosities and
-// 1 - S_wr as multiplications by reciprocals.  FP32 is left only in the
-// powers for lambda != 2 (powf, and expm1f / log1pf for 1 - S_e^e2), where
-// FP64 pow costs several times more.  Everything else is FP64: the mobility-
-// density products L1 = -Xi1*ro1 and L2 = -Xi2*ro2 (kept in LDS), the fluxes,
-// the pressure differences, the ro(P)*S base, dt/phi, the Newton, K and
-// porosity, and the mass sums.  On this GPU an unpacked FP32 operation costs
-// the issue slot of an FP64 one: with L1 and the water fluxes in FP32 the
-// form ran at the same speed and its NAPL mass drifted from the FP64 form's
-// by up to 1e-9 relative (through the pressure); with them in FP64 by 4e-13.
+// The fast form (FILT_PREC_MIXED, named for FP32 parts it no longer has).
+// Its arithmetic differs from the FP64 form's where that buys speed without
+// moving the results: the Newton in closed form (capillary-free case) or with
+// refined reciprocals; the Brooks-Corey powers as products for lambda = 2 and
+// as exp / expm1 of one log1p otherwise; divisions by the uniform spacings,
+// the viscosities and 1 - S_wr as multiplications by reciprocals.  All of it
+// is FP64, the mobility-density products L1 = -Xi1*ro1 and L2 = -Xi2*ro2 kept
+// in LDS included.  On this GPU an unpacked FP32 operation costs the issue
+// slot of an FP64 one: with L1 and the water fluxes in FP32 the form ran at
+// the same speed and its NAPL mass drifted from the FP64 form's by up to 1e-9
+// relative (through the pressure), with FP32 powers for lambda != 2 by 5e-8.
 struct M1 {
     double P;
     double L2;
@@ -209,29 +207,32 @@ __device__ inline RawM mixed_raw(const FusedGrid& G, const double* med, const do
     return r;
 }
 
+// GENERAL: media with lambda != 2 occur in the batch.  Without it every
+// medium has lambda = 2 (the launch checks) and the kernel carries no
+// exp / log1p code, whose registers would halve its occupancy.
+template <bool GENERAL>
 __device__ inline M1 mixed_eval(const FusedGrid& G, const double* med, const RawM& q)
 {
     const int md = q.md;
     const double K = G.perm ? q.K : mtab(med, M_K, md);
     const double P = q.P;
-    // 1 - S_e = S/(1-S_wr) formed directly: at small NAPL saturation the FP32
-    // difference 1 - S_e would cancel to a few significant bits.
-    const float oms = (float)(q.S * mtab(med, M_INV1, md));
-    const float se = 1.f - oms;
+    // 1 - S_e = S/(1-S_wr) formed directly, without the cancellation of
+    // 1 - S_e at small NAPL saturation
     double k1;
     double k2d;
     const double lamd = mtab(med, M_LAM, md);
     const double omsd = q.S * mtab(med, M_INV1, md);
-    if (lamd == 2.0) {  // Brooks-Corey exponents 4 and 2
+    if (!GENERAL || lamd == 2.0) {  // Brooks-Corey exponents 4 and 2
         const double sed = 1.0 - omsd;
         const double s2 = sed * sed;
         k1 = s2 * s2;
         k2d = omsd * omsd * (omsd * (2. - omsd));  // (1-S_e)^2 (1-S_e^2)
     } else {
-        const float lam = (float)lamd;
-        k1 = powf(se, (2.f + 3.f * lam) / lam);
-        // 1 - S_e^e2 = -expm1(e2*log1p(-oms)), accurate for S_e near 1
-        k2d = (double)(oms * oms * -expm1f((2.f + lam) / lam * log1pf(-oms)));
+        // S_e^e1, and 1 - S_e^e2 = -expm1(e2*log1p(-oms)), accurate for S_e
+        // near 1
+        const double l1p = log1p(-omsd);
+        k1 = exp((2. + 3. * lamd) / lamd * l1p);
+        k2d = omsd * omsd * -expm1((2. + lamd) / lamd * l1p);
     }
     const double r1 = G.p.ro01 * (1. + G.p.beta1 * (P - G.p.P_atm));
     const double r2 = G.p.ro02 * (1. + G.p.beta2 * (P - G.p.P_atm));
@@ -446,7 +447,7 @@ __device__ inline void shell_cell(const FusedGrid& G, const double* Pin_, const 
 // index.  Two LDS tiles alternate, so one barrier per plane
 // orders both the writes before the reads and the reads of plane k before
 // the overwrite at plane k+2.
-template <bool MIXED>
+template <bool MIXED, bool GENERAL>
 __device__ void interior_tile(const FusedGrid& G, long long lb, int parity, char* lds,
                               const double* med)
 {
@@ -514,7 +515,7 @@ __device__ void interior_tile(const FusedGrid& G, long long lb, int parity, char
     };
     auto eval = [&](const Raw& q) -> V {
         if constexpr (MIXED)
-            return mixed_eval(G, med, q);
+            return mixed_eval<GENERAL>(G, med, q);
         else
             return fp64_eval(G, med, q);
     };
@@ -591,7 +592,7 @@ __device__ void interior_tile(const FusedGrid& G, long long lb, int parity, char
 
 constexpr size_t LDS_BYTES = 2 * 5 * SN * sizeof(double);  // two tiles of the FP64 layout
 
-template <bool MIXED>
+template <bool MIXED, bool GENERAL>
 __global__ __launch_bounds__(FT) void k_fused(const FusedGrid* __restrict__ grids,
                                               const int* __restrict__ block_grid,
                                               long long total_blocks, int parity, int order)
@@ -612,7 +613,7 @@ __global__ __launch_bounds__(FT) void k_fused(const FusedGrid* __restrict__ grid
                    : row == M_M ? p.m[q] : row == M_K ? p.K[q] : G.inv_1mswr[q];
         }
         __syncthreads();
-        interior_tile<MIXED>(G, lb, parity, lds, med);
+        interior_tile<MIXED, GENERAL>(G, lb, parity, lds, med);
         return;
     }
     const long long q = (lb - G.n_interior_blocks) * FT + threadIdx.y * FTX + threadIdx.x;
@@ -705,19 +706,32 @@ void fused_prepare(FusedGrid& G)
     G.n_shell_blocks = (shell + FT - 1) / FT;
 }
 
+bool fused_general_lambda(const FusedGrid& g)
+{
+    const int n = (g.media && g.p.n_media > 1) ? g.p.n_media : 1;
+    for (int q = 0; q < n; q++)
+        if (g.p.lambda[q] != 2.0)
+            return true;
+    return false;
+}
+
 int fused_launch(const FusedGrid* dev_grids, const int* dev_block_grid, long long total_blocks,
-                 int parity, int precision, int order, hipStream_t s)
+                 int parity, int precision, bool general_lambda, int order, hipStream_t s)
 {
     if (total_blocks <= 0)
         return FILT_OK;
     if (total_blocks > 0x7fffffffLL)
         return set_error(FILT_ERR_INVALID, "batch too large for one launch");
-    if (precision == FILT_PREC_MIXED)
-        hipLaunchKernelGGL(k_fused<true>, dim3((unsigned)total_blocks), dim3(FTX, FTY), 0, s,
-                           dev_grids, dev_block_grid, total_blocks, parity, order);
+    const dim3 grid((unsigned)total_blocks), block(FTX, FTY);
+    if (precision == FILT_PREC_MIXED && general_lambda)
+        hipLaunchKernelGGL((k_fused<true, true>), grid, block, 0, s, dev_grids, dev_block_grid,
+                           total_blocks, parity, order);
+    else if (precision == FILT_PREC_MIXED)
+        hipLaunchKernelGGL((k_fused<true, false>), grid, block, 0, s, dev_grids, dev_block_grid,
+                           total_blocks, parity, order);
     else
-        hipLaunchKernelGGL(k_fused<false>, dim3((unsigned)total_blocks), dim3(FTX, FTY), 0, s,
-                           dev_grids, dev_block_grid, total_blocks, parity, order);
+        hipLaunchKernelGGL((k_fused<false, false>), grid, block, 0, s, dev_grids, dev_block_grid,
+                           total_blocks, parity, order);
     FILT_LAUNCH_CHECK("k_fused");
     return FILT_OK;
 }

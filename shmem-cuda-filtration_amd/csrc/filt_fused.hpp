@@ -27,10 +27,15 @@ struct FusedGrid {
 // Fill the derived members (reciprocals, tile counts) from nx, ny, nz, kz, p.
 void fused_prepare(FusedGrid& G);
 
+// Whether a grid uses a medium with lambda != 2 (the fast form then needs
+// its general Brooks-Corey powers).
+bool fused_general_lambda(const FusedGrid& g);
+
 // One step of every grid in the batch: blocks [0, total_blocks) map to grids
 // through dev_block_grid; reads buffer `parity`, writes buffer parity ^ 1.
+// general_lambda: some grid has fused_general_lambda.
 int fused_launch(const FusedGrid* dev_grids, const int* dev_block_grid, long long total_blocks,
-                 int parity, int precision, int order, hipStream_t s);
+                 int parity, int precision, bool general_lambda, int order, hipStream_t s);
 
 // Deterministic total water / NAPL mass of one grid into dev_out[2].
 int fused_mass(const FusedGrid* dev_grid, const FusedGrid& G, int parity, double* dev_partial,

@@ -2,8 +2,8 @@
 
 FP64 fused: same bits as the plain stage path on the GPU (same expressions),
 relative L-inf <= 1e-10 on P1 / S2 and 1e-12 on mass against the oracle.
-Mixed (the fast form: closed-form Newton, FP32 only in the lambda != 2
-powers): errors against the
+Mixed (the fast form: closed-form Newton, lambda = 2 powers as products,
+all FP64): errors against the
 oracle stated per field and held to S2 L-inf <= 1e-4, P1 relative L-inf
 <= 1e-6, and water, NAPL and total mass <= 1e-9 relative.  Batching: a
 member's result is the same alone and in a mix, bit for bit."""
@@ -134,6 +134,29 @@ def test_mixed_heterogeneous_error(hip, orc):
     print(f"mixed 48^3 sigma 1.5, 100 steps: S2 Linf {eS:.3e}, P1 rel {max_rel(Pm, g['P1']):.3e},"
           f" water mass {e1:.3e}, NAPL mass {e2:.3e}, total mass {et:.3e}")
     assert eS <= 1e-4 and e1 <= 1e-9 and et <= 1e-9 and e2 <= 1e-9
+
+
+def test_mixed_reference_media_trajectory_error(hip, orc):
+    """The fast form on the reference's own media table (lambda 2.7 / 2.0,
+    P_d 755 / 2060: the general Brooks-Corey powers and the Newton with
+    capillary terms) on a random 0/1 media map, 100 steps, against the FP64
+    oracle, to the same bars (FP32 powers missed the NAPL one: 4.6e-8)."""
+    p = ref_params(24, 20, 16, source=2)
+    g = HostGrid(p)
+    orc.initial_data(g, p)
+    g.a["media"][...] = np.random.default_rng(3).integers(0, 2, g["P1"].shape)
+    Pm, Sm, mm = fused_run(hip, p, g, 100, precision=1)
+    orc.run(g, p, 100, with_u=0)
+    eS = float(np.max(np.abs(Sm - g["S2"])))
+    eP = max_rel(Pm, g["P1"])
+    mo = orc.mass(g, p)
+    e1, e2 = abs(mm[0] - mo[0]) / mo[0], abs(mm[1] - mo[1]) / mo[1]
+    et = abs(mm[0] + mm[1] - mo[0] - mo[1]) / (mo[0] + mo[1])
+    print(f"mixed reference media 100 steps: S2 Linf {eS:.3e}, P1 rel Linf {eP:.3e}, "
+          f"water mass {e1:.3e}, NAPL mass {e2:.3e}, total mass {et:.3e}")
+    assert mo[1] > 0
+    assert eS <= 1e-4 and eP <= 1e-6
+    assert e1 <= 1e-9 and e2 <= 1e-9 and et <= 1e-9
 
 
 @pytest.mark.parametrize("precision", [0, 1])
