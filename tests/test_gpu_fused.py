@@ -187,6 +187,34 @@ def test_member_same_alone_and_in_mix(hip, precision):
         e.close()
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_lambda2_member_same_beside_reference_media_member(hip, orc, precision):
+    """A batch that holds a member with lambda != 2 or P_d != 0 runs the
+    kernel with the general Brooks-Corey powers; a lambda = 2, P_d = 0 member
+    alone runs the kernel without them.  Its results are the same bits either
+    way, so a member's result still does not depend on the other members."""
+    import importlib
+    W = importlib.import_module("shmem-cuda-filtration_amd").workloads
+    p = W.base_params(20, 18, 14, h=W.INI_H)
+    perm, poro = W.gen_fields(p, 7, sigma=1.2, corr=4.0, clip=2.5)
+    g = W.initial_grid(p, perm, poro)
+    Pa, Sa, ma = fused_run(hip, p, g, 30, precision=precision)
+    q = ref_params(16, 12, 10, source=2)
+    h = HostGrid(q)
+    orc.initial_data(h, q)
+    h.a["media"][...] = np.random.default_rng(5).integers(0, 2, h["P1"].shape)
+    e = hip.ensemble(precision)
+    try:
+        a = e.add(p, g["P1"], g["S2"], g["perm"], g["poro"])
+        e.add(q, h["P1"], h["S2"], None, None, h.a["media"])
+        e.step(30)
+        P, S = e.get(a)
+        assert np.array_equal(P, Pa) and np.array_equal(S, Sa)
+        assert e.mass(a) == ma
+    finally:
+        e.close()
+
+
 def test_continuous_batching_add_remove_between_steps(hip):
     """Members admitted and retired while others run keep their own results."""
     import importlib
