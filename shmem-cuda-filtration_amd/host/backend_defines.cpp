@@ -25,7 +25,9 @@
 // whatever is deferred, so data read back through load_data_to_host or the
 // exchange functions is what the separate calls would have left.  Device
 // memory read by other means between u_calculation and P1_S2_calculation
-// would not hold u yet.  FILT_FUSE_STAGES=0 turns the deferral off.
+// would not hold u yet.  FILT_FUSE_STAGES=0 turns the deferral off.  The
+// deferred call is process-wide state: call the shims from one thread at a
+// time, as the reference driver does.
 #ifdef FILT_DEFINES_HEADER
 #include FILT_DEFINES_HEADER
 #else
