@@ -369,18 +369,21 @@ def _shim_get(shim, dev, name, g, c):
     return out
 
 
-@pytest.mark.parametrize("fuse", ["1", "0"])
+@pytest.mark.parametrize("fuse,scheme", [("1", 0), ("0", 0), ("1", 1)])
 @pytest.mark.parametrize("dims", [(33, 20, 17), (5, 4, 3), (9, 2, 6)])
-def test_defines_shim_time_step_equals_stage_calls(hip, dims, fuse, monkeypatch):
+def test_defines_shim_time_step_equals_stage_calls(hip, dims, fuse, scheme, monkeypatch):
     """The defines.h shims called in time_step_function's order
     (main.cpp:86-92) defer u and roS and run stages 2-4 as one launch in
     P1_S2_calculation; with FILT_FUSE_STAGES=0 they run one by one.  Either
     way, two steps leave every array with the bits of the five C-ABI stage
-    calls.  (9, 2, 6) has an extent below 3: the fused entry point runs the
-    stages in turn."""
+    calls.  (9, 2, 6) has an extent below 3, and the divgrad scheme
+    (FILT_ROS_SCHEME=1) has no one-kernel form: there the fused entry point
+    runs the stages in turn."""
     import ctypes
     monkeypatch.setenv("FILT_FUSE_STAGES", fuse)
+    monkeypatch.setenv("FILT_ROS_SCHEME", str(scheme))
     shim_mod, shim, p, g, c = _shim_case(dims, 21)
+    p.roS_scheme = scheme
     d = hip.grid(p)
     try:
         d.upload(g)
@@ -401,9 +404,9 @@ def test_defines_shim_time_step_equals_stage_calls(hip, dims, fuse, monkeypatch)
             shim.time_step(dev, c, 0.25 * (s + 1), g.nx)
         fused = hip.lib.filt_step_zm_launches() - before
         # the one-kernel path ran exactly when deferral is on and the grid allows it
-        assert fused == (2 if fuse == "1" and min(dims) >= 3 else 0)
+        assert fused == (2 if fuse == "1" and scheme == 0 and min(dims) >= 3 else 0)
         for n in FIELD_NAMES:
-            assert np.array_equal(_shim_get(shim, dev, n, g, c), want[n]), n
+            assert np.array_equal(_shim_get(shim, dev, n, g, c), want[n], equal_nan=True), n
     finally:
         shim.device_memory_free(dev, buf)
 
