@@ -281,7 +281,7 @@ def run_shim(args, p, perm, poro, steps, fuse):
             "state_finite": bool(np.isfinite(P1).all() and np.isfinite(S2).all())}
 
 
-FUSED_BYTES = {0: 48, 1: 48}  # P, S in + out (32) + perm, poro in FP64 (16)
+FUSED_BYTES = {0: 48, 1: 48, 2: 48}  # P, S in + out (32) + perm, poro in FP64 (16)
 
 
 def time_ensemble(e, steps, hiprt, lib):
@@ -309,7 +309,7 @@ def run_fused(args, p, perm, poro, precision):
     e = be.ensemble(precision)
     try:
         e.add(p, g["P1"], g["S2"], perm, poro)
-        kz, order, _ = e.autotune(3)
+        tiling = e.autotune(3)
         ms, wall = time_ensemble(e, args.steps, hiprt, be.lib)
     finally:
         e.close()
@@ -317,14 +317,14 @@ def run_fused(args, p, perm, poro, precision):
     e = be.ensemble(precision)
     try:
         mid = e.add(p, g["P1"], g["S2"], perm, poro)
-        e.set_tiling(kz, order)
+        e.apply_tiling(tiling)
         e.step(args.steps)
         P, S = e.get(mid)
         m = e.mass(mid)
     finally:
         e.close()
     cells = p.NX * p.Ny * p.Nz
-    return {"value": cells / (ms * 1e-3), "ms_per_step": ms, "kz": kz, "order": order,
+    return {"value": cells / (ms * 1e-3), "ms_per_step": ms, "tiling": tiling,
             "bytes_per_cell_update": FUSED_BYTES[precision],
             "hbm_GBps": FUSED_BYTES[precision] * cells / (ms * 1e-3) / 1e9,
             "wall_s": wall}, (P, S, m)
@@ -372,13 +372,13 @@ def run_c4(args):
     out["plain_per_realization"] = {"value": cells * steps / (plain_ms * 1e-3),
                                     "total_ms": plain_ms}
     results = {}
-    for prec, name in ((0, "batched_fp64"), (1, "batched_mixed")):
+    for prec, name in ((0, "batched_fp64"), (1, "batched_fast"), (2, "batched_mixed")):
         e = be.ensemble(prec)
         ids = []
         for m in members:
             g = W.initial_grid(m["params"], m["perm"], m["poro"])
             ids.append(e.add(m["params"], g["P1"], g["S2"], m["perm"], m["poro"]))
-        kz, order, _ = e.autotune(2)
+        tiling = e.autotune(2)
         e.close()
         # fresh ensemble: the timed run starts from the initial states
         e = be.ensemble(prec)
@@ -386,7 +386,7 @@ def run_c4(args):
         for m in members:
             g = W.initial_grid(m["params"], m["perm"], m["poro"])
             ids.append(e.add(m["params"], g["P1"], g["S2"], m["perm"], m["poro"]))
-        e.set_tiling(kz, order)
+        e.apply_tiling(tiling)
         e.capture(steps)
         be.sync()
         a.record(stream)
@@ -394,8 +394,8 @@ def run_c4(args):
         b.record(stream)
         b.sync()
         ms = a.elapsed_ms(b)
-        out[name] = {"value": cells * steps / (ms * 1e-3), "total_ms": ms, "kz": kz,
-                     "order": order, "bytes_per_cell_update": FUSED_BYTES[prec],
+        out[name] = {"value": cells * steps / (ms * 1e-3), "total_ms": ms, "tiling": tiling,
+                     "bytes_per_cell_update": FUSED_BYTES[prec],
                      "hbm_GBps": FUSED_BYTES[prec] * cells * steps / (ms * 1e-3) / 1e9}
         results[prec] = [(e.get(mid), e.mass(mid)) for mid in ids]
         e.close()
@@ -438,12 +438,12 @@ def run_serving(args):
 
 
 def stream_mix_ceiling(kernel, cells):
-    """Time of a pure stream with the kernel's mix of FP64 arrays (lib/stream_mix,
-    built from csrc/stream_mix.hip): the fastest any kernel reading and
+    """Time of a pure stream with the kernel's mix of FP64 arrays
+    (tests/hip/bin/stream_mix, built from tests/hip/stream_mix.hip): the fastest any kernel reading and
     writing those arrays once can run on this GPU.  Best of plain and
     non-temporal stores; None when the tool is missing or has no such mix."""
     import subprocess
-    exe = os.path.join(PKG.abi.LIB_DIR, "stream_mix")
+    exe = os.path.join(ROOT, "tests", "hip", "bin", "stream_mix")
     if not os.path.exists(exe):
         return None
     out = subprocess.run([exe, str(cells), kernel], capture_output=True, text=True,
@@ -521,9 +521,11 @@ def main():
     if not args.plain_only:
         import numpy as np
         f64, (P64, S64, m64) = run_fused(args, p, perm, poro, 0)
-        mix, (Pm, Sm, mm) = run_fused(args, p, perm, poro, 1)
+        fast, (Pf, Sf, mf) = run_fused(args, p, perm, poro, 1)
+        fast["error_vs_fp64"] = mixed_errors(P64, S64, m64, Pf, Sf, mf)
+        mix, (Pm, Sm, mm) = run_fused(args, p, perm, poro, 2)
         mix["error_vs_fp64"] = mixed_errors(P64, S64, m64, Pm, Sm, mm)
-        forms = {"fused_fp64": f64, "mixed": mix}
+        forms = {"fused_fp64": f64, "fast": fast, "mixed": mix}
         if args.slab_steps > 0:
             for shape in ((512, 512, 256), (256, 512, 512)):
                 forms["slab_%dx%dx%d" % shape] = run_slab(args, *shape)

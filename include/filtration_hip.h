@@ -308,22 +308,39 @@ int filt_autotune_stencil(const filt_fields* dev, const filt_params* p, int loca
  * An ensemble holds member grids of any shapes (every extent >= 3, one rank),
  * each with its own parameters, optional per-cell permeability / porosity /
  * media and its state, all resident in HBM.  Results of a member do not
- * depend on the other members.  FILT_PREC_FP64 computes what the stage path
- * computes (same expressions); FILT_PREC_MIXED (named for FP32 parts it no
- * longer has) is the fast form, all FP64: the Newton in closed form where the
- * capillary pressure vanishes, the Brooks-Corey powers as products for
- * lambda = 2 and as exp / expm1 of one log1p otherwise, divisions by uniform
- * quantities as multiplications by reciprocals (water, NAPL and total mass
- * within 1e-9 relative of the FP64 form; measured 4e-13 or less).
+ * depend on the other members.  Three arithmetic forms:
+ *   FILT_PREC_FP64  computes what the stage path computes (same expressions,
+ *                   same bits as filt_time_step);
+ *   FILT_PREC_FAST  all FP64, with algebraic shortcuts: the Newton in closed
+ *                   form where the capillary pressure vanishes, the
+ *                   Brooks-Corey powers as products for lambda = 2 and as
+ *                   exp / expm1 of one log1p otherwise, divisions by uniform
+ *                   quantities as multiplications by reciprocals;
+ *   FILT_PREC_MIXED the upwind fluxes and the ro*S increment in FP32 (FP32
+ *                   mobilities in LDS); the pressure differences and face
+ *                   forces, the ro(P)(1-S) base, the Newton and the mass sums
+ *                   in FP64.
+ * Members must pass the stability guard (filt_stability_number <= 2 with
+ * their fields), else filt_ensemble_add returns FILT_ERR_UNSTABLE.
  * Use an ensemble from one host thread at a time.
  */
 #define FILT_PREC_FP64 0
-#define FILT_PREC_MIXED 1
+#define FILT_PREC_FAST 1
+#define FILT_PREC_MIXED 2
 
 typedef struct filt_ensemble filt_ensemble;
 
 int filt_ensemble_create(filt_ensemble** out, int precision);
 int filt_ensemble_destroy(filt_ensemble* e);
+
+/* What filt_ensemble_add would refuse for these parameters and fields
+ * (perm, poro may be null), without touching the device: FILT_ERR_INVALID
+ * (bad precision or media table), FILT_ERR_UNSUPPORTED (an extent below 3,
+ * the divgrad scheme, an array of 4 GiB or more, or for the FP64 form a
+ * divisor outside [2^-100, 2^101]), FILT_ERR_UNSTABLE (stability guard > 2);
+ * else FILT_OK. */
+int filt_ensemble_check(const filt_params* p, int precision, const double* perm,
+                        const double* poro);
 
 /* Add a member: copies the host arrays (NX*Ny*Nz cells each, x fastest) to
  * the device.  perm, poro, media may be null (media table entry 0 / the
@@ -354,13 +371,17 @@ int filt_ensemble_get(filt_ensemble* e, int id, double* P1, double* S2);
  * sum phi*ro(P)*S*h1*h2*h3, FP64, fixed summation order. */
 int filt_ensemble_mass(filt_ensemble* e, int id, double* out2);
 
-/* Tiling of the fused kernel: kz planes per z-chunk (tiles are 16x16
- * columns), order 0 = XCD-grouped, 1 = hardware block order. */
-int filt_ensemble_set_tiling(filt_ensemble* e, int kz, int order);
-/* Autotuner: times each (kz, order) candidate over reps steps of the current
- * membership (the members advance by the steps timed) and keeps the fastest. */
-int filt_ensemble_autotune(filt_ensemble* e, int reps, int* best_kz, int* best_order,
-                           double* best_ms_per_step);
+/* Tiling of the fused kernel: tx x ty columns per tile (16x16, 32x8, 64x4,
+ * 32x16 or 64x8; 0x0: 16x16), kz planes per z-chunk, order 0 = XCD-grouped,
+ * 1 = hardware block order.  Results do not depend on the tiling. */
+int filt_ensemble_set_tiling(filt_ensemble* e, int tx, int ty, int kz, int order);
+int filt_ensemble_get_tiling(const filt_ensemble* e, int* tx, int* ty, int* kz, int* order);
+/* Autotuner: times each (tile shape, kz, order) candidate -- every tile
+ * shape above, kz in {4, 8, 16, 32, 64}, both orders -- over reps steps of
+ * the current membership (the members advance by the steps run) and keeps
+ * the fastest.  Outputs may be null. */
+int filt_ensemble_autotune(filt_ensemble* e, int reps, int* best_tx, int* best_ty, int* best_kz,
+                           int* best_order, double* best_ms_per_step);
 
 /* Free / total device memory (admission control). */
 int filt_device_free_bytes(size_t* free_bytes, size_t* total_bytes);
@@ -381,11 +402,16 @@ typedef struct filt_server filt_server;
 int filt_serve_create(filt_server** out, int precision, size_t reserve_bytes, int max_active,
                       int max_round_steps);
 int filt_serve_destroy(filt_server* s);
-/* Queue a request (arrays copied; perm, poro, media may be null). */
+/* Queue a request (arrays copied; perm, poro, media may be null).  Refuses
+ * what filt_ensemble_check refuses, and with FILT_ERR_NOMEM a request whose
+ * device bytes plus the reserve exceed the device's total memory. */
 int filt_serve_submit(filt_server* s, const filt_params* p, const double* P1, const double* S2,
                       const double* perm, const double* poro, const int* media,
                       long long nsteps, int* id);
-/* One round: admit, step, retire.  Counts of this round (may be null). */
+/* One round: admit, step, retire.  Counts of this round (may be null).
+ * Returns FILT_ERR_NOMEM when nothing is active and the request at the head
+ * of the queue still does not fit in the free device memory (it stays
+ * queued). */
 int filt_serve_round(filt_server* s, int* n_admitted, int* n_retired);
 int filt_serve_status(const filt_server* s, int* queued, int* active, int* done,
                       long long* active_cells, long long* cell_updates);
@@ -444,7 +470,10 @@ int filt_save_plots_host(const char* dir, const filt_fields* host, const filt_pa
  * plots (plot_dir, or NULL) as filt_save / filt_save_plots would have at
  * the moment of the take.  wait: joins the writer and returns its status.
  * restore: stream-ordered copy back into a grid (rollback, preemption).
- * A take or write_async waits for the snapshot's previous write. */
+ * A take or write_async waits for the snapshot's previous write.  A failed
+ * write (bad path, full disk, short write) is reported once, by whichever
+ * of take, write_async, wait or destroy comes next: that call returns the
+ * write's status (take and write_async then do nothing). */
 #define FILT_SNAP_SAVE 1
 #define FILT_SNAP_PLOTS 2
 typedef struct filt_snapshot filt_snapshot;

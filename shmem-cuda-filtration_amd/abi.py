@@ -26,6 +26,9 @@ FILT_SNAP_SAVE = 1
 FILT_SNAP_PLOTS = 2
 FILT_SCHEME_UPWIND = 0
 FILT_SCHEME_DIVGRAD = 1
+FILT_PREC_FP64 = 0
+FILT_PREC_FAST = 1
+FILT_PREC_MIXED = 2
 
 _D = c_double * FILT_MAX_MEDIA
 
@@ -156,6 +159,7 @@ def load_library(path=HIP_LIB):
     E = c_void_p
     _sig(lib, "filt_ensemble_create", c_int, POINTER(c_void_p), c_int)
     _sig(lib, "filt_ensemble_destroy", c_int, E)
+    _sig(lib, "filt_ensemble_check", c_int, P, c_int, c_void_p, c_void_p)
     _sig(lib, "filt_ensemble_add", c_int, E, P, c_void_p, c_void_p, c_void_p, c_void_p,
          c_void_p, pi)
     _sig(lib, "filt_ensemble_remove", c_int, E, c_int)
@@ -169,8 +173,9 @@ def load_library(path=HIP_LIB):
     _sig(lib, "filt_ensemble_replay", c_int, E)
     _sig(lib, "filt_ensemble_get", c_int, E, c_int, c_void_p, c_void_p)
     _sig(lib, "filt_ensemble_mass", c_int, E, c_int, POINTER(c_double))
-    _sig(lib, "filt_ensemble_set_tiling", c_int, E, c_int, c_int)
-    _sig(lib, "filt_ensemble_autotune", c_int, E, c_int, pi, pi, POINTER(c_double))
+    _sig(lib, "filt_ensemble_set_tiling", c_int, E, c_int, c_int, c_int, c_int)
+    _sig(lib, "filt_ensemble_get_tiling", c_int, E, pi, pi, pi, pi)
+    _sig(lib, "filt_ensemble_autotune", c_int, E, c_int, pi, pi, pi, pi, POINTER(c_double))
     _sig(lib, "filt_device_free_bytes", c_int, POINTER(c_size_t), POINTER(c_size_t))
     _sig(lib, "filt_serve_create", c_int, POINTER(c_void_p), c_int, c_size_t, c_int, c_int)
     _sig(lib, "filt_serve_destroy", c_int, c_void_p)

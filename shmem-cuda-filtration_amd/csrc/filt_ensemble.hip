@@ -25,6 +25,7 @@ using namespace filt;
 
 struct filt_ensemble {
     int precision = FILT_PREC_FP64;
+    int tile = 0;  // index into FUSED_TILES
     int kz = 32;
     int order = 0;
     int parity = 0;
@@ -81,7 +82,7 @@ int sync_tables(filt_ensemble* e)
     for (size_t q = 0; q < e->members.size(); q++) {
         FusedGrid g = e->members[q].g;
         g.kz = e->kz;
-        fused_prepare(g);
+        fused_prepare(g, e->tile);
         g.first_block = first;
         const long long nb = g.n_interior_blocks + g.n_shell_blocks;
         first += nb;
@@ -138,8 +139,10 @@ int filt_ensemble_create(filt_ensemble** out, int precision)
     return filt::guarded([&]() -> int {
         if (!out)
             return set_error(FILT_ERR_INVALID, "null out");
-        if (precision != FILT_PREC_FP64 && precision != FILT_PREC_MIXED)
-            return set_error(FILT_ERR_INVALID, "precision must be FILT_PREC_FP64 or FILT_PREC_MIXED");
+        if (precision != FILT_PREC_FP64 && precision != FILT_PREC_FAST &&
+            precision != FILT_PREC_MIXED)
+            return set_error(FILT_ERR_INVALID,
+                             "precision must be FILT_PREC_FP64, FILT_PREC_FAST or FILT_PREC_MIXED");
         filt_ensemble* e = new filt_ensemble;
         e->precision = precision;
         hipError_t r = hipMalloc((void**)&e->d_mass, 2 * sizeof(double));
@@ -172,10 +175,12 @@ int filt_ensemble_destroy(filt_ensemble* e)
     return FILT_OK;
 }
 
-int filt_ensemble_set_tiling(filt_ensemble* e, int kz, int order)
+int filt_ensemble_set_tiling(filt_ensemble* e, int tx, int ty, int kz, int order)
 {
-    if (!e || kz < 1 || kz > 4096 || (order != 0 && order != 1))
-        return set_error(FILT_ERR_INVALID, "bad tiling kz=%d order=%d", kz, order);
+    const int tile = (tx == 0 && ty == 0) ? 0 : fused_tile_index(tx, ty);
+    if (!e || tile < 0 || kz < 1 || kz > 4096 || (order != 0 && order != 1))
+        return set_error(FILT_ERR_INVALID, "bad tiling %dx%d kz=%d order=%d", tx, ty, kz, order);
+    e->tile = tile;
     e->kz = kz;
     e->order = order;
     e->dirty = true;
@@ -189,9 +194,46 @@ size_t filt_ensemble_member_bytes(const filt_params* p, int precision, int with_
     if (!p)
         return 0;
     const size_t n = (size_t)p->NX * p->Ny * p->Nz;
-    (void)precision;  // both forms keep K and porosity in FP64
+    (void)precision;  // every form keeps K and porosity in FP64
     return 4 * round256(n * 8) + (with_perm ? round256(n * 8) : 0) +
            (with_poro ? round256(n * 8) : 0) + (with_media ? round256(n * 4) : 0);
+}
+
+int filt_ensemble_check(const filt_params* p, int precision, const double* perm,
+                        const double* poro)
+{
+    return filt::guarded([&]() -> int {
+        if (!p)
+            return set_error(FILT_ERR_INVALID, "null params");
+        if (precision != FILT_PREC_FP64 && precision != FILT_PREC_FAST &&
+            precision != FILT_PREC_MIXED)
+            return set_error(FILT_ERR_INVALID, "unknown precision %d", precision);
+        if (p->NX < 3 || p->Ny < 3 || p->Nz < 3)
+            return set_error(FILT_ERR_UNSUPPORTED, "the fused step needs every extent >= 3");
+        if (p->roS_scheme != FILT_SCHEME_UPWIND)
+            return set_error(FILT_ERR_UNSUPPORTED,
+                             "the fused step implements the upwind roS scheme only (the divgrad "
+                             "scheme runs on the stage path)");
+        // the kernels address arrays by 32-bit byte offsets
+        if ((double)p->NX * p->Ny * p->Nz * 8.0 >= 4294967296.0)
+            return set_error(FILT_ERR_UNSUPPORTED,
+                             "the fused step keeps every array below 4 GiB (%dx%dx%d)", p->NX,
+                             p->Ny, p->Nz);
+        if (p->n_media < 0 || p->n_media > FILT_MAX_MEDIA || p->newton_iterations < 0)
+            return set_error(FILT_ERR_INVALID, "bad n_media / newton_iterations");
+        if (precision == FILT_PREC_FP64 && !fused_divisors_ok(*p))
+            return set_error(FILT_ERR_UNSUPPORTED,
+                             "the FP64 form needs h1, h2, h3, mu1, mu2 and 1 - S_wr in "
+                             "[2^-100, 2^101]");
+        // the guard maximises over the media table and the per-cell fields
+        const size_t n = (size_t)p->NX * p->Ny * p->Nz;
+        const double r = filt_stability_number(p, perm, poro, n);
+        if (!(r <= 2.0))
+            return set_error(FILT_ERR_UNSTABLE,
+                             "stability guard: amplification number %.3g > 2 (explicit scheme "
+                             "unstable for these fields and dt)", r);
+        return FILT_OK;
+    });
 }
 
 int filt_ensemble_add(filt_ensemble* e, const filt_params* p, const double* P1, const double* S2,
@@ -200,16 +242,9 @@ int filt_ensemble_add(filt_ensemble* e, const filt_params* p, const double* P1, 
     return filt::guarded([&]() -> int {
         if (!e || !p || !P1 || !S2)
             return set_error(FILT_ERR_INVALID, "null argument");
-        if (p->NX < 3 || p->Ny < 3 || p->Nz < 3)
-            return set_error(FILT_ERR_UNSUPPORTED, "the fused step needs every extent >= 3");
-        if (p->roS_scheme != FILT_SCHEME_UPWIND)
-            return set_error(FILT_ERR_UNSUPPORTED,
-                             "the fused step implements the upwind roS scheme only (the divgrad "
-                             "scheme runs on the stage path)");
-        if ((double)p->NX * p->Ny * p->Nz > 2.0e9)
-            return set_error(FILT_ERR_INVALID, "grid too large");
-        if (p->n_media < 0 || p->n_media > FILT_MAX_MEDIA || p->newton_iterations < 0)
-            return set_error(FILT_ERR_INVALID, "bad n_media / newton_iterations");
+        int rc = filt_ensemble_check(p, e->precision, perm, poro);
+        if (rc)
+            return rc;
         const size_t n = (size_t)p->NX * p->Ny * p->Nz;
         const size_t bytes = filt_ensemble_member_bytes(p, e->precision, perm != nullptr,
                                                         poro != nullptr, media != nullptr);
@@ -306,7 +341,7 @@ int filt_ensemble_step(filt_ensemble* e, int nsteps)
             return rc;
         for (int s = 0; s < nsteps; s++) {
             rc = fused_launch(e->d_grids, e->d_block_grid, e->total_blocks, e->parity, e->precision,
-                              e->general_lambda, e->order, stream());
+                              e->general_lambda, e->tile, e->order, stream());
             if (rc)
                 return rc;
             e->parity ^= 1;
@@ -332,7 +367,7 @@ int filt_ensemble_capture(filt_ensemble* e, int nsteps)
             int par = start;
             for (int s = 0; s < nsteps && rc == FILT_OK; s++) {
                 rc = fused_launch(e->d_grids, e->d_block_grid, e->total_blocks, par, e->precision,
-                                  e->general_lambda, e->order, stream());
+                                  e->general_lambda, e->tile, e->order, stream());
                 par ^= 1;
             }
             hipGraph_t g = nullptr;
@@ -440,46 +475,62 @@ int filt_device_free_bytes(size_t* free_b, size_t* total_b)
     return FILT_OK;
 }
 
-// Time every (kz, order) for the current membership: reps steps each (the
-// state advances by the steps timed), keep the fastest.
-int filt_ensemble_autotune(filt_ensemble* e, int reps, int* best_kz, int* best_order,
-                           double* best_ms)
+int filt_ensemble_get_tiling(const filt_ensemble* e, int* tx, int* ty, int* kz, int* order)
+{
+    if (!e)
+        return set_error(FILT_ERR_INVALID, "null ensemble");
+    if (tx) *tx = FUSED_TILES[e->tile].tx;
+    if (ty) *ty = FUSED_TILES[e->tile].ty;
+    if (kz) *kz = e->kz;
+    if (order) *order = e->order;
+    return FILT_OK;
+}
+
+// Time every (tile shape, kz, order) for the current membership: reps steps
+// each after one untimed step (the state advances by the steps run), keep
+// the fastest.
+int filt_ensemble_autotune(filt_ensemble* e, int reps, int* best_tx, int* best_ty, int* best_kz,
+                           int* best_order, double* best_ms)
 {
     return filt::guarded([&]() -> int {
         if (!e || reps < 1)
             return set_error(FILT_ERR_INVALID, "bad arguments");
         static const int kzs[] = {4, 8, 16, 32, 64};
-        hipEvent_t a, b;
-        FILT_HIP_TRY(hipEventCreate(&a));
-        FILT_HIP_TRY(hipEventCreate(&b));
+        hipEvent_t a = nullptr, b = nullptr;
+        int rc = FILT_OK;
+        hipError_t he = hipEventCreate(&a);
+        if (he == hipSuccess)
+            he = hipEventCreate(&b);
         double best = 1e30;
-        int bk = e->kz, bo = e->order;
-        for (int kz : kzs)
-            for (int order = 0; order < 2; order++) {
-                int rc = filt_ensemble_set_tiling(e, kz, order);
-                if (!rc) rc = filt_ensemble_step(e, 1);  // warm
-                if (rc) {
-                    (void)hipEventDestroy(a);
-                    (void)hipEventDestroy(b);
-                    return rc;
+        int bt = e->tile, bk = e->kz, bo = e->order;
+        for (int tile = 0; tile < N_FUSED_TILES && rc == FILT_OK && he == hipSuccess; tile++)
+            for (int kz : kzs)
+                for (int order = 0; order < 2 && rc == FILT_OK && he == hipSuccess; order++) {
+                    rc = filt_ensemble_set_tiling(e, FUSED_TILES[tile].tx, FUSED_TILES[tile].ty,
+                                                  kz, order);
+                    if (!rc) rc = filt_ensemble_step(e, 1);  // warm
+                    if (!rc) he = hipEventRecord(a, stream());
+                    if (!rc && he == hipSuccess) rc = filt_ensemble_step(e, reps);
+                    if (!rc && he == hipSuccess) he = hipEventRecord(b, stream());
+                    if (!rc && he == hipSuccess) he = hipEventSynchronize(b);
+                    float ms = 0;
+                    if (!rc && he == hipSuccess) he = hipEventElapsedTime(&ms, a, b);
+                    if (!rc && he == hipSuccess && ms / reps < best) {
+                        best = ms / reps;
+                        bt = tile;
+                        bk = kz;
+                        bo = order;
+                    }
                 }
-                FILT_HIP_TRY(hipEventRecord(a, stream()));
-                rc = filt_ensemble_step(e, reps);
-                if (rc)
-                    return rc;
-                FILT_HIP_TRY(hipEventRecord(b, stream()));
-                FILT_HIP_TRY(hipEventSynchronize(b));
-                float ms = 0;
-                FILT_HIP_TRY(hipEventElapsedTime(&ms, a, b));
-                if (ms / reps < best) {
-                    best = ms / reps;
-                    bk = kz;
-                    bo = order;
-                }
-            }
-        (void)hipEventDestroy(a);
-        (void)hipEventDestroy(b);
-        int rc = filt_ensemble_set_tiling(e, bk, bo);
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+        if (he != hipSuccess)
+            return hip_fail(he, "filt_ensemble_autotune");
+        if (rc)
+            return rc;
+        rc = filt_ensemble_set_tiling(e, FUSED_TILES[bt].tx, FUSED_TILES[bt].ty, bk, bo);
+        if (best_tx) *best_tx = FUSED_TILES[bt].tx;
+        if (best_ty) *best_ty = FUSED_TILES[bt].ty;
         if (best_kz) *best_kz = bk;
         if (best_order) *best_order = bo;
         if (best_ms) *best_ms = best;

@@ -82,6 +82,18 @@ int join(filt_snapshot* s)
     return s->status;
 }
 
+// The previous write's failure, reported once: by whichever of take,
+// write_async, wait or destroy comes first after it.
+int report_previous(filt_snapshot* s, const char* what)
+{
+    const int rc = join(s);
+    if (rc == FILT_OK)
+        return FILT_OK;
+    s->status = FILT_OK;
+    return filt::set_error(rc, "%s: the snapshot's previous write failed: %s", what,
+                           s->err.c_str());
+}
+
 // Writer thread body: pinned copy of the snapshot, then the files.
 int write_files(filt_snapshot* s, const std::string& save_path, bool truncate,
                 const std::string& plot_dir)
@@ -171,7 +183,7 @@ int filt_snapshot_destroy(filt_snapshot* s)
 {
     if (!s)
         return FILT_OK;
-    const int rc = join(s);
+    const int rc = report_previous(s, "filt_snapshot_destroy");
     if (s->dbuf) {
         (void)hipStreamSynchronize(filt::stream());  // a take may still be copying
         (void)hipFree(s->dbuf);
@@ -191,7 +203,11 @@ int filt_snapshot_take(filt_snapshot* s, const filt_fields* dev, int j, double t
     using namespace filt;
     if (!s || !dev)
         return set_error(FILT_ERR_INVALID, "null argument");
-    join(s);  // the previous write reads the buffer until it ends
+    // the previous write reads the buffer until it ends; its failure is
+    // returned here rather than lost (the take is not made)
+    int prev = report_previous(s, "filt_snapshot_take");
+    if (prev)
+        return prev;
     for (int q = 0; q < NSLOT; q++) {
         if (!s->has[q])
             continue;
@@ -230,7 +246,9 @@ int filt_snapshot_write_async(filt_snapshot* s, const char* save_path, int trunc
             return set_error(FILT_ERR_INVALID, "snapshot made without FILT_SNAP_SAVE");
         if (plot_dir && !(s->what & FILT_SNAP_PLOTS))
             return set_error(FILT_ERR_INVALID, "snapshot made without FILT_SNAP_PLOTS");
-        join(s);
+        int prev = report_previous(s, "filt_snapshot_write_async");
+        if (prev)
+            return prev;
         const std::string sp = save_path ? save_path : "", pd = plot_dir ? plot_dir : "";
         const bool tr = truncate != 0;
         s->status = FILT_OK;
@@ -248,8 +266,7 @@ int filt_snapshot_wait(filt_snapshot* s)
 {
     if (!s)
         return filt::set_error(FILT_ERR_INVALID, "null snapshot");
-    const int rc = join(s);
-    return rc ? filt::set_error(rc, "snapshot writer: %s", s->err.c_str()) : FILT_OK;
+    return report_previous(s, "filt_snapshot_wait");
 }
 
 int filt_snapshot_restore(filt_snapshot* s, const filt_fields* dev, int* j)

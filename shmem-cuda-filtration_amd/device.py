@@ -124,8 +124,10 @@ class DeviceGrid:
             pass
 
 
-PREC_FP64 = 0
-PREC_MIXED = 1
+PREC_FP64 = abi.FILT_PREC_FP64    # the stage path's expressions, same bits
+PREC_FAST = abi.FILT_PREC_FAST    # all FP64, algebraic shortcuts
+PREC_MIXED = abi.FILT_PREC_MIXED  # FP32 fluxes and ro*S increment, FP64 Newton
+PRECISIONS = {"fp64": PREC_FP64, "fast": PREC_FAST, "mixed": PREC_MIXED}
 
 
 class Ensemble:
@@ -188,15 +190,31 @@ class Ensemble:
     def steps(self, mid):
         return self.lib.filt_ensemble_steps(self.h, mid)
 
-    def set_tiling(self, kz, order=0):
-        check(self.lib, self.lib.filt_ensemble_set_tiling(self.h, kz, order), "set_tiling")
+    def set_tiling(self, kz, order=0, tile=(0, 0)):
+        """tile = (tx, ty) columns per tile ((0, 0): 16x16), kz planes per
+        z-chunk, order 0 XCD-grouped / 1 hardware order."""
+        check(self.lib, self.lib.filt_ensemble_set_tiling(self.h, tile[0], tile[1], kz, order),
+              "set_tiling")
+
+    def tiling(self):
+        v = [ctypes.c_int() for _ in range(4)]
+        check(self.lib, self.lib.filt_ensemble_get_tiling(self.h, *[ctypes.byref(x) for x in v]),
+              "get_tiling")
+        return {"tile": [v[0].value, v[1].value], "kz": v[2].value, "order": v[3].value}
 
     def autotune(self, reps=3):
-        kz, order, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
-        check(self.lib, self.lib.filt_ensemble_autotune(self.h, reps, ctypes.byref(kz),
-                                                        ctypes.byref(order), ctypes.byref(ms)),
-              "filt_ensemble_autotune")
-        return kz.value, order.value, ms.value
+        """Search tile shape x kz x order; returns the chosen tiling with its
+        ms per step."""
+        tx, ty, kz, order = (ctypes.c_int() for _ in range(4))
+        ms = ctypes.c_double()
+        check(self.lib, self.lib.filt_ensemble_autotune(
+            self.h, reps, ctypes.byref(tx), ctypes.byref(ty), ctypes.byref(kz),
+            ctypes.byref(order), ctypes.byref(ms)), "filt_ensemble_autotune")
+        return {"tile": [tx.value, ty.value], "kz": kz.value, "order": order.value,
+                "ms_per_step": ms.value}
+
+    def apply_tiling(self, t):
+        self.set_tiling(t["kz"], t["order"], tuple(t["tile"]))
 
     def cells(self):
         return self.lib.filt_ensemble_cells(self.h)

@@ -93,8 +93,20 @@ int filt_serve_submit(filt_server* s, const filt_params* p, const double* P1, co
     return filt::guarded([&]() -> int {
         if (!s || !p || !P1 || !S2 || nsteps < 0)
             return FILT_ERR_INVALID;
-        if (p->NX < 3 || p->Ny < 3 || p->Nz < 3)
-            return FILT_ERR_UNSUPPORTED;
+        int rc = filt_ensemble_check(p, s->precision, perm, poro);
+        if (rc)
+            return rc;
+        // a request that can never fit would block the FIFO queue for good
+        const size_t need = filt_ensemble_member_bytes(p, s->precision, perm != nullptr,
+                                                       poro != nullptr, media != nullptr);
+        size_t free_b = 0, total_b = 0;
+        rc = filt_device_free_bytes(&free_b, &total_b);
+        if (rc)
+            return rc;
+        if (need + s->reserve > total_b)
+            return filt::set_error(FILT_ERR_NOMEM,
+                                   "request needs %zu device bytes plus a reserve of %zu; the "
+                                   "device has %zu", need, s->reserve, total_b);
         auto r = std::make_unique<Request>();
         r->id = s->next_id++;
         r->p = *p;
@@ -129,8 +141,14 @@ int filt_serve_round(filt_server* s, int* n_admitted, int* n_retired)
             int rc = filt_device_free_bytes(&free_b, &total_b);
             if (rc)
                 return rc;
-            if (free_b < need + s->reserve)
+            if (free_b < need + s->reserve) {
+                if (s->active.empty())  // nothing will retire and free memory
+                    return filt::set_error(FILT_ERR_NOMEM,
+                                           "request %d needs %zu device bytes plus a reserve of "
+                                           "%zu; %zu are free and no request is active",
+                                           r.id, need, s->reserve, free_b);
                 break;
+            }
             rc = filt_ensemble_add(s->ens, &r.p, r.P1.data(), r.S2.data(),
                                    r.perm.empty() ? nullptr : r.perm.data(),
                                    r.poro.empty() ? nullptr : r.poro.data(),

@@ -127,28 +127,31 @@ def c3_workload(nx=512, ny=512, nz=256, seed=3033):
 C4_EDGES = (48, 64, 96, 128, 160)
 
 
-def ensemble_member(rng, edges=C4_EDGES, corr=8.0, seed=0, shape=None, threads=0):
+def ensemble_member(rng, edges=C4_EDGES, corr=8.0, seed=0, shape=None, threads=0,
+                    clip=2.0, r_max=1.5):
     """One realization of the batched-ensemble / serving configurations: each
     axis drawn from `edges` (or the given shape), sigma ~ U[0.5, 2],
     correlation `corr` cells, porosity U[0.35, 0.45], a 4x4 source patch at a
-    random position on the top face.  ln K is clipped at +-2 sigma (with
+    random position on the top face, ln K clipped at +-`clip` sigma (with
     sigma = 2 on a 160-cell column the unclipped tail exceeds the stability
-    limit); the clip is tightened further if the guard still exceeds 1.5."""
+    limit).  The field keeps its stated sigma and clip; where the stability
+    guard still reads above r_max the member's time step is lowered to
+    dt * r_max / r (the guard is linear in dt), so every member is stable
+    and carries the variance its sigma states.  Returns the member with its
+    sigma, clip, guard value r (after any dt change) and dt."""
     if shape is None:
         shape = tuple(int(rng.choice(edges)) for _ in range(3))
     nx, ny, nz = shape
     p = base_params(nx, ny, nz, h=INI_H)
     set_patch(p, int(rng.integers(0, nx - 3)), int(rng.integers(0, nz - 3)))
     sigma = float(rng.uniform(0.5, 2.0))
-    clip = 2.0
-    while True:
-        perm, poro = gen_fields(p, seed, sigma=sigma, corr=corr, clip=clip, threads=threads)
+    perm, poro = gen_fields(p, seed, sigma=sigma, corr=corr, clip=clip, threads=threads)
+    r = stability(p, perm, poro)
+    if r > r_max:
+        p.dt = p.dt * r_max / r
         r = stability(p, perm, poro)
-        if r <= 1.5 or clip <= 0.5:
-            break
-        clip -= 0.25
     return {"params": p, "perm": perm, "poro": poro, "sigma": sigma, "clip": clip,
-            "stability_r": r, "shape": shape}
+            "stability_r": r, "dt": p.dt, "shape": shape}
 
 
 def c4_members(n=96, seed=4096):

@@ -2,11 +2,12 @@
 
 FP64 fused: same bits as the plain stage path on the GPU (same expressions),
 relative L-inf <= 1e-10 on P1 / S2 and 1e-12 on mass against the oracle.
-Mixed (the fast form: closed-form Newton, lambda = 2 powers as products,
-all FP64): errors against the
-oracle stated per field and held to S2 L-inf <= 1e-4, P1 relative L-inf
-<= 1e-6, and water, NAPL and total mass <= 1e-9 relative.  Batching: a
-member's result is the same alone and in a mix, bit for bit."""
+Fast (closed-form Newton, lambda = 2 powers as products, all FP64) and mixed
+(FP32 fluxes and ro*S increment, FP64 Newton): errors against the oracle
+stated per field and held to S2 L-inf <= 1e-4, P1 relative L-inf <= 1e-6,
+and for the fast form water, NAPL and total mass <= 1e-9 relative (the mixed
+form's mass bars are in its tests).  Every tile shape gives the same bits.
+Batching: a member's result is the same alone and in a mix, bit for bit."""
 import numpy as np
 import pytest
 
@@ -26,13 +27,17 @@ def plain_run(hip, p, g, steps):
         d.free()
 
 
-def fused_run(hip, p, g, steps, precision=0, graph=False, kz=None):
+FP64, FAST, MIXED = 0, 1, 2
+
+
+def fused_run(hip, p, g, steps, precision=0, graph=False, kz=None, tile=(0, 0)):
     e = hip.ensemble(precision)
     try:
         mid = e.add(p, g["P1"], g["S2"], g.a.get("perm"), g.a.get("poro"),
                     g.a["media"] if p.n_media > 1 else None)
-        if kz:
-            e.set_tiling(*kz)
+        if kz or tile != (0, 0):
+            kz_, order = kz or (32, 0)
+            e.set_tiling(kz_, order, tile)
         if graph:
             e.capture(steps)
             e.replay()
@@ -101,19 +106,57 @@ def test_fused_fp64_c1_trajectory_vs_oracle(hip, orc):
     assert abs(mf[1] - mo[1]) <= 1e-12 * mo[1]
 
 
+def errors_vs_oracle(P, S, m, g, mo):
+    """Per-field errors of a form's result against the oracle's state g
+    (mass mo): S2 L-inf, P1 relative L-inf, water / NAPL / total mass."""
+    return {"S2_linf": float(np.max(np.abs(S - g["S2"]))), "P1_rel_linf": max_rel(P, g["P1"]),
+            "water_mass_rel": abs(m[0] - mo[0]) / mo[0],
+            "napl_mass_rel": abs(m[1] - mo[1]) / mo[1] if mo[1] else abs(m[1]),
+            "total_mass_rel": abs(m[0] + m[1] - mo[0] - mo[1]) / (mo[0] + mo[1])}
+
+
+TILES = [(16, 16), (32, 8), (64, 4), (32, 16), (64, 8)]
+
+
+@pytest.mark.parametrize("precision", [FP64, FAST, MIXED])
+def test_every_tile_shape_gives_the_same_bits(hip, precision):
+    """The autotuner may pick any tile shape, z-chunk and block order: all
+    give the same bits (each cell's arithmetic does not depend on them)."""
+    p, g = c2_like(41)
+    P0, S0, m0 = fused_run(hip, p, g, 20, precision=precision)
+    for tile in TILES:
+        for kz in ((5, 0), (64, 1)):
+            P, S, m = fused_run(hip, p, g, 20, precision=precision, kz=kz, tile=tile)
+            assert np.array_equal(P, P0) and np.array_equal(S, S0), (tile, kz)
+            assert m == m0
+
+
 def test_mixed_c1_trajectory_error(hip, orc):
     """Mixed form against the FP64 oracle on configuration 1, 500 steps."""
     p = c1_params()
     g = HostGrid(p)
     orc.initial_data(g, p)
-    Pm, Sm, mm = fused_run(hip, p, g, 500, precision=1)
+    Pm, Sm, mm = fused_run(hip, p, g, 500, precision=MIXED)
+    orc.run(g, p, 500, with_u=0)
+    e = errors_vs_oracle(Pm, Sm, mm, g, orc.mass(g, p))
+    print("mixed C1 500 steps vs oracle:", e)
+    assert e["S2_linf"] <= 1e-4 and e["P1_rel_linf"] <= 1e-6
+    assert e["water_mass_rel"] <= 1e-9 and e["total_mass_rel"] <= 1e-9
+
+
+def test_fast_c1_trajectory_error(hip, orc):
+    """Fast form against the FP64 oracle on configuration 1, 500 steps."""
+    p = c1_params()
+    g = HostGrid(p)
+    orc.initial_data(g, p)
+    Pm, Sm, mm = fused_run(hip, p, g, 500, precision=FAST)
     orc.run(g, p, 500, with_u=0)
     eS = float(np.max(np.abs(Sm - g["S2"])))
     eP = max_rel(Pm, g["P1"])
     mo = orc.mass(g, p)
     e1, e2 = abs(mm[0] - mo[0]) / mo[0], abs(mm[1] - mo[1]) / mo[1]
     et = abs(mm[0] + mm[1] - mo[0] - mo[1]) / (mo[0] + mo[1])
-    print(f"mixed C1 500 steps: S2 Linf {eS:.3e}, P1 rel Linf {eP:.3e}, "
+    print(f"fast C1 500 steps: S2 Linf {eS:.3e}, P1 rel Linf {eP:.3e}, "
           f"water mass {e1:.3e}, NAPL mass {e2:.3e}, total mass {et:.3e}")
     assert eS <= 1e-4
     assert eP <= 1e-6
@@ -121,9 +164,10 @@ def test_mixed_c1_trajectory_error(hip, orc):
     assert e2 <= 1e-9
 
 
-def test_mixed_heterogeneous_error(hip, orc):
+@pytest.mark.parametrize("precision", [FAST, MIXED])
+def test_heterogeneous_error(hip, orc, precision):
     p, g = c2_like(48, seed=9, sigma=1.5)
-    Pm, Sm, mm = fused_run(hip, p, g, 100, precision=1)
+    Pm, Sm, mm = fused_run(hip, p, g, 100, precision=precision)
     orc.set_threads(8)
     orc.run(g, p, 100, with_u=0)
     orc.set_threads(1)
@@ -131,13 +175,17 @@ def test_mixed_heterogeneous_error(hip, orc):
     mo = orc.mass(g, p)
     e1, e2 = abs(mm[0] - mo[0]) / mo[0], abs(mm[1] - mo[1]) / mo[1]
     et = abs(mm[0] + mm[1] - mo[0] - mo[1]) / (mo[0] + mo[1])
-    print(f"mixed 48^3 sigma 1.5, 100 steps: S2 Linf {eS:.3e}, P1 rel {max_rel(Pm, g['P1']):.3e},"
-          f" water mass {e1:.3e}, NAPL mass {e2:.3e}, total mass {et:.3e}")
-    assert eS <= 1e-4 and e1 <= 1e-9 and et <= 1e-9 and e2 <= 1e-9
+    print(f"precision {precision} 48^3 sigma 1.5, 100 steps: S2 Linf {eS:.3e}, "
+          f"P1 rel {max_rel(Pm, g['P1']):.3e}, water mass {e1:.3e}, NAPL mass {e2:.3e}, "
+          f"total mass {et:.3e}")
+    assert eS <= 1e-4 and e1 <= 1e-9 and et <= 1e-9
+    if precision == FAST:
+        assert e2 <= 1e-9
 
 
-def test_mixed_reference_media_trajectory_error(hip, orc):
-    """The fast form on the reference's own media table (lambda 2.7 / 2.0,
+@pytest.mark.parametrize("precision", [FAST, MIXED])
+def test_reference_media_trajectory_error(hip, orc, precision):
+    """The fast and mixed forms on the reference's own media table (lambda 2.7 / 2.0,
     P_d 755 / 2060: the general Brooks-Corey powers and the Newton with
     capillary terms) on a random 0/1 media map, 100 steps, against the FP64
     oracle, to the same bars (FP32 powers missed the NAPL one: 4.6e-8)."""
@@ -145,21 +193,23 @@ def test_mixed_reference_media_trajectory_error(hip, orc):
     g = HostGrid(p)
     orc.initial_data(g, p)
     g.a["media"][...] = np.random.default_rng(3).integers(0, 2, g["P1"].shape)
-    Pm, Sm, mm = fused_run(hip, p, g, 100, precision=1)
+    Pm, Sm, mm = fused_run(hip, p, g, 100, precision=precision)
     orc.run(g, p, 100, with_u=0)
     eS = float(np.max(np.abs(Sm - g["S2"])))
     eP = max_rel(Pm, g["P1"])
     mo = orc.mass(g, p)
     e1, e2 = abs(mm[0] - mo[0]) / mo[0], abs(mm[1] - mo[1]) / mo[1]
     et = abs(mm[0] + mm[1] - mo[0] - mo[1]) / (mo[0] + mo[1])
-    print(f"mixed reference media 100 steps: S2 Linf {eS:.3e}, P1 rel Linf {eP:.3e}, "
-          f"water mass {e1:.3e}, NAPL mass {e2:.3e}, total mass {et:.3e}")
+    print(f"precision {precision} reference media 100 steps: S2 Linf {eS:.3e}, "
+          f"P1 rel Linf {eP:.3e}, water mass {e1:.3e}, NAPL mass {e2:.3e}, total mass {et:.3e}")
     assert mo[1] > 0
     assert eS <= 1e-4 and eP <= 1e-6
-    assert e1 <= 1e-9 and e2 <= 1e-9 and et <= 1e-9
+    assert e1 <= 1e-9 and et <= 1e-9
+    if precision == FAST:
+        assert e2 <= 1e-9
 
 
-@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("precision", [FP64, FAST, MIXED])
 def test_member_same_alone_and_in_mix(hip, precision):
     import importlib
     W = importlib.import_module("shmem-cuda-filtration_amd").workloads
@@ -187,7 +237,7 @@ def test_member_same_alone_and_in_mix(hip, precision):
         e.close()
 
 
-@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("precision", [FP64, FAST, MIXED])
 def test_lambda2_member_same_beside_reference_media_member(hip, orc, precision):
     """A batch that holds a member with lambda != 2 or P_d != 0 runs the
     kernel with the general Brooks-Corey powers; a lambda = 2, P_d = 0 member
@@ -224,8 +274,8 @@ def test_continuous_batching_add_remove_between_steps(hip):
         p = W.base_params(n, n + 3, n - 2, h=W.INI_H)
         perm, poro = W.gen_fields(p, 300 + q, sigma=1.0, corr=4.0, clip=2.5)
         grids.append((p, W.initial_grid(p, perm, poro)))
-    alone = [fused_run(hip, p, g, 15, precision=1) for p, g in grids]
-    e = hip.ensemble(1)
+    alone = [fused_run(hip, p, g, 15, precision=MIXED) for p, g in grids]
+    e = hip.ensemble(MIXED)
     try:
         a = e.add(grids[0][0], grids[0][1]["P1"], grids[0][1]["S2"], grids[0][1]["perm"],
                   grids[0][1]["poro"])

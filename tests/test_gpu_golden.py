@@ -6,7 +6,7 @@ the boundary known-answer test bit for bit; stages that call pow (ro_P2_Xi,
 the Newton with P_d != 0, which every fixture has) to 1e-13 relative per
 element; trajectories to relative L-inf 1e-10 on the arrays the fixtures hold
 after step 1 and at the end.  The fused forms run the 3D trajectories: FP64
-to 1e-10, mixed precision to its bars (S2 within 1e-4, P1 within 1e-6).  The
+to 1e-10, the fast and mixed forms to their bars (S2 within 1e-4, P1 within 1e-6).  The
 initial state of a trajectory comes from the oracle's initial_data, which
 test_golden pins bit for bit."""
 import numpy as np
@@ -106,11 +106,11 @@ def test_trajectory_on_gpu(hip, orc, name):
     assert checked >= 4
 
 
-@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("precision", [0, 1, 2])
 @pytest.mark.parametrize("name", ["traj_15cube_s3", "traj_16cube_two_media"])
 def test_fused_forms_trajectory_against_golden(hip, orc, name, precision):
     """The fused one-launch step on the 3D golden trajectories: FP64 to the
-    trajectory bar (1e-10 relative L-inf), mixed precision to its own bars
+    trajectory bar (1e-10 relative L-inf), the fast and mixed forms to their own bars
     (S2 within 1e-4, P1 within 1e-6 relative)."""
     from test_gpu_fused import fused_run
     d, meta = load(name)

@@ -168,3 +168,35 @@ def test_snapshot_reuse_and_kind_checks(hip, pkg, tmp_path):
     finally:
         s.close()
         d.free()
+
+
+def test_failed_snapshot_write_is_reported_by_the_next_take(hip, pkg, tmp_path):
+    """A write_async that fails (its folder cannot be created: a file stands
+    in the way) is not lost when the caller takes the next snapshot before
+    waiting: the take returns the write's FILT_ERR_IO once, and later calls
+    succeed again."""
+    p = ref_params(9, 8, 7)
+    d = started(hip, p, 2)
+    s = pkg.device.Snapshot(d, what=pkg.abi.FILT_SNAP_SAVE)
+    blocker = tmp_path / "not_a_folder"
+    blocker.write_bytes(b"x")
+    try:
+        s.take(d, 2)
+        s.write_async(blocker / "save" / "save.dat")
+        with pytest.raises(pkg.abi.FiltError) as err:
+            s.take(d, 3)
+        assert err.value.code == pkg.abi.FILT_ERR_IO
+        assert "previous write failed" in str(err.value)
+        s.take(d, 3)  # reported once; the snapshot works again
+        s.write_async(tmp_path / "ok.dat")
+        s.wait()
+        d.save(tmp_path / "sync.dat", j=3)
+        assert (tmp_path / "ok.dat").read_bytes() == (tmp_path / "sync.dat").read_bytes()
+        # the same failure reported by wait, then nothing left to report
+        s.write_async(blocker / "again.dat")
+        with pytest.raises(pkg.abi.FiltError):
+            s.wait()
+        s.wait()
+    finally:
+        s.close()
+        d.free()

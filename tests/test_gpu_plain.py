@@ -622,7 +622,7 @@ def test_fused_step_equals_stage_calls(hip, params, dims, with_u):
 def test_divby_equals_ieee_division():
     """The fused step's division by a grid spacing (DivBy, filt_device.hpp)
     against n / d, bit for bit, on 2^24 numerators x 64 divisors."""
-    exe = os.path.join(ROOT, PKG_DIR, "lib", "divby_check")
+    exe = os.path.join(ROOT, "tests", "hip", "bin", "divby_check")
     assert os.path.exists(exe), "build the package first (make -C shmem-cuda-filtration_amd)"
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     line = r.stdout.strip().splitlines()[-1]

@@ -1,6 +1,6 @@
 """The continuous-batching server (filt_serve_*) on the GPU: requests that
 arrive while others run are admitted and batched, and each result equals the
-same request run alone, bit for bit, in both precisions."""
+same request run alone, bit for bit, in every precision form."""
 import importlib
 
 import numpy as np
@@ -24,7 +24,7 @@ def alone(hip, precision, req, nsteps):
         e.close()
 
 
-@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("precision", [0, 1, 2])
 def test_results_same_alone_and_in_the_mix(hip, precision):
     stream = serving.c5_stream(rate=40.0, duration=0.4, seed=7, edge_lo=16, edge_hi=40,
                                steps_lo=10, steps_hi=60)
@@ -59,5 +59,29 @@ def test_admission_respects_slots_and_queue_order(hip):
             P, S, m = srv.take(i)
             P0, S0, m0 = alone(hip, 1, reqs[q], 5 + q)
             assert np.array_equal(P, P0) and np.array_equal(S, S0) and m == m0
+    finally:
+        srv.close()
+
+
+def test_request_that_can_never_fit_is_refused_at_submit(hip):
+    """A request larger than the device (with the reserve) would block the
+    FIFO queue for good: submit refuses it with FILT_ERR_NOMEM.  A request
+    the stability guard refuses is refused at submit too."""
+    srv = serving.Server(hip, 1, reserve_bytes=1 << 50, max_active=2)
+    try:
+        req = serving.make_request((12, 10, 11), 7)
+        with pytest.raises(pkg.abi.FiltError) as err:
+            srv.submit(*req, 5)
+        assert err.value.code == pkg.abi.FILT_ERR_NOMEM
+    finally:
+        srv.close()
+    srv = serving.Server(hip, 1)
+    try:
+        p, P1, S2, perm, poro = serving.make_request((12, 10, 11), 8)
+        p.dt *= 1e6  # far past the explicit scheme's limit
+        with pytest.raises(pkg.abi.FiltError) as err:
+            srv.submit(p, P1, S2, perm, poro, 5)
+        assert err.value.code == pkg.abi.FILT_ERR_UNSTABLE
+        assert srv.status()["queued"] == 0
     finally:
         srv.close()

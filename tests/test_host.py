@@ -187,3 +187,38 @@ def test_host_allocation_failure_is_a_status_not_an_abort(pkg):
                                 0.0, 1)
     assert rc == pkg.abi.FILT_ERR_NOMEM
     assert b"memory" in lib.filt_last_error()
+
+
+def test_ensemble_check_refusals_without_a_device(pkg):
+    """filt_ensemble_check (what filt_ensemble_add and filt_serve_submit
+    refuse) runs on the host alone."""
+    import ctypes
+    W = pkg.workloads
+    lib = pkg.abi.load_library()
+    p = W.base_params(20, 16, 12, h=W.INI_H)
+    chk = lambda q, prec=1, perm=None: lib.filt_ensemble_check(
+        ctypes.byref(q), prec, None if perm is None else perm.ctypes.data, None)
+    for prec in (0, 1, 2):
+        assert chk(p, prec) == pkg.abi.FILT_OK
+    assert chk(p, 3) == pkg.abi.FILT_ERR_INVALID
+    q = p.copy()
+    q.Nz = 2
+    assert chk(q) == pkg.abi.FILT_ERR_UNSUPPORTED
+    q = p.copy()
+    q.roS_scheme = pkg.abi.FILT_SCHEME_DIVGRAD
+    assert chk(q) == pkg.abi.FILT_ERR_UNSUPPORTED
+    q = p.copy()
+    q.NX, q.Ny, q.Nz = 1024, 1024, 512  # 4 GiB per FP64 array
+    assert chk(q) == pkg.abi.FILT_ERR_UNSUPPORTED
+    q = p.copy()
+    q.mu2 = 1e31  # outside the FP64 form's exact-division range (an immobile NAPL)
+    assert chk(q, 0) == pkg.abi.FILT_ERR_UNSUPPORTED
+    assert chk(q, 1) == pkg.abi.FILT_OK
+    q = p.copy()
+    q.dt *= 1e5
+    assert chk(q) == pkg.abi.FILT_ERR_UNSTABLE
+    import numpy as np
+    perm = np.full((12, 16, 20), W.K_MEAN)
+    assert chk(p, 1, perm) == pkg.abi.FILT_OK
+    perm[3, 4, 5] = W.K_MEAN * 1e6  # one cell far too permeable for dt
+    assert chk(p, 1, perm) == pkg.abi.FILT_ERR_UNSTABLE
