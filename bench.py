@@ -3,20 +3,29 @@
 Default workload (the configuration the headline metric is quoted on that
 fits one GPU): the plain FP64 path on a 256^3 grid with per-cell log-normal
 permeability (sigma 1, correlation length 8 cells) and porosity U[0.35, 0.45],
-200 steps of time_step_function as filt_time_step runs it: stage 1, stages
-2-4 (u, roS, Newton) in one z-march, the boundary conditions, every array of
+200 steps of time_step_function as filt_time_step runs it, every array of
 ptr_Arrays materialised with the bits the five stages give.  Inputs are
-resident in HBM before the timed region.  The five defines.h stage functions
-called one by one (the reference driver's calls) are reported beside it.
+resident in HBM before the timed region.
 
-Prints one JSON line (rank 0):
-  value      cell-updates/s of the whole job (Nx*Ny*Nz*steps / wall time)
-  roofline   the dominant kernel's algorithmic bytes per launch / its mean
-             duration (HIP events around every launch inside the timed region)
-             against MI355X's 8 TB/s; traffic from rocprofv3 PMC counters when
-             a summary for this workload is committed under profiles/
-  cpu_baseline  the reference CPU path (oracle/_ref, compiled from the
-             reference sources) or the oracle port, on a bounded sample
+Beside the headline the JSON line carries the other configurations:
+  stage_calls / defines_shims  the five defines.h functions called one by one,
+                 and the reference driver's calls through the defines.h shims;
+  forms          the fused FP64, fast and mixed forms on the same grid;
+  c2_oracle      the plain path and every form against the oracle port run
+                 on all host threads this process may use (cell for cell);
+  slab_*         one rank's share (512 x 512 x 256 and its transpose): plain
+                 vs fused FP64 / fast / mixed, un-batched; the 512 x 512 x 256
+                 slab also against the oracle with its cumulative mass-balance
+                 error every 10 steps;
+  batched_ensemble  96 uneven realizations, 300 steps, every form batched in
+                 one graph against the plain path run realization by
+                 realization;
+  serving        Poisson arrivals at 20 requests/s for 60 s through the
+                 continuous-batching server, FP64 vs mixed;
+  cpu_baseline   the reference's own CPU path (oracle/_ref) on one thread on
+                 a bounded sample of the headline workload, and on
+                 configuration 1 in full; the oracle port on all threads.
+Progress goes to stderr; the one JSON line to stdout.
 """
 import argparse
 import ctypes
@@ -33,6 +42,7 @@ PKG = importlib.import_module("shmem-cuda-filtration_amd")
 
 HBM_PEAK = 8.0e12
 PROFILES = os.path.join(ROOT, "profiles")
+T0 = time.perf_counter()
 
 # Algorithmic bytes per cell of each plain-path kernel for this workload
 # (one medium: the media array is not read; per-cell perm / poro are):
@@ -50,25 +60,16 @@ STEP_STAGES = ("ro_P2_Xi", "u_roS_newton", "boundary")     # time_step_function
 SPLIT_STAGES = ("ro_P2_Xi", "u", "roS", "newton", "boundary")  # the five stage calls
 KERNEL_NAMES = {"ro_P2_Xi": "k_ro_P2_Xi", "u": "k_u_zm", "roS": "k_roS_zm", "newton": "k_newton",
                 "u_roS_newton": "k_step_zm", "boundary": "k_bc_a+k_bc_b"}
+FORMS = {0: "fused_fp64", 1: "fast", 2: "mixed"}
+# P, S in + out (32) + perm, poro in FP64 (16), every form
+FUSED_BYTES = {0: 48, 1: 48, 2: 48}
 
 
-def cpu_baseline(args):
-    """Reference CPU path on a bounded sample: two time steps of a
-    256 x 256 x 128 slab of the same grid (the reference's rate is the same
-    per cell at 256^3; its i-outer / k-inner loop order strides whole
-    xy-planes), one thread, about 15-20 s.  Beside it, the oracle port
-    (loop-interchanged, OpenMP on up to 16 host threads, bitwise equal to the
-    reference) on the same sample."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle
+def log(msg):
+    print(f"[bench {time.perf_counter() - T0:7.1f}s] {msg}", file=sys.stderr, flush=True)
 
-    nx, ny, nz = args.n, args.n, max(3, args.n // 2)
-    p = PKG.abi.default_params_py()
-    p.h1, p.h2, p.h3 = PKG.workloads.INI_H
-    p.dt, p.beta1, p.beta2 = 1e-4, 1e-4, 1e-8
-    p.NX, p.Ny, p.Nz, p.source = nx, ny, nz, 2
-    PKG.abi.finalize_patch_py(p)
-    steps = 2
+
+def host_info():
     cpu = platform.processor() or platform.machine()
     try:
         with open("/proc/cpuinfo") as fh:
@@ -78,36 +79,47 @@ def cpu_baseline(args):
                     break
     except OSError:
         pass
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = os.cpu_count() or 1
+    return {"cpu_model": cpu, "nproc": os.cpu_count(), "affinity_cpus": aff,
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
 
-    def timed(impl):
-        g = PKG.grid.HostGrid(p)
-        impl.initial_data(g, p)
-        t0 = time.perf_counter()
-        impl.run(g, p, steps)
-        return time.perf_counter() - t0, g
 
-    if oracle.ref_available():
-        impl, kind = oracle.RefCPU(), "reference"
-    else:
-        impl, kind = oracle.Oracle(), "port"
-        impl.set_loop_order(1)
-    dt, g_ref = timed(impl)
-    out = {"value": nx * ny * nz * steps / dt, "unit": "cell-updates/s", "cores": 1,
-           "kind": kind,
-           "sample": f"{steps} steps of a {nx}x{ny}x{nz} slab (media table, same h/dt/beta), "
-                     f"{dt:.1f} s on 1 thread of {cpu}"}
-    port = oracle.Oracle()
-    threads = max(1, min(16, os.cpu_count() or 1))
-    port.set_threads(threads)
-    port.set_loop_order(0)
-    dtp, g_port = timed(port)
+def host_threads():
+    """Threads for the CPU legs: the CPUs this process may use, at most 16
+    (the host share of one GPU)."""
+    return max(1, min(16, host_info()["affinity_cpus"]))
+
+
+def form_errors(P, S, m, Pr, Sr, mr, reference):
+    """Per-field errors of a result (P, S, masses m) against a reference
+    state: S2 L-inf, P1 and S2 relative L-inf, water / NAPL / total mass."""
     import numpy as np
-    out["best_effort_port"] = {
-        "value": nx * ny * nz * steps / dtp, "cores": threads,
-        "sample": f"same sample, the oracle port with x-fastest loops and OpenMP, {dtp:.2f} s",
-        "bitwise_equal_to_reference": bool(np.array_equal(g_port["P1"], g_ref["P1"]) and
-                                           np.array_equal(g_port["S2"], g_ref["S2"]))}
-    return out
+    return {
+        "S2_linf": float(np.max(np.abs(S - Sr))),
+        "S2_rel_linf": float(np.max(np.abs(S - Sr)) / np.max(np.abs(Sr))),
+        "P1_rel_linf": float(np.max(np.abs(P - Pr)) / np.max(np.abs(Pr))),
+        "water_mass_rel": abs(m[0] - mr[0]) / mr[0],
+        "napl_mass_rel": abs(m[1] - mr[1]) / mr[1] if mr[1] > 0 else abs(m[1]),
+        "total_mass_rel": abs(m[0] + m[1] - mr[0] - mr[1]) / (mr[0] + mr[1]),
+        "reference": reference,
+    }
+
+
+def mixed_errors(P64, S64, m64, Pm, Sm, mm):
+    """A form's errors against the fused FP64 form after the same steps."""
+    return form_errors(Pm, Sm, mm, P64, S64, m64,
+                       "fused FP64 (bit-identical to the plain path) after the same steps")
+
+
+def mass_series_errors(series, ref_series, every):
+    return [{"step": every * (q + 1),
+             "water": abs(m[0] - r[0]) / r[0],
+             "napl": abs(m[1] - r[1]) / r[1] if r[1] else abs(m[1]),
+             "total": abs(m[0] + m[1] - r[0] - r[1]) / (r[0] + r[1])}
+            for q, (m, r) in enumerate(zip(series, ref_series))]
 
 
 def load_traffic(workload_tag, kernel):
@@ -118,55 +130,15 @@ def load_traffic(workload_tag, kernel):
         return json.load(fh).get("bytes_per_launch")
 
 
-def mixed_errors(P64, S64, m64, Pm, Sm, mm):
-    """Mixed-form errors against the FP64 form after the same steps."""
-    import numpy as np
-    return {
-        "S2_linf": float(np.max(np.abs(Sm - S64))),
-        "P1_rel_linf": float(np.max(np.abs(Pm - P64)) / np.max(np.abs(P64))),
-        "water_mass_rel": abs(mm[0] - m64[0]) / m64[0],
-        "napl_mass_rel": abs(mm[1] - m64[1]) / m64[1] if m64[1] > 0 else 0.0,
-        "total_mass_rel": abs(mm[0] + mm[1] - m64[0] - m64[1]) / (m64[0] + m64[1]),
-        "reference": "fused FP64 (bit-identical to the plain path) after the same steps",
-    }
-
-
-def run_slab(args, nx, ny, nz):
-    """One large grid, un-batched: a 512 x 512 x 256 slab (sigma 1.5,
-    correlation 16 cells, ln K clipped at +-2 sigma), args.slab_steps steps
-    through the plain five-stage path, the fused FP64 form and the mixed
-    form, so that the fusion and fast-form gain shows apart from batching."""
-    import types
-    W = PKG.workloads
-    t0 = time.perf_counter()
-    p, perm, poro = W.c3_workload(nx, ny, nz)
-    gen_s = time.perf_counter() - t0
-    sub = types.SimpleNamespace(steps=args.slab_steps, warmup=3, autotune=False)
-    p, wall, stage_ms, r, finite, tuned = run_plain(sub, p, perm, poro)
-    cells = nx * ny * nz
-    plain = cells * sub.steps / wall
-    out = {"grid": [nx, ny, nz], "steps": sub.steps, "field_generation_s": gen_s,
-           "stability_r": r, "state_finite": finite,
-           "plain": {"value": plain, "ms_per_step": wall * 1e3 / sub.steps,
-                     "stages_ms": stage_ms, "stencil_tiles": tuned}}
-    f64, (P64, S64, m64) = run_fused(sub, p, perm, poro, 0)
-    mix, (Pm, Sm, mm) = run_fused(sub, p, perm, poro, 1)
-    mix["error_vs_fp64"] = mixed_errors(P64, S64, m64, Pm, Sm, mm)
-    out["fused_fp64"] = f64
-    out["mixed"] = mix
-    out["speedup_fused_fp64_vs_plain"] = f64["value"] / plain
-    out["speedup_mixed_vs_plain"] = mix["value"] / plain
-    return out
-
+# ---------------------------------------------------------------- plain --
 
 def run_plain(args, p, perm, poro, split=False, steps=None):
-    """The plain FP64 path on one grid, inputs resident.  Default: the time
-    step as filt_time_step runs it (stage 1, stages 2-4 in one z-march, the
-    boundary conditions), three launches with HIP events between them on the
-    library stream.  split=True: the five defines.h stage functions called
+    """The plain FP64 path on one grid, inputs resident, timed.  Default: the
+    time step as filt_time_step runs it (stage 1, stages 2-4 in one z-march,
+    the boundary conditions), three launches with HIP events between them on
+    the library stream.  split=True: the five defines.h stage functions called
     one after another, as the reference's own driver calls them."""
-    from importlib import import_module
-    hiprt = import_module(PKG.__name__ + ".hiprt")
+    hiprt = importlib.import_module(PKG.__name__ + ".hiprt")
     be = PKG.device.Backend()
     r = PKG.workloads.stability(p, perm, poro)
     if r > 2.0:
@@ -228,7 +200,28 @@ def run_plain(args, p, perm, poro, split=False, steps=None):
     d.free()
     import numpy as np
     finite = bool(np.isfinite(P1).all() and np.isfinite(S2).all())
-    return p, wall, stage_ms, r, finite, tuned
+    return {"wall": wall, "steps": K, "stage_ms": stage_ms, "stability_r": r, "finite": finite,
+            "tiles": tuned}
+
+
+def plain_state(p, perm, poro, steps, every=None):
+    """The plain path's state after exactly `steps` steps from the initial
+    state (u included), with its device mass; with `every`, the masses after
+    every `every` steps."""
+    be = PKG.device.Backend()
+    d = be.grid(p, perm=True, poro=True)
+    try:
+        d.upload(PKG.workloads.initial_grid(p, perm, poro))
+        series, done = [], 0
+        while done < steps:
+            k = min(every or steps, steps - done)
+            d.run(k, step0=done, with_u=1)
+            done += k
+            if every:
+                series.append(d.mass())
+        return d.get("P1"), d.get("S2"), d.mass(), series
+    finally:
+        d.free()
 
 
 def run_shim(args, p, perm, poro, steps, fuse):
@@ -239,9 +232,8 @@ def run_shim(args, p, perm, poro, steps, fuse):
     otherwise FILT_FUSE_STAGES=0 and five launches.  The defines.h interface
     has no per-cell K or porosity: the same initial state runs on the first
     medium of the table, so the kernels read the media array instead."""
-    from importlib import import_module
     import numpy as np
-    shim_mod = import_module(PKG.__name__ + ".defines_shim")
+    shim_mod = importlib.import_module(PKG.__name__ + ".defines_shim")
     shim = shim_mod.Shim()
     PKG.device.Backend()
     old = os.environ.get("FILT_FUSE_STAGES")
@@ -281,8 +273,7 @@ def run_shim(args, p, perm, poro, steps, fuse):
             "state_finite": bool(np.isfinite(P1).all() and np.isfinite(S2).all())}
 
 
-FUSED_BYTES = {0: 48, 1: 48, 2: 48}  # P, S in + out (32) + perm, poro in FP64 (16)
-
+# ---------------------------------------------------------------- fused --
 
 def time_ensemble(e, steps, hiprt, lib):
     """Capture `steps` fused steps as one graph, replay it once untimed and
@@ -301,40 +292,121 @@ def time_ensemble(e, steps, hiprt, lib):
     return a.elapsed_ms(b) / steps, wall
 
 
-def run_fused(args, p, perm, poro, precision):
-    """The same workload through the fused one-launch-per-step form."""
+def run_fused(args, p, perm, poro, precision, steps=None, every=None):
+    """One grid through a fused one-launch-per-step form: the autotuner picks
+    the tiling, a graph of `steps` steps is timed; then an accuracy run of
+    exactly `steps` steps from the initial state (masses every `every`
+    steps when given)."""
     hiprt = importlib.import_module(PKG.__name__ + ".hiprt")
     be = PKG.device.Backend()
+    steps = steps or args.steps
     g = PKG.workloads.initial_grid(p, perm, poro)
     e = be.ensemble(precision)
     try:
         e.add(p, g["P1"], g["S2"], perm, poro)
         tiling = e.autotune(3)
-        ms, wall = time_ensemble(e, args.steps, hiprt, be.lib)
+        ms, wall = time_ensemble(e, steps, hiprt, be.lib)
     finally:
         e.close()
-    # accuracy run: exactly args.steps steps from the initial state
     e = be.ensemble(precision)
+    series = []
     try:
         mid = e.add(p, g["P1"], g["S2"], perm, poro)
         e.apply_tiling(tiling)
-        e.step(args.steps)
+        done = 0
+        while done < steps:
+            k = min(every or steps, steps - done)
+            e.step(k)
+            done += k
+            if every:
+                series.append(e.mass(mid))
         P, S = e.get(mid)
         m = e.mass(mid)
     finally:
         e.close()
     cells = p.NX * p.Ny * p.Nz
-    return {"value": cells / (ms * 1e-3), "ms_per_step": ms, "tiling": tiling,
+    info = {"value": cells / (ms * 1e-3), "ms_per_step": ms, "tiling": tiling,
             "bytes_per_cell_update": FUSED_BYTES[precision],
             "hbm_GBps": FUSED_BYTES[precision] * cells / (ms * 1e-3) / 1e9,
-            "wall_s": wall}, (P, S, m)
+            "hbm_frac": FUSED_BYTES[precision] * cells / (ms * 1e-3) / HBM_PEAK}
+    return info, (P, S, m, series)
 
+
+def oracle_run(p, perm, poro, steps, every=None, with_u=1):
+    """The oracle port (loop-interchanged, OpenMP) on all host threads this
+    process may use: (state, seconds, masses every `every` steps)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    orc = oracle.Oracle()
+    orc.set_threads(host_threads())
+    orc.set_loop_order(0)
+    g = PKG.workloads.initial_grid(p, perm, poro)
+    series, done, secs = [], 0, 0.0
+    while done < steps:
+        k = min(every or steps, steps - done)
+        t0 = time.perf_counter()
+        orc.run(g, p, k, step0=done, with_u=with_u)
+        secs += time.perf_counter() - t0
+        done += k
+        if every:
+            series.append(orc.mass(g, p))
+            log(f"oracle {p.NX}x{p.Ny}x{p.Nz}: step {done}/{steps}")
+    return g, secs, orc.mass(g, p), series
+
+
+def run_slab(args, nx, ny, nz, with_oracle):
+    """One large grid, un-batched: one rank's share (sigma 1.5, correlation
+    16 cells, ln K clipped at +-2 sigma), args.slab_steps steps through the
+    plain path and the fused FP64, fast and mixed forms, so that the fusion
+    and precision gains show apart from batching.  with_oracle: every form
+    against the oracle, final per-field errors and the cumulative
+    mass-balance error every 10 steps."""
+    import types
+    W = PKG.workloads
+    t0 = time.perf_counter()
+    p, perm, poro = W.c3_workload(nx, ny, nz)
+    gen_s = time.perf_counter() - t0
+    steps = args.slab_steps
+    sub = types.SimpleNamespace(steps=steps, warmup=3, autotune=False)
+    pr = run_plain(sub, p, perm, poro)
+    cells = nx * ny * nz
+    plain = cells * steps / pr["wall"]
+    out = {"grid": [nx, ny, nz], "steps": steps, "field_generation_s": gen_s,
+           "stability_r": pr["stability_r"], "state_finite": pr["finite"],
+           "plain": {"value": plain, "ms_per_step": pr["wall"] * 1e3 / steps,
+                     "stages_ms": pr["stage_ms"], "stencil_tiles": pr["tiles"]}}
+    every = 10 if with_oracle else None
+    states = {"plain": plain_state(p, perm, poro, steps, every)}
+    for prec, name in FORMS.items():
+        info, st = run_fused(sub, p, perm, poro, prec, steps, every)
+        out[name] = info
+        out[f"speedup_{name}_vs_plain"] = info["value"] / plain
+        states[name] = st
+    if with_oracle:
+        log(f"slab {nx}x{ny}x{nz}: oracle, {steps} steps on {host_threads()} threads")
+        ref, secs, mref, rseries = oracle_run(p, perm, poro, steps, every, with_u=0)
+        out["oracle"] = {"seconds": secs, "threads": host_threads(),
+                         "cell_updates_per_s": cells * steps / secs}
+        for name, (P, S, m, series) in states.items():
+            tgt = out[name] if name in out else out["plain"]
+            tgt["error_vs_oracle"] = form_errors(P, S, m, ref["P1"], ref["S2"], mref,
+                                                 f"oracle after {steps} steps")
+            tgt["mass_balance_error_every_10_steps"] = mass_series_errors(series, rseries, 10)
+    else:
+        P64, S64, m64, _ = states["fused_fp64"]
+        for name in ("fast", "mixed"):
+            P, S, m, _ = states[name]
+            out[name]["error_vs_fp64"] = mixed_errors(P64, S64, m64, P, S, m)
+    return out
+
+
+# ---------------------------------------------------------- ensembles --
 
 def run_c4(args):
     """Batched ensemble: args.c4 realizations with uneven grids, args.c4_steps
-    steps.  Plain FP64 path run realization by realization (all 5 stages,
-    all arrays, default tiles) against the batched forms (every member in one
-    launch per step, the whole run one hipGraph)."""
+    steps.  Plain FP64 path run realization by realization (all stages, all
+    arrays, default tiles) against the batched forms (every member in one
+    launch per step, the whole run one hipGraph, tiling from the autotuner)."""
     import numpy as np
     hiprt = importlib.import_module(PKG.__name__ + ".hiprt")
     be = PKG.device.Backend()
@@ -346,11 +418,20 @@ def run_c4(args):
     gen_s = time.perf_counter() - t0
     cells = sum(m["params"].NX * m["params"].Ny * m["params"].Nz for m in members)
     steps = args.c4_steps
+    base_dt = W.base_params(4, 4, 4).dt
+    lowered = [m for m in members if m["dt"] < base_dt]
     out = {"realizations": len(members), "cells": cells, "steps": steps,
            "field_generation_s": gen_s,
            "sigma_range": [min(m["sigma"] for m in members), max(m["sigma"] for m in members)],
-           "max_stability_r": max(m["stability_r"] for m in members)}
-    # plain, one realization after another (inputs resident first)
+           "lnK_clip_sigma": sorted({m["clip"] for m in members}),
+           "max_stability_r": max(m["stability_r"] for m in members),
+           "dt_lowered_for_stability": {"members": len(lowered),
+                                        "min_dt": min((m["dt"] for m in members)),
+                                        "sigmas": [round(m["sigma"], 3) for m in lowered]},
+           "members": [{"shape": list(m["shape"]), "sigma": round(m["sigma"], 4),
+                        "clip": m["clip"], "dt": m["dt"], "r": round(m["stability_r"], 4)}
+                       for m in members]}
+    log(f"C4: {len(members)} members generated ({gen_s:.1f} s); plain per realization")
     grids = []
     for m in members:
         g = W.initial_grid(m["params"], m["perm"], m["poro"])
@@ -372,12 +453,13 @@ def run_c4(args):
     out["plain_per_realization"] = {"value": cells * steps / (plain_ms * 1e-3),
                                     "total_ms": plain_ms}
     results = {}
-    for prec, name in ((0, "batched_fp64"), (1, "batched_fast"), (2, "batched_mixed")):
+    for prec, form in FORMS.items():
+        name = "batched_" + form.replace("fused_", "")
+        log(f"C4: {name}")
         e = be.ensemble(prec)
-        ids = []
         for m in members:
             g = W.initial_grid(m["params"], m["perm"], m["poro"])
-            ids.append(e.add(m["params"], g["P1"], g["S2"], m["perm"], m["poro"]))
+            e.add(m["params"], g["P1"], g["S2"], m["perm"], m["poro"])
         tiling = e.autotune(2)
         e.close()
         # fresh ensemble: the timed run starts from the initial states
@@ -399,22 +481,34 @@ def run_c4(args):
                      "hbm_GBps": FUSED_BYTES[prec] * cells * steps / (ms * 1e-3) / 1e9}
         results[prec] = [(e.get(mid), e.mass(mid)) for mid in ids]
         e.close()
-    errs = {"S2_linf": 0.0, "P1_rel_linf": 0.0, "water_mass_rel": 0.0, "napl_mass_rel": 0.0,
-            "total_mass_rel": 0.0}
-    for ((P64, S64), m64), ((Pm, Sm), mm) in zip(results[0], results[1]):
-        errs["S2_linf"] = max(errs["S2_linf"], float(np.max(np.abs(Sm - S64))))
-        errs["P1_rel_linf"] = max(errs["P1_rel_linf"],
-                                  float(np.max(np.abs(Pm - P64)) / np.max(np.abs(P64))))
-        errs["water_mass_rel"] = max(errs["water_mass_rel"], abs(mm[0] - m64[0]) / m64[0])
-        if m64[1] > 0:
-            errs["napl_mass_rel"] = max(errs["napl_mass_rel"], abs(mm[1] - m64[1]) / m64[1])
-        errs["total_mass_rel"] = max(errs["total_mass_rel"],
-                                     abs(sum(mm) - sum(m64)) / sum(m64))
-    out["batched_mixed"]["error_vs_fp64_max_over_members"] = errs
-    out["speedup_batched_mixed_vs_plain"] = (out["batched_mixed"]["value"] /
-                                             out["plain_per_realization"]["value"])
-    out["speedup_batched_fp64_vs_plain"] = (out["batched_fp64"]["value"] /
-                                            out["plain_per_realization"]["value"])
+        out[f"speedup_{name}_vs_plain"] = out[name]["value"] / out["plain_per_realization"]["value"]
+    keys = ("S2_linf", "P1_rel_linf", "water_mass_rel", "napl_mass_rel", "total_mass_rel")
+    for prec in (1, 2):
+        errs = {k: 0.0 for k in keys}
+        for ((P64, S64), m64), ((Pm, Sm), mm) in zip(results[0], results[prec]):
+            e = mixed_errors(P64, S64, m64, Pm, Sm, mm)
+            for k in keys:
+                errs[k] = max(errs[k], e[k])
+        out["batched_" + FORMS[prec]]["error_vs_fp64_max_over_members"] = errs
+    # the three smallest members against the oracle (every form)
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        orc = oracle.Oracle()
+        orc.set_threads(host_threads())
+        small = sorted(range(len(members)), key=lambda q: out["members"][q]["shape"][0] *
+                       out["members"][q]["shape"][1] * out["members"][q]["shape"][2])[:3]
+        vs = {}
+        for q in small:
+            m = members[q]
+            ref = W.initial_grid(m["params"], m["perm"], m["poro"])
+            orc.run(ref, m["params"], steps, with_u=0)
+            mr = orc.mass(ref, m["params"])
+            vs[str(q)] = {FORMS[prec]: form_errors(results[prec][q][0][0], results[prec][q][0][1],
+                                                   results[prec][q][1], ref["P1"], ref["S2"], mr,
+                                                   f"oracle after {steps} steps")
+                          for prec in FORMS}
+        out["smallest_members_vs_oracle"] = vs
     return out
 
 
@@ -422,8 +516,8 @@ def run_serving(args):
     """Serving configuration: Poisson arrivals (rate args.serve_rate per s for
     args.serve_duration s), edges log-uniform in [32, 192], steps U[100, 2000],
     continuous batching through filt_serve_*.  FP64 (the fused form, whose
-    results equal the plain path's bit for bit) against mixed precision on the
-    same request stream."""
+    results equal the plain path's bit for bit) against the mixed form on the
+    same request stream; the fast form too with --serve-fast."""
     serving = importlib.import_module(PKG.__name__ + ".serving")
     be = PKG.device.Backend()
     stream = serving.c5_stream(rate=args.serve_rate, duration=args.serve_duration)
@@ -431,17 +525,93 @@ def run_serving(args):
            "requests": len(stream),
            "offered_cell_updates_per_s": sum(s[1][0] * s[1][1] * s[1][2] * s[2] for s in stream)
            / args.serve_duration}
-    for prec, name in ((0, "fp64"), (1, "mixed")):
+    legs = [(0, "fp64"), (2, "mixed")] + ([(1, "fast")] if args.serve_fast else [])
+    for prec, name in legs:
+        log(f"serving: {name}, {len(stream)} requests over {args.serve_duration:.0f} s")
         stats, _ = serving.run(be, prec, stream)
         out[name] = stats
     return out
 
 
+# ----------------------------------------------------------------- CPU --
+
+def cpu_baseline(args, c2_port=None):
+    """The reference's own CPU path (oracle/_ref, compiled from its sources;
+    i-outer / k-inner loops, one thread, as its 1-rank CPU build runs):
+      value: two time steps of the headline 256^3 grid (its rate per cell
+             does not depend on the step count), about 20 s;
+      c1:    configuration 1 in full (32^3, 500 steps).
+    The compiled reference cannot express per-cell fields, a 4 x 4 patch or
+    Sn = 0.5: its samples use the configurations' grid, spacing, time step,
+    fluids and Brooks-Corey values through its media table, and its own
+    5 x 5 patch with Sn = 0.4 (the cost per cell is the same).  Beside it the
+    oracle port (loop-interchanged, OpenMP) on all threads this process may
+    use: on configuration 1 in full (bitwise equal to the reference there)
+    and on the headline workload in full (c2_port, from the oracle check)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle
+    W = PKG.workloads
+    info = host_info()
+
+    def ref_expressible(nx, ny, nz, h):
+        p = PKG.abi.default_params_py()
+        p.h1, p.h2, p.h3 = h
+        p.dt, p.beta1, p.beta2 = 1e-4, W.STABLE_BETA[0], W.STABLE_BETA[1]
+        p.NX, p.Ny, p.Nz, p.source = nx, ny, nz, 2
+        for q in range(2):
+            p.K[q], p.lambda_[q], p.S_wr[q], p.m[q], p.P_d[q] = W.K_MEAN, 2.0, 0.1, 0.4, 0.0
+        return PKG.abi.finalize_patch_py(p)
+
+    def timed(impl, p, steps):
+        g = PKG.grid.HostGrid(p)
+        impl.initial_data(g, p)
+        t0 = time.perf_counter()
+        impl.run(g, p, steps)
+        return time.perf_counter() - t0, g
+
+    have_ref = oracle.ref_available()
+    if have_ref:
+        impl, kind = oracle.RefCPU(), "reference"
+    else:
+        impl, kind = oracle.Oracle(), "port"
+        impl.set_loop_order(1)
+    n = args.n
+    p2 = ref_expressible(n, n, n, W.INI_H)
+    log(f"cpu_baseline: {kind} on 2 steps of {n}^3, 1 thread")
+    dt2, _ = timed(impl, p2, 2)
+    p1 = ref_expressible(32, 32, 32, (1 / 31, 1 / 31, 1 / 31))
+    log(f"cpu_baseline: {kind} on configuration 1 (32^3, 500 steps), 1 thread")
+    dt1, g_ref1 = timed(impl, p1, 500)
+    port = oracle.Oracle()
+    port.set_threads(host_threads())
+    port.set_loop_order(0)
+    dtp1, g_port1 = timed(port, p1, 500)
+    out = {"value": n ** 3 * 2 / dt2, "unit": "cell-updates/s", "cores": 1, "kind": kind,
+           "sample": f"2 steps of the {n}^3 grid (media table with the configuration's "
+                     f"fluids, reference 5x5 patch), {dt2:.1f} s on 1 thread of "
+                     f"{info['cpu_model']}; the reference's rate per cell does not depend on "
+                     f"the step count",
+           "host": info,
+           "c1_full": {"value": 32 ** 3 * 500 / dt1, "cores": 1, "kind": kind,
+                       "sample": f"configuration 1 in full: 32^3, 500 steps, {dt1:.1f} s "
+                                 "(reference 5x5 patch, Sn 0.4)"},
+           "best_effort_port": {
+               "c1_full": {"value": 32 ** 3 * 500 / dtp1, "cores": host_threads(),
+                           "seconds": dtp1,
+                           "bitwise_equal_to_reference": bool(
+                               np.array_equal(g_port1["P1"], g_ref1["P1"]) and
+                               np.array_equal(g_port1["S2"], g_ref1["S2"]))},
+               "c2_full": c2_port}}
+    return out
+
+
 def stream_mix_ceiling(kernel, cells):
     """Time of a pure stream with the kernel's mix of FP64 arrays
-    (tests/hip/bin/stream_mix, built from tests/hip/stream_mix.hip): the fastest any kernel reading and
-    writing those arrays once can run on this GPU.  Best of plain and
-    non-temporal stores; None when the tool is missing or has no such mix."""
+    (tests/hip/bin/stream_mix, built from tests/hip/stream_mix.hip): the
+    fastest any kernel reading and writing those arrays once can run on this
+    GPU.  Best of plain and non-temporal stores; None when the tool is
+    missing or has no such mix."""
     import subprocess
     exe = os.path.join(ROOT, "tests", "hip", "bin", "stream_mix")
     if not os.path.exists(exe):
@@ -490,17 +660,20 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=256, help="grid edge")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-cpu", action="store_true",
+                    help="skip the CPU legs: the baseline and the oracle checks")
     ap.add_argument("--c4", type=int, default=96,
                     help="realizations of the batched-ensemble leg (0: skip)")
     ap.add_argument("--c4-steps", type=int, default=300)
     ap.add_argument("--slab-steps", type=int, default=100,
                     help="steps of the 512x512x256 and 256x512x512 slab legs (0: skip)")
-    ap.add_argument("--serve-duration", type=float, default=20.0,
+    ap.add_argument("--serve-duration", type=float, default=60.0,
                     help="arrival window of the serving leg in seconds (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-fast", action="store_true",
+                    help="also serve the request stream with the fast form")
     ap.add_argument("--plain-only", action="store_true",
-                    help="skip the fused FP64 / mixed forms and the defines.h shim legs")
+                    help="only the headline plain path (the profiled workload)")
     ap.add_argument("--autotune", action="store_true",
                     help="autotune the stencil tile shapes of the plain path on this grid "
                          "(default: the library's built-in shapes)")
@@ -516,25 +689,52 @@ def main():
         return
 
     p, perm, poro = PKG.workloads.c2_workload(args.n)
-    p, wall, stage_ms, r, finite, tuned = run_plain(args, p, perm, poro)
-    forms = {}
+    log(f"plain {args.n}^3: {args.steps} timed steps")
+    pr = run_plain(args, p, perm, poro)
+    wall, stage_ms = pr["wall"], pr["stage_ms"]
+    cells = p.NX * p.Ny * p.Nz
+    value = cells * args.steps / wall
+    forms, states, split, shims, c2_oracle, c2_port = {}, {}, None, None, None, None
     if not args.plain_only:
-        import numpy as np
-        f64, (P64, S64, m64) = run_fused(args, p, perm, poro, 0)
-        fast, (Pf, Sf, mf) = run_fused(args, p, perm, poro, 1)
-        fast["error_vs_fp64"] = mixed_errors(P64, S64, m64, Pf, Sf, mf)
-        mix, (Pm, Sm, mm) = run_fused(args, p, perm, poro, 2)
-        mix["error_vs_fp64"] = mixed_errors(P64, S64, m64, Pm, Sm, mm)
-        forms = {"fused_fp64": f64, "fast": fast, "mixed": mix}
+        states["plain"] = plain_state(p, perm, poro, args.steps)
+        for prec, name in FORMS.items():
+            log(f"{name} {args.n}^3")
+            info, st = run_fused(args, p, perm, poro, prec)
+            forms[name] = info
+            states[name] = st
+        P64, S64, m64, _ = states["fused_fp64"]
+        for name in ("fast", "mixed"):
+            P, S, m, _ = states[name]
+            forms[name]["error_vs_fp64"] = mixed_errors(P64, S64, m64, P, S, m)
+        # the five stage functions one by one (the reference driver's calls)
+        log("stage calls and defines.h shims")
+        k5 = max(1, args.steps // 2)
+        r5 = run_plain(args, p, perm, poro, split=True, steps=k5)
+        split = {"value": cells * k5 / r5["wall"], "ms_per_step": r5["wall"] * 1e3 / k5,
+                 "stages_ms": r5["stage_ms"], "state_finite": r5["finite"],
+                 "bytes_per_cell_update": sum(PLAIN_BYTES[s] for s in SPLIT_STAGES),
+                 "hbm_frac": sum(PLAIN_BYTES[s] for s in SPLIT_STAGES) * cells * k5 /
+                 r5["wall"] / HBM_PEAK}
+        shims = {"fused": run_shim(args, p, perm, poro, k5, True),
+                 "separate": run_shim(args, p, perm, poro, k5, False)}
+        if not args.no_cpu:
+            log(f"oracle port on {args.n}^3, {args.steps} steps, {host_threads()} threads")
+            ref, secs, mref, _ = oracle_run(p, perm, poro, args.steps)
+            c2_port = {"value": cells * args.steps / secs, "cores": host_threads(),
+                       "seconds": secs, "sample": f"the headline workload in full: {args.n}^3, "
+                                                  f"{args.steps} steps, per-cell K and porosity"}
+            c2_oracle = {name: form_errors(P, S, m, ref["P1"], ref["S2"], mref,
+                                           f"oracle port after {args.steps} steps")
+                         for name, (P, S, m, _) in states.items()}
         if args.slab_steps > 0:
-            for shape in ((512, 512, 256), (256, 512, 512)):
-                forms["slab_%dx%dx%d" % shape] = run_slab(args, *shape)
+            for shape, with_oracle in (((512, 512, 256), not args.no_cpu),
+                                       ((256, 512, 512), False)):
+                log("slab %dx%dx%d" % shape)
+                forms["slab_%dx%dx%d" % shape] = run_slab(args, *shape, with_oracle)
         if args.c4 > 0:
             forms["batched_ensemble"] = run_c4(args)
         if args.serve_duration > 0:
             forms["serving"] = run_serving(args)
-    cells = p.NX * p.Ny * p.Nz
-    value = cells * args.steps / wall
     dom = max((s for s in STEP_STAGES if PLAIN_BYTES[s]), key=lambda s: stage_ms[s])
     dom_bytes = PLAIN_BYTES[dom] * cells
     achieved = dom_bytes / (stage_ms[dom] * 1e-3)
@@ -545,22 +745,6 @@ def main():
         mix_ceiling["kernel_frac_of_mix"] = mix_ceiling["ms"] / stage_ms[dom]
     step_bytes = sum(PLAIN_BYTES[s] for s in STEP_STAGES) * cells
     split_bytes = sum(PLAIN_BYTES[s] for s in SPLIT_STAGES) * cells
-    # the five stage functions one by one (the reference driver's calls)
-    _, wall5, stage5, _, finite5, _ = run_plain(args, p, perm, poro, split=True,
-                                                  steps=max(1, args.steps // 2))
-    split = {"value": cells * max(1, args.steps // 2) / wall5,
-             "ms_per_step": wall5 * 1e3 / max(1, args.steps // 2),
-             "stages_ms": stage5, "state_finite": finite5,
-             "bytes_per_cell_update": sum(PLAIN_BYTES[s] for s in SPLIT_STAGES),
-             "hbm_frac": split_bytes * max(1, args.steps // 2) / wall5 / HBM_PEAK}
-    # the same calls through the defines.h shims, as the reference driver makes
-    # them (another workload: media table instead of per-cell K and porosity,
-    # so not under --plain-only, whose launches the kernel profiles average)
-    shim_steps = max(1, args.steps // 2)
-    shims = None
-    if not args.plain_only:
-        shims = {"fused": run_shim(args, p, perm, poro, shim_steps, True),
-                 "separate": run_shim(args, p, perm, poro, shim_steps, False)}
     line = {
         "metric": "cell-updates/s",
         "value": value,
@@ -581,10 +765,10 @@ def main():
             "grid": [p.NX, p.Ny, p.Nz],
             "form": "plain",
             "deviations": "beta1=1e-4, beta2=1e-8 (stated 4.4e-10 is unstable); "
-                          "defines.ini spacing",
-            "stability_r": r,
-            "stencil_tiles": tuned,
-            "state_finite": finite,
+                          "defines.ini spacing; ln K clipped at +-2.5 sigma",
+            "stability_r": pr["stability_r"],
+            "stencil_tiles": pr["tiles"],
+            "state_finite": pr["finite"],
         },
         "roofline": {
             "bound": "hbm",
@@ -602,15 +786,15 @@ def main():
             "bytes_per_cell_update": sum(PLAIN_BYTES[s] for s in STEP_STAGES),
             "achieved_GBps": step_bytes * args.steps / wall / 1e9,
             "frac": step_bytes * args.steps / wall / HBM_PEAK,
-            "vs_split_model": {"bytes_per_cell_update": sum(PLAIN_BYTES[s] for s in SPLIT_STAGES),
-                               "frac": split_bytes * args.steps / wall / HBM_PEAK},
         },
         "stage_calls": split,
         "defines_shims": shims,
+        "c2_oracle": c2_oracle,
         "forms": forms,
     }
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(args)
+        line["cpu_baseline"] = cpu_baseline(args, c2_port)
+    log("done")
     print(json.dumps(line), flush=True)
 
 

@@ -248,6 +248,15 @@ int filt_boundary_conditions(const filt_fields* dev, const filt_params* p,
 int filt_u_roS_P1_S2_calculation(const filt_fields* dev, const filt_params* p,
                                  double t, int localNx, int rank, int size, int with_u);
 
+/* Total water and NAPL mass of a device grid, FP64:
+ * mass2[0] = sum phi*ro1(P1)*(1-S2)*V, mass2[1] = sum phi*ro2(P1)*S2*V over
+ * the interior cells (cpu.cpp:220), V = h1*h2*h3, densities as stage 1
+ * computes them (cpu.cpp:72-73), phi from poro or the medium.  No reference
+ * counterpart: the mass-balance accumulator a driver reads step by step.
+ * Deterministic (a fixed summation tree).  Waits for the library stream and
+ * writes mass2 (host memory) before returning. */
+int filt_mass(const filt_fields* dev, const filt_params* p, int localNx, double* mass2);
+
 /* Number of launches of the one-kernel stages 2-4 (k_step_zm) since the
  * library was loaded, from any entry point.  A diagnostic with no
  * counterpart in the reference: it lets a caller see which path its calls

@@ -148,6 +148,9 @@ struct FormFP64 {
     static constexpr int NQ = 5;
     enum { QP = 0, QX1 = 1, QX2 = 2, QR1 = 3, QR2 = 4 };
     static constexpr int SLOT_BYTES = NQ * 8;
+    static constexpr bool TOP64 = false;
+    template <int SN, int TX>
+    static constexpr int tile_bytes() { return SN * SLOT_BYTES; }
     static constexpr int WAVES = 4;  // waves per SIMD the kernel is built for
     static constexpr bool PIPELINED = false;
 
@@ -189,9 +192,10 @@ struct FormFP64 {
 
     // roS_calculation (cpu.cpp:218-291) then the Newton (cpu.cpp:293-320)
     // of an interior cell, expression for expression as k_step_zm.
-    template <int SN, int SX>
+    template <int SN, int SX, int TX>
     __device__ static void update(const Ctx& c, const Val& zc, const char* Tm, const Val& zp,
-                                  const char* T, int s, const Raw& q, double& nP, double& nS)
+                                  const char* T, int s, const Raw& q, bool, int, double& nP,
+                                  double& nS)
     {
         const double* L = (const double*)T;
         const double* M = (const double*)Tm;
@@ -359,6 +363,9 @@ struct FormFast {
         double P, L1, L2;
     };
     static constexpr int SLOT_BYTES = 24;
+    static constexpr bool TOP64 = false;
+    template <int SN, int TX>
+    static constexpr int tile_bytes() { return SN * SLOT_BYTES; }
     static constexpr int WAVES = GEN ? 3 : 4;
     static constexpr bool PIPELINED = true;
 
@@ -379,9 +386,10 @@ struct FormFast {
         d[2 * SN + s] = v.L2;
     }
 
-    template <int SN, int SX>
+    template <int SN, int SX, int TX>
     __device__ static void update(const Ctx& c, const Val& zc, const char* Tm, const Val& zp,
-                                  const char* T, int s, const Raw& q, double& nP, double& nS)
+                                  const char* T, int s, const Raw& q, bool, int, double& nP,
+                                  double& nS)
     {
         const double* L = (const double*)T;
         const double* M = (const double*)Tm;
@@ -421,6 +429,17 @@ struct FormFast {
 // ro(P)(1-S), the factor dt/phi and its product with the FP32 flux sum (a
 // per-cell rounding of dt/phi would bias every increment of the cell alike),
 // the Newton and the mass sums.  LDS slot: P (FP64), L1, L2 (FP32).
+//
+// One face keeps FP64 fluxes: the face between the top boundary row j = 0
+// and the first interior row j = 1.  The top boundary holds a fixed state
+// (P_atm, and S2_gr on the source patch), so the FP32 rounding of its
+// mobility would be the same relative error in the inflow on every step, a
+// bias that accumulates in the NAPL mass (measured 1e-8 relative on the
+// configurations; with this face in FP64 the NAPL mass stays within 1e-11 of
+// the all-FP64 result).  Interior x and z face fluxes telescope exactly in
+// FP32 as well (both neighbours form each face flux from the same FP32
+// operands), so their rounding moves no mass.  The j = 0 row's FP64
+// mobilities are kept in an extra LDS row of the top tiles.
 template <bool GEN>
 struct FormMixed {
     struct Val {
@@ -430,6 +449,21 @@ struct FormMixed {
     static constexpr int SLOT_BYTES = 16;
     static constexpr int WAVES = GEN ? 3 : 4;
     static constexpr bool PIPELINED = true;
+    static constexpr bool TOP64 = true;
+    // per tile: the slots, then FP64 L1 and L2 of the j = 0 ring row
+    template <int SN, int TX>
+    static constexpr int tile_bytes() { return SN * SLOT_BYTES + TX * 16; }
+
+    // FP64 mobilities of the top boundary cell at tile column x
+    template <int SN, int TX>
+    __device__ static void put_top(char* T, int x, const Ctx& c, const Raw& q)
+    {
+        double* t = (double*)(T + SN * SLOT_BYTES);
+        double L1, L2;
+        mobilities<GEN>(c, q, L1, L2);
+        t[x] = L1;
+        t[TX + x] = L2;
+    }
 
     __device__ static Val eval(const Ctx& c, const Raw& q)
     {
@@ -448,9 +482,10 @@ struct FormMixed {
         f[SN + s] = v.L2;
     }
 
-    template <int SN, int SX>
+    template <int SN, int SX, int TX>
     __device__ static void update(const Ctx& c, const Val& zc, const char* Tm, const Val& zp,
-                                  const char* T, int s, const Raw& q, double& nP, double& nS)
+                                  const char* T, int s, const Raw& q, bool jtop, int tx,
+                                  double& nP, double& nS)
     {
         const double* LP = (const double*)T;
         const double* MP = (const double*)Tm;
@@ -469,18 +504,34 @@ struct FormMixed {
         const float ax2 = (float)((p1 - LP[xp]) * ihx), ax1 = (float)((LP[xm] - p1) * ihx);
         const float az2 = (float)((p1 - zp.P) * ihz), az1 = (float)((MP[s] - p1) * ihz);
         const double dy2 = (p1 - LP[yp]) * ihy, dy1 = (LP[ym] - p1) * ihy;
-        const float ay2w = (float)(dy2 + r1c * g), ay1w = (float)(dy1 + r1c * g);
-        const float ay2n = (float)(dy2 + r2c * g), ay1n = (float)(dy1 + r2c * g);
+        const float ay2w = (float)(dy2 + r1c * g), ay2n = (float)(dy2 + r2c * g);
         const float fx = (float)ihx, fy = (float)ihy, fzz = (float)ihz;
-        const float F1 = uflux(ax2, ax1, zc.L1, L[xm], L[xp]) * fx +
-                         uflux(ay2w, ay1w, zc.L1, L[ym], L[yp]) * fy +
+        const float X1 = uflux(ax2, ax1, zc.L1, L[xm], L[xp]) * fx +
                          uflux(az2, az1, zc.L1, M[s], zp.L1) * fzz;
-        const float F2 = uflux(ax2, ax1, zc.L2, L[SN + xm], L[SN + xp]) * fx +
-                         uflux(ay2n, ay1n, zc.L2, L[SN + ym], L[SN + yp]) * fy +
+        const float X2 = uflux(ax2, ax1, zc.L2, L[SN + xm], L[SN + xp]) * fx +
                          uflux(az2, az1, zc.L2, M[SN + s], zp.L2) * fzz;
+        double F1, F2;
+        if (!jtop) {
+            const float ay1w = (float)(dy1 + r1c * g), ay1n = (float)(dy1 + r2c * g);
+            F1 = (double)(X1 + uflux(ay2w, ay1w, zc.L1, L[ym], L[yp]) * fy);
+            F2 = (double)(X2 + uflux(ay2n, ay1n, zc.L2, L[SN + ym], L[SN + yp]) * fy);
+        } else {
+            // + face in FP32; - face (to the top boundary) in FP64
+            const float z = 0.f;
+            const float pl1 = fmaxf(ay2w, z) * zc.L1 + fminf(ay2w, z) * L[yp];
+            const float pl2 = fmaxf(ay2n, z) * zc.L2 + fminf(ay2n, z) * L[SN + yp];
+            double L1c, L2c;
+            mobilities<GEN>(c, q, L1c, L2c);
+            const double* top = (const double*)(T + SN * SLOT_BYTES);
+            const double a1w = dy1 + r1c * g, a1n = dy1 + r2c * g;
+            const double mi1 = fmin(a1w, 0.) * L1c + fmax(a1w, 0.) * top[tx];
+            const double mi2 = fmin(a1n, 0.) * L2c + fmax(a1n, 0.) * top[TX + tx];
+            F1 = (double)X1 + ((double)pl1 - mi1) * ihy;
+            F2 = (double)X2 + ((double)pl2 - mi2) * ihy;
+        }
         const double dtphi = dt_over_phi(c, q);
-        const double A1 = r1c * (1 - S) - dtphi * (double)F1;
-        const double A2 = r2c * S - dtphi * (double)F2;
+        const double A1 = r1c * (1 - S) - dtphi * F1;
+        const double A2 = r2c * S - dtphi * F2;
         newton_fast(c, q.md, p1, S, A1, A2, nP, nS);
     }
 };
@@ -616,7 +667,10 @@ __device__ __forceinline__ void march(const Ctx& c, const FusedGrid& G, unsigned
         Pt0 = ro1_of(P10, Gc.p) * Gc.p.g_const * Gc.p.h2;
     }
     char* const T0 = lds;
-    char* const T1 = lds + SN * F::SLOT_BYTES;
+    char* const T1 = lds + F::template tile_bytes<SN, TX>();
+    // the top tiles' ring cells of row j = 0, and the cells of row j = 1
+    const bool top_ring = ring && ry == -1 && j0 == 1;
+    const bool jtop = j == 1;
     using Val = typename F::Val;
     Val zc{}, zp{};
     if constexpr (!F::PIPELINED) {
@@ -637,14 +691,19 @@ __device__ __forceinline__ void march(const Ctx& c, const FusedGrid& G, unsigned
                 qp = load_raw(c, col + (k + 1) * c.sz(), true);  // k + 1 <= ke <= nz - 1
                 F::template put<SN>(L, s, c, zc);
             }
-            if (ring)
-                F::template put<SN>(L, rs, c, F::eval(c, load_raw(c, rcol + k * c.sz(), false)));
+            if (ring) {
+                const Raw rr = load_raw(c, rcol + k * c.sz(), false);
+                F::template put<SN>(L, rs, c, F::eval(c, rr));
+                if constexpr (F::TOP64)
+                    if (top_ring)
+                        F::template put_top<SN, TX>(L, rx, c, rr);
+            }
             if (in_grid)
                 zp = F::eval(c, qp);
             __syncthreads();
             if (inner) {
                 double nP, nS;
-                F::template update<SN, SX>(c, zc, Lm, zp, L, s, qc, nP, nS);
+                F::template update<SN, SX, TX>(c, zc, Lm, zp, L, s, qc, jtop, tx, nP, nS);
                 write_cell(c, col + k * c.sz(), i1, lastcol, top, Pt0, k == 1, k == nz - 2, nP,
                            nS);
             }
@@ -671,8 +730,12 @@ __device__ __forceinline__ void march(const Ctx& c, const FusedGrid& G, unsigned
             zp = F::eval(c, qp);
             F::template put<SN>(L, s, c, zc);
         }
-        if (ring)
+        if (ring) {
             F::template put<SN>(L, rs, c, F::eval(c, rq));
+            if constexpr (F::TOP64)
+                if (top_ring)
+                    F::template put_top<SN, TX>(L, rx, c, rq);
+        }
         if (k + 1 < ke) {  // the next plane is updated: fetch its inputs
             if (in_grid)
                 qn = load_raw(c, col + (k + 2) * c.sz(), true);  // k + 2 <= ke <= nz - 1
@@ -682,7 +745,7 @@ __device__ __forceinline__ void march(const Ctx& c, const FusedGrid& G, unsigned
         __syncthreads();
         if (inner) {
             double nP, nS;
-            F::template update<SN, SX>(c, zc, Lm, zp, L, s, qc, nP, nS);
+            F::template update<SN, SX, TX>(c, zc, Lm, zp, L, s, qc, jtop, tx, nP, nS);
             write_cell(c, col + k * c.sz(), i1, lastcol, top, Pt0, k == 1, k == nz - 2, nP, nS);
         }
         zc = zp;
@@ -698,7 +761,7 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(F::WAVES
 {
     constexpr int NT = TX * TY;
     constexpr int SN = (TX + 2) * (TY + 2);
-    __shared__ __attribute__((aligned(16))) char lds[2 * SN * F::SLOT_BYTES];
+    __shared__ __attribute__((aligned(16))) char lds[2 * F::template tile_bytes<SN, TX>()];
     __shared__ double med[M_ROWS * FILT_MAX_MEDIA];
     __shared__ double rh[5];
     long long b = blockIdx.x;

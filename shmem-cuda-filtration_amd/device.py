@@ -90,6 +90,13 @@ class DeviceGrid:
     def sync(self):
         check(self.lib, self.lib.filt_synchronize(), "filt_synchronize")
 
+    def mass(self):
+        """(water, NAPL) mass over the interior cells, summed on the device."""
+        out = (ctypes.c_double * 2)()
+        check(self.lib, self.lib.filt_mass(ctypes.byref(self.f), ctypes.byref(self.params),
+                                           self.nx, out), "filt_mass")
+        return out[0], out[1]
+
     # -- checkpoint / resume, plots (save, restore, save_data_plots) --------
     def save(self, path, j, truncate=True, rank=0, size=1):
         check(self.lib, self.lib.filt_save(str(path).encode(), ctypes.byref(self.f),

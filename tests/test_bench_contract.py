@@ -22,7 +22,8 @@ def load_bench():
 def test_helpers_and_tables():
     b = load_bench()
     for name in ("cpu_baseline", "load_traffic", "run_plain", "run_fused", "run_c4",
-                 "run_serving", "run_slab", "copy_calibration", "mixed_errors", "main"):
+                 "run_serving", "run_slab", "copy_calibration", "mixed_errors", "form_errors",
+                 "oracle_run", "plain_state", "host_info", "main"):
         assert callable(getattr(b, name)), name
     for s in b.STEP_STAGES + b.SPLIT_STAGES:
         assert s in b.PLAIN_BYTES and s in b.KERNEL_NAMES, s

@@ -5,8 +5,7 @@ relative L-inf <= 1e-10 on P1 / S2 and 1e-12 on mass against the oracle.
 Fast (closed-form Newton, lambda = 2 powers as products, all FP64) and mixed
 (FP32 fluxes and ro*S increment, FP64 Newton): errors against the oracle
 stated per field and held to S2 L-inf <= 1e-4, P1 relative L-inf <= 1e-6,
-and for the fast form water, NAPL and total mass <= 1e-9 relative (the mixed
-form's mass bars are in its tests).  Every tile shape gives the same bits.
+and water, NAPL and total mass <= 1e-9 relative.  Every tile shape gives the same bits.
 Batching: a member's result is the same alone and in a mix, bit for bit."""
 import numpy as np
 import pytest
@@ -142,6 +141,7 @@ def test_mixed_c1_trajectory_error(hip, orc):
     print("mixed C1 500 steps vs oracle:", e)
     assert e["S2_linf"] <= 1e-4 and e["P1_rel_linf"] <= 1e-6
     assert e["water_mass_rel"] <= 1e-9 and e["total_mass_rel"] <= 1e-9
+    assert e["napl_mass_rel"] <= 1e-9
 
 
 def test_fast_c1_trajectory_error(hip, orc):
@@ -178,9 +178,7 @@ def test_heterogeneous_error(hip, orc, precision):
     print(f"precision {precision} 48^3 sigma 1.5, 100 steps: S2 Linf {eS:.3e}, "
           f"P1 rel {max_rel(Pm, g['P1']):.3e}, water mass {e1:.3e}, NAPL mass {e2:.3e}, "
           f"total mass {et:.3e}")
-    assert eS <= 1e-4 and e1 <= 1e-9 and et <= 1e-9
-    if precision == FAST:
-        assert e2 <= 1e-9
+    assert eS <= 1e-4 and e1 <= 1e-9 and et <= 1e-9 and e2 <= 1e-9
 
 
 @pytest.mark.parametrize("precision", [FAST, MIXED])
@@ -204,9 +202,7 @@ def test_reference_media_trajectory_error(hip, orc, precision):
           f"P1 rel Linf {eP:.3e}, water mass {e1:.3e}, NAPL mass {e2:.3e}, total mass {et:.3e}")
     assert mo[1] > 0
     assert eS <= 1e-4 and eP <= 1e-6
-    assert e1 <= 1e-9 and et <= 1e-9
-    if precision == FAST:
-        assert e2 <= 1e-9
+    assert e1 <= 1e-9 and e2 <= 1e-9 and et <= 1e-9
 
 
 @pytest.mark.parametrize("precision", [FP64, FAST, MIXED])

@@ -667,3 +667,24 @@ def test_fused_step_every_tile_config(hip, tile, kz, order):
         fused_vs_staged(hip, p, g, 2, 1)
     finally:
         lib.filt_set_tile_config(2, 0, 0, 0, 0)
+
+
+@pytest.mark.parametrize("media,poro", [(False, False), (True, False), (True, True)])
+@pytest.mark.parametrize("dims", [(17, 9, 11), (5, 5, 1), (40, 33, 70)])
+def test_device_mass_equals_oracle_mass(hip, orc, media, poro, dims):
+    """filt_mass (the device's mass-balance accumulator) against the
+    oracle's sum over the same random state: interior cells only, porosity
+    per cell or per medium; the two summation orders differ by rounding."""
+    p = ref_params(*dims)
+    g = random_state(p, np.random.default_rng(sum(dims)), media=media, poro=poro)
+    if not media:
+        p.n_media = 1
+    d = hip.grid(p, poro=poro)
+    try:
+        d.upload(g)
+        m = d.mass()
+        assert d.mass() == m  # deterministic
+    finally:
+        d.free()
+    mo = orc.mass(g, p)
+    assert abs(m[0] - mo[0]) <= 1e-13 * abs(mo[0]) and abs(m[1] - mo[1]) <= 1e-13 * abs(mo[1])
