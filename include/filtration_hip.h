@@ -257,6 +257,20 @@ int filt_u_roS_P1_S2_calculation(const filt_fields* dev, const filt_params* p,
  * writes mass2 (host memory) before returning. */
 int filt_mass(const filt_fields* dev, const filt_params* p, int localNx, double* mass2);
 
+/* filt_mass without any allocation: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Enqueues the same
+ * kernels as filt_mass on the library stream and returns without waiting;
+ * the two masses (water, NAPL) land in dev_mass2, device memory of 2
+ * doubles the caller owns, bit for bit what filt_mass returns.  The partial
+ * sums live in a fixed 1 MiB array in each GPU's memory (held until the
+ * process ends), so grids of at most 268435456 cells (65536 partials of
+ * 4096 cells); larger grids return FILT_ERR_INVALID.  Calls are
+ * independent (nothing carries over, nothing needs clearing), but calls that
+ * overlap on one GPU share that array and race: keep them on one stream. */
+int filt_mass_noalloc(const filt_fields* dev, const filt_params* p, int localNx,
+                      double* dev_mass2);
+
 /* Number of launches of the one-kernel stages 2-4 (k_step_zm) since the
  * library was loaded, from any entry point.  A diagnostic with no
  * counterpart in the reference: it lets a caller see which path its calls

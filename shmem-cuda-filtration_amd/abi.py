@@ -143,6 +143,7 @@ def load_library(path=HIP_LIB):
     _sig(lib, "filt_device_copy", c_int, c_void_p, c_void_p, c_size_t)
     _sig(lib, "filt_step_zm_launches", ctypes.c_longlong)
     _sig(lib, "filt_mass", c_int, F, P, c_int, POINTER(c_double))
+    _sig(lib, "filt_mass_noalloc", c_int, F, P, c_int, c_void_p)
     _sig(lib, "filt_device_malloc", c_int, POINTER(c_void_p), c_size_t)
     _sig(lib, "filt_device_free_ptr", c_int, c_void_p)
     for name in ("filt_ro_P2_Xi_calculation", "filt_u_calculation",

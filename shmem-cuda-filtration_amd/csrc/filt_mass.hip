@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <mutex>
+#include <vector>
 
 #include "../../include/filtration_hip.h"
 #include "filt_device.hpp"
@@ -113,6 +114,39 @@ MassIn mass_in(const filt_fields& f, const filt_params& p, int nx)
     return in;
 }
 
+// Partials of filt_mass_noalloc: a fixed array in each GPU's memory, held
+// until the process ends.  Host code never passes its name to a kernel or a
+// copy (in host code it stands for a host-side shadow, not a device
+// address): it uses the device address hipGetSymbolAddress returns for the
+// current device, looked up once per device (scratch_on).
+constexpr long long NOALLOC_MAX_PARTIALS = 65536;  // 2 doubles each: 1 MiB
+constexpr long long NOALLOC_MAX_CELLS = NOALLOC_MAX_PARTIALS * CHUNK;
+static __device__ __align__(256) double scratch[2 * NOALLOC_MAX_PARTIALS];
+
+std::mutex g_scratch_mutex;
+std::vector<double*> g_scratch_ptr;  // per device; null until looked up
+
+// The current device's address of `scratch` (looked up on the first call
+// with that device current).
+int scratch_on(double** out)
+{
+    int dev = 0;
+    FILT_HIP_TRY(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if ((int)g_scratch_ptr.size() <= dev)
+        g_scratch_ptr.resize(dev + 1, nullptr);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return hip_fail(e != hipSuccess ? e : hipErrorInvalidSymbol,
+                            "hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[dev] = (double*)ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return FILT_OK;
+}
+
 // Partials of the mass kernels: one buffer per process, grown on demand and
 // kept until the process ends (calls are serialised by the mutex).
 std::mutex g_mass_mutex;
@@ -121,11 +155,19 @@ size_t g_mass_cap = 0;  // doubles
 
 }  // namespace
 
+// Looks up this device's scratch address at set-up (filt_device_initialize),
+// so that a later no-allocation call finds it ready.
+int mass_scratch_init()
+{
+    double* p = nullptr;
+    return scratch_on(&p);
+}
+
 long long mass_partials(long long cells) { return (cells + CHUNK - 1) / CHUNK; }
 
-// Enqueue the two mass kernels: partials (2 per CHUNK cells) in scratch,
-// the result in out2 (device).
-int launch_grid_mass(const filt_fields& f, const filt_params& p, int nx, double* scratch,
+// Enqueue the two mass kernels: partials (2 per CHUNK cells) in `partial`
+// (device), the result in out2 (device).
+int launch_grid_mass(const filt_fields& f, const filt_params& p, int nx, double* partial,
                      double* out2, hipStream_t s)
 {
     const long long n = (long long)nx * p.Ny * p.Nz;
@@ -133,9 +175,9 @@ int launch_grid_mass(const filt_fields& f, const filt_params& p, int nx, double*
     if (nb <= 0 || nb > 0x7fffffffLL)
         return set_error(FILT_ERR_INVALID, "grid size out of range for the mass sum");
     hipLaunchKernelGGL(k_grid_mass_partial, dim3((unsigned)nb), dim3(NT), 0, s, mass_in(f, p, nx),
-                       scratch);
+                       partial);
     FILT_LAUNCH_CHECK("k_grid_mass_partial");
-    hipLaunchKernelGGL(k_grid_mass_final, dim3(1), dim3(NT), 0, s, scratch, nb, out2);
+    hipLaunchKernelGGL(k_grid_mass_final, dim3(1), dim3(NT), 0, s, partial, nb, out2);
     FILT_LAUNCH_CHECK("k_grid_mass_final");
     return FILT_OK;
 }
@@ -172,5 +214,31 @@ extern "C" int filt_mass(const filt_fields* dev, const filt_params* p, int local
                                     stream()));
         FILT_HIP_TRY(hipStreamSynchronize(stream()));
         return FILT_OK;
+    });
+}
+
+extern "C" int filt_mass_noalloc(const filt_fields* dev, const filt_params* p, int localNx,
+                                 double* dev_mass2)
+{
+    using namespace filt;
+    return guarded([&]() -> int {
+        if (!dev || !dev_mass2 || !dev->P1 || !dev->S2)
+            return set_error(FILT_ERR_INVALID, "null argument");
+        int rc = check_shape(p, localNx);
+        if (rc)
+            return rc;
+        const long long cells = (long long)localNx * p->Ny * p->Nz;
+        if (cells > NOALLOC_MAX_CELLS)
+            return set_error(FILT_ERR_INVALID,
+                             "filt_mass_noalloc: %lld cells exceed its fixed scratch of %lld "
+                             "partials (at most %lld cells); use filt_mass",
+                             cells, NOALLOC_MAX_PARTIALS, NOALLOC_MAX_CELLS);
+        double* sp = nullptr;
+        rc = scratch_on(&sp);
+        if (rc)
+            return rc;
+        // every partial the final pass reads is written by this call's first
+        // kernel, so nothing needs clearing first
+        return launch_grid_mass(*dev, *p, localNx, sp, dev_mass2, stream());
     });
 }

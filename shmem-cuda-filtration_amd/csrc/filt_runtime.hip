@@ -147,6 +147,9 @@ int filt_device_initialize(int rank, int localNx, const filt_params* p, int* bx,
     if (count < 1)
         return set_error(FILT_ERR_HIP, "no HIP device");
     FILT_HIP_TRY(hipSetDevice(rank % count));
+    rc = mass_scratch_init();  // filt_mass_noalloc's scratch on this device
+    if (rc)
+        return rc;
     if (bx) *bx = (localNx + 15) / 16;
     if (by) *by = (p->Ny + 3) / 4;
     if (bz) *bz = (p->Nz + 3) / 4;

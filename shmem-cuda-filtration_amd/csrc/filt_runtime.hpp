@@ -35,6 +35,10 @@ int launch_step_zm(const filt_fields& f, const filt_params& p, int nx, bool with
 int launch_roS_zm(const filt_fields& f, const filt_params& p, int nx);
 int launch_u_zm(const filt_fields& f, const filt_params& p, int nx, int rank, int size);
 
+// The no-allocation mass sum's scratch address on the current device
+// (filt_mass.hip), looked up at set-up.
+int mass_scratch_init();
+
 }  // namespace filt
 
 #define FILT_HIP_TRY(call)                                   \
