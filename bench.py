@@ -54,12 +54,19 @@ T0 = time.perf_counter()
 #             ro1, ro2 follow from P2 as stage 1 computed them):
 #             P2, Xi1, Xi2, S2, poro in (40); six u, four roS,
 #             P1, S2 out (96)                                           = 136
+#   the whole step but the boundary in one z-march (k_step_full):
+#             P1, S2, perm, poro in (32); P2, Xi1, Xi2, ro1, ro2, six u,
+#             four roS, P1, S2 out (136)                                = 168
+#   the stash copy (k_unstash) moves no algorithmic bytes: it is what the
+#             one-kernel form pays for its in-place writes
 PLAIN_BYTES = {"ro_P2_Xi": 64, "u": 96, "roS": 96, "newton": 48, "u_roS_newton": 136,
-               "boundary": 0}
-STEP_STAGES = ("ro_P2_Xi", "u_roS_newton", "boundary")     # time_step_function
+               "step_full": 168, "unstash": 0, "boundary": 0}
+STEP_STAGES = ("step_full", "unstash", "boundary")   # filt_time_step (lambda = 2)
+TWO_KERNEL_STAGES = ("ro_P2_Xi", "u_roS_newton", "boundary")  # filt_time_step otherwise
 SPLIT_STAGES = ("ro_P2_Xi", "u", "roS", "newton", "boundary")  # the five stage calls
 KERNEL_NAMES = {"ro_P2_Xi": "k_ro_P2_Xi", "u": "k_u_zm", "roS": "k_roS_zm", "newton": "k_newton",
-                "u_roS_newton": "k_step_zm", "boundary": "k_bc_a+k_bc_b"}
+                "u_roS_newton": "k_step_zm", "step_full": "k_step_full", "unstash": "k_unstash",
+                "boundary": "k_bc_a+k_bc_b"}
 FORMS = {0: "fused_fp64", 1: "fast", 2: "mixed"}
 # P, S in + out (32) + perm, poro in FP64 (16), every form
 FUSED_BYTES = {0: 48, 1: 48, 2: 48}
@@ -152,6 +159,7 @@ def run_plain(args, p, perm, poro, split=False, steps=None):
     nx = d.nx
     stage1 = lambda t: lib.filt_ro_P2_Xi_calculation(fref, pref, nx, 0, 1)
     bc = lambda t: lib.filt_boundary_conditions(fref, pref, nx, 0, 1)
+    path = ctypes.c_int(-1)
     if split:
         names = SPLIT_STAGES
         calls = (stage1,
@@ -160,10 +168,11 @@ def run_plain(args, p, perm, poro, split=False, steps=None):
                  lambda t: lib.filt_P1_S2_calculation(fref, pref, nx, 0, 1),
                  bc)
     else:
-        names = STEP_STAGES
-        calls = (stage1,
-                 lambda t: lib.filt_u_roS_P1_S2_calculation(fref, pref, t, nx, 0, 1, 1),
-                 bc)
+        # filt_time_step, with HIP events recorded between its launches
+        PKG.abi.check(lib, lib.filt_time_step_timed(fref, pref, 0.0, nx, 0, 1, 1, None,
+                                                    ctypes.byref(path)), "path probe")
+        names = STEP_STAGES if path.value == 1 else TWO_KERNEL_STAGES
+        calls = None
     best = (ctypes.c_double * 2)()
     if args.autotune and split:
         # stage 1 fills the inputs the stencil kernels read, then the
@@ -171,27 +180,39 @@ def run_plain(args, p, perm, poro, split=False, steps=None):
         PKG.abi.check(lib, stage1(0.0), "stage 1")
         PKG.abi.check(lib, lib.filt_autotune_stencil(fref, pref, nx, 5, best), "autotune")
     tuned = {"autotuned": bool(args.autotune and split)}
-    for kern, name in ((0, "roS"), (1, "u"), (2, "u_roS_newton")):
+    for kern, name in ((0, "roS"), (1, "u"), (2, "u_roS_newton"), (3, "step_full")):
         v = [ctypes.c_int() for _ in range(4)]
         lib.filt_get_tile_config(kern, nx, p.Ny, p.Nz, *[ctypes.byref(x) for x in v])
         tuned[name] = {"tile": [v[0].value, v[1].value], "kz": v[2].value, "order": v[3].value}
         if args.autotune and split and kern < 2:
             tuned[name]["tuned_ms"] = best[kern]
+    def step_timed(t, ev):
+        arr = (ctypes.c_void_p * 4)(*[e.h.value for e in ev])
+        PKG.abi.check(lib, lib.filt_time_step_timed(fref, pref, t, nx, 0, 1, 1, arr, None),
+                      "step")
+
     for s in range(1, args.warmup + 1):
-        for c in calls:
-            PKG.abi.check(lib, c(s * p.dt), "warmup")
+        if calls:
+            for c in calls:
+                PKG.abi.check(lib, c(s * p.dt), "warmup")
+        else:
+            PKG.abi.check(lib, lib.filt_time_step(fref, pref, s * p.dt, nx, 0, 1, 1), "warmup")
     d.sync()
     K = steps or args.steps
-    ev = [[hiprt.Event() for _ in range(len(calls) + 1)] for _ in range(K)]
+    nev = len(calls) + 1 if calls else 4
+    ev = [[hiprt.Event() for _ in range(nev)] for _ in range(K)]
     hiprt.device_synchronize()
     t0 = time.perf_counter()
     for s in range(K):
         t = (args.warmup + s + 1) * p.dt
         e = ev[s]
-        e[0].record(stream)
-        for q, c in enumerate(calls):
-            PKG.abi.check(lib, c(t), "step")
-            e[q + 1].record(stream)
+        if calls:
+            e[0].record(stream)
+            for q, c in enumerate(calls):
+                PKG.abi.check(lib, c(t), "step")
+                e[q + 1].record(stream)
+        else:
+            step_timed(t, e)
     hiprt.device_synchronize()
     wall = time.perf_counter() - t0
     stage_ms = {name: sum(ev[s][q].elapsed_ms(ev[s][q + 1]) for s in range(K)) / K
@@ -201,7 +222,7 @@ def run_plain(args, p, perm, poro, split=False, steps=None):
     import numpy as np
     finite = bool(np.isfinite(P1).all() and np.isfinite(S2).all())
     return {"wall": wall, "steps": K, "stage_ms": stage_ms, "stability_r": r, "finite": finite,
-            "tiles": tuned}
+            "tiles": tuned, "names": names}
 
 
 def plain_state(p, perm, poro, steps, every=None):
@@ -330,6 +351,35 @@ def run_fused(args, p, perm, poro, precision, steps=None, every=None):
             "hbm_GBps": FUSED_BYTES[precision] * cells / (ms * 1e-3) / 1e9,
             "hbm_frac": FUSED_BYTES[precision] * cells / (ms * 1e-3) / HBM_PEAK}
     return info, (P, S, m, series)
+
+
+def run_one_form(args):
+    """One fused form on the headline grid: args.warmup eager steps, then
+    args.steps steps as one graph, timed; the tiling from --tiling or the
+    autotuner."""
+    hiprt = importlib.import_module(PKG.__name__ + ".hiprt")
+    be = PKG.device.Backend()
+    prec = {"fp64": 0, "fast": 1, "mixed": 2}[args.form]
+    p, perm, poro = PKG.workloads.c2_workload(args.n)
+    g = PKG.workloads.initial_grid(p, perm, poro)
+    e = be.ensemble(prec)
+    try:
+        e.add(p, g["P1"], g["S2"], perm, poro)
+        if args.tiling:
+            t, kz, order = args.tiling.split(",")
+            tx, ty = (int(v) for v in t.split("x"))
+            e.set_tiling(int(kz), int(order), (tx, ty))
+        else:
+            e.autotune(3)
+        tiling = e.tiling()
+        e.step(args.warmup)
+        ms, _ = time_ensemble(e, args.steps, hiprt, be.lib)
+    finally:
+        e.close()
+    cells = p.NX * p.Ny * p.Nz
+    return {"form": args.form, "value": cells / (ms * 1e-3), "unit": "cell-updates/s",
+            "ms_per_step": ms, "tiling": tiling, "grid": [p.NX, p.Ny, p.Nz],
+            "hbm_frac": FUSED_BYTES[prec] * cells / (ms * 1e-3) / HBM_PEAK}
 
 
 def oracle_run(p, perm, poro, steps, every=None, with_u=1):
@@ -679,6 +729,12 @@ def main():
                          "(default: the library's built-in shapes)")
     ap.add_argument("--copy", action="store_true",
                     help="only run the copy-kernel calibration (1 GiB) and print it")
+    ap.add_argument("--form", choices=("fp64", "fast", "mixed"),
+                    help="only run this fused form on the headline grid and print its line "
+                         "(for profiles)")
+    ap.add_argument("--tiling", default=None,
+                    help="with --form: fixed tiling TXxTY,kz,order (e.g. 64x8,64,1) instead "
+                         "of the autotuner")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -686,6 +742,9 @@ def main():
         raise SystemExit(2)
     if args.copy:
         print(json.dumps(copy_calibration()), flush=True)
+        return
+    if args.form:
+        print(json.dumps(run_one_form(args)), flush=True)
         return
 
     p, perm, poro = PKG.workloads.c2_workload(args.n)
@@ -695,6 +754,7 @@ def main():
     cells = p.NX * p.Ny * p.Nz
     value = cells * args.steps / wall
     forms, states, split, shims, c2_oracle, c2_port = {}, {}, None, None, None, None
+    one_kernel = None
     if not args.plain_only:
         states["plain"] = plain_state(p, perm, poro, args.steps)
         for prec, name in FORMS.items():
@@ -717,6 +777,20 @@ def main():
                  r5["wall"] / HBM_PEAK}
         shims = {"fused": run_shim(args, p, perm, poro, k5, True),
                  "separate": run_shim(args, p, perm, poro, k5, False)}
+        # the opt-in whole-step kernel (stage 1 with stages 2-4, stash copy)
+        lib = PKG.abi.load_library()
+        PKG.abi.check(lib, lib.filt_set_tile_config(3, 64, 16, 64, 0), "whole-step kernel on")
+        try:
+            r1 = run_plain(args, p, perm, poro, steps=k5)
+        finally:
+            lib.filt_set_tile_config(3, 0, 0, 0, 0)
+        full_ms = r1["stage_ms"].get("step_full")
+        one_kernel = {"value": cells * k5 / r1["wall"], "ms_per_step": r1["wall"] * 1e3 / k5,
+                      "stages_ms": r1["stage_ms"], "tile": [64, 16], "kz": 64, "order": 0,
+                      "bytes_per_cell_update": PLAIN_BYTES["step_full"],
+                      "k_step_full_frac": (PLAIN_BYTES["step_full"] * cells /
+                                           (full_ms * 1e-3) / HBM_PEAK) if full_ms else None,
+                      "stream_mix_ceiling": stream_mix_ceiling("k_step_full", cells)}
         if not args.no_cpu:
             log(f"oracle port on {args.n}^3, {args.steps} steps, {host_threads()} threads")
             ref, secs, mref, _ = oracle_run(p, perm, poro, args.steps)
@@ -735,7 +809,8 @@ def main():
             forms["batched_ensemble"] = run_c4(args)
         if args.serve_duration > 0:
             forms["serving"] = run_serving(args)
-    dom = max((s for s in STEP_STAGES if PLAIN_BYTES[s]), key=lambda s: stage_ms[s])
+    step_names = pr["names"]
+    dom = max((s for s in step_names if PLAIN_BYTES[s]), key=lambda s: stage_ms[s])
     dom_bytes = PLAIN_BYTES[dom] * cells
     achieved = dom_bytes / (stage_ms[dom] * 1e-3)
     tag = f"plain_{args.n}cube"
@@ -743,8 +818,7 @@ def main():
     mix_ceiling = stream_mix_ceiling(KERNEL_NAMES[dom], cells)
     if mix_ceiling:
         mix_ceiling["kernel_frac_of_mix"] = mix_ceiling["ms"] / stage_ms[dom]
-    step_bytes = sum(PLAIN_BYTES[s] for s in STEP_STAGES) * cells
-    split_bytes = sum(PLAIN_BYTES[s] for s in SPLIT_STAGES) * cells
+    step_bytes = sum(PLAIN_BYTES[s] for s in step_names) * cells
     line = {
         "metric": "cell-updates/s",
         "value": value,
@@ -759,9 +833,10 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"plain FP64, {args.n}^3, time_step_function per step (stage 1, "
-                        "stages 2-4 fused, boundary; every ptr_Arrays array materialised), "
-                        "per-cell log-normal K (sigma 1, corr 8 cells) and porosity",
+            "workload": f"plain FP64, {args.n}^3, time_step_function per step (stage 1 and "
+                        "stages 2-4 in one kernel, the boundary; every ptr_Arrays array "
+                        "materialised), per-cell log-normal K (sigma 1, corr 8 cells) and "
+                        "porosity",
             "grid": [p.NX, p.Ny, p.Nz],
             "form": "plain",
             "deviations": "beta1=1e-4, beta2=1e-8 (stated 4.4e-10 is unstable); "
@@ -783,11 +858,12 @@ def main():
         },
         "stages_ms": stage_ms,
         "step_roofline": {
-            "bytes_per_cell_update": sum(PLAIN_BYTES[s] for s in STEP_STAGES),
+            "bytes_per_cell_update": sum(PLAIN_BYTES[s] for s in step_names),
             "achieved_GBps": step_bytes * args.steps / wall / 1e9,
             "frac": step_bytes * args.steps / wall / HBM_PEAK,
         },
         "stage_calls": split,
+        "plain_one_kernel": one_kernel,
         "defines_shims": shims,
         "c2_oracle": c2_oracle,
         "forms": forms,

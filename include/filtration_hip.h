@@ -280,9 +280,26 @@ long long filt_step_zm_launches(void);
 /* One whole time step in the order of time_step_function (main.cpp:83-94):
  * stage 1, filt_u_roS_P1_S2_calculation, boundary conditions; the exchanges
  * are no-ops on one GPU (no_communication.cpp:3-14).  with_u = 0 skips
- * u_calculation, which nothing on the P/S path reads. */
+ * u_calculation, which nothing on the P/S path reads.  After
+ * filt_set_tile_config(3, tx, ty, kz, order), on one rank whose grid has
+ * every extent >= 3, arrays below 4 GiB, the upwind scheme and media with
+ * lambda = 2, stage 1 and stages 2-4 run as one z-marching kernel that reads
+ * P1, S2 once and writes every array the four stages write (168 B per cell
+ * instead of 200); the new P1, S2 of the cells other blocks read go through
+ * a stash that a second kernel copies back.  Every array ends with the
+ * same bits either way. */
 int filt_time_step(const filt_fields* dev, const filt_params* p, double t,
                    int localNx, int rank, int size, int with_u);
+
+/* filt_time_step, recording caller-supplied hipEvent_t between its launches
+ * on the library stream (no reference counterpart: per-kernel timing for a
+ * benchmark).  events (4 entries, any may be null): [0] before the step,
+ * [1] after stage 1 with stages 2-4 in one kernel (*path = 1; the
+ * lambda = 2 single-rank case) or after stage 1 alone (*path = 0), [2]
+ * after the stash copy (path 1) or stages 2-4 (path 0), [3] after the
+ * boundary conditions.  Same results as filt_time_step. */
+int filt_time_step_timed(const filt_fields* dev, const filt_params* p, double t, int localNx,
+                         int rank, int size, int with_u, void* const* events, int* path);
 
 /* load_exchange_data / save_exchange_data (defines.h:115-116,
  * gpu.cu:613-653): pack x-planes i=1 and i=localNx-2 of dev_array into
@@ -297,15 +314,18 @@ int filt_save_exchange_data(double* dev_array, double* dev_buffer,
                             int size, const filt_params* p);
 
 /* Tile shape of the z-marching stencil kernels of u_calculation (kernel 1),
- * roS_calculation (kernel 0) and the fused stages 2-4 (kernel 2): tx*ty
- * threads per block (64x4, 32x8, 16x16 or 64x8), kz planes per z-chunk,
- * order 0 = XCD-grouped tile
+ * roS_calculation (kernel 0), the fused stages 2-4 (kernel 2) and the whole
+ * step, stage 1 with stages 2-4 (kernel 3): tx*ty threads per block (64x4,
+ * 32x8, 16x16 or 64x8), kz planes per z-chunk, order 0 = XCD-grouped tile
  * numbering, 1 = hardware order.  Used by grids with every extent >= 3.
- * tx = 0 restores the built-in default (the fastest configuration on a
- * 256^3 grid: roS 64x4, kz 16, order 1; u 64x8, kz 64, order 1); tx = -1
- * selects the one-cell-per-thread kernels, which grids with an extent below
- * 3 or arrays of 4 GiB and more use anyway (results are the same).
- * The setting is process-wide; results do not depend on it. */
+ * The whole-step kernel also takes 32x16 and 64x16 (1024 threads), and
+ * so does the stages 2-4 kernel.  tx = 0 restores the built-in default
+ * (roS 64x8, kz 64, order 1; u 64x8, kz 64, order 1; stages 2-4 64x8, kz 16,
+ * order 1; the whole step off); tx = -1 switches the kernel off: the
+ * one-cell-per-thread kernels for 0-1, the stages one by one for 2, stage 1
+ * and stages 2-4 apart for 3 (grids with an extent below 3 or arrays of
+ * 4 GiB and more take those anyway; results are the same).  Setting a tile
+ * for kernel 3 switches the whole-step kernel on.  Process-wide. */
 int filt_set_tile_config(int kernel, int tx, int ty, int kz, int order);
 /* The configuration used for an nx*ny*nz grid (tuned or default). */
 int filt_get_tile_config(int kernel, int nx, int ny, int nz, int* tx, int* ty, int* kz,

@@ -151,6 +151,8 @@ def load_library(path=HIP_LIB):
         _sig(lib, name, c_int, F, P, c_int, c_int, c_int)
     _sig(lib, "filt_roS_calculation", c_int, F, P, c_double, c_int, c_int, c_int)
     _sig(lib, "filt_time_step", c_int, F, P, c_double, c_int, c_int, c_int, c_int)
+    _sig(lib, "filt_time_step_timed", c_int, F, P, c_double, c_int, c_int, c_int, c_int,
+         POINTER(c_void_p), POINTER(c_int))
     _sig(lib, "filt_u_roS_P1_S2_calculation", c_int, F, P, c_double, c_int, c_int, c_int, c_int)
     _sig(lib, "filt_load_exchange_data", c_int, c_void_p, c_void_p, c_void_p, c_int, P)
     _sig(lib, "filt_save_exchange_data", c_int, c_void_p, c_void_p, c_void_p, c_int,

@@ -26,12 +26,24 @@ int check_shape(const filt_params* p, int localNx);
 // arrays below 4 GiB (tiled_fits).
 bool tiled_fits(int nx, int ny, int nz);
 // False after filt_set_tile_config(kernel, -1, ...): kernel 0 roS, 1 u,
-// 2 the fused step.
+// 2 the fused stages 2-4, 3 the whole step.
 bool tiled_enabled(int kernel);
 // The fused step (filt_step.hip): u, roS and the Newton in one z-march;
 // false when the grid or the parameters need the stages one by one.
 bool step_fusable(const filt_params& p, int nx, int rank, int size);
 int launch_step_zm(const filt_fields& f, const filt_params& p, int nx, bool with_u);
+// The whole step but the boundary in one launch (filt_step.hip): stage 1
+// with stages 2-4, then the copy of its stash; for grids step_fusable
+// accepts whose media in use all have lambda = 2 (and DivBy-able mu1, mu2,
+// 1 - S_wr), unless switched off with filt_set_tile_config(3, -1, ...).
+bool full_step_fusable(const filt_params& p, int nx, int rank, int size, bool per_cell_media);
+// Tile shapes of the whole-step kernel: 64x4, 32x8, 16x16, 64x8, 32x16, 64x16.
+bool full_tile_ok(int tx, int ty);
+// Tile shapes of the stages 2-4 kernel (the same six).
+bool step_tile_ok(int tx, int ty);
+// mid (may be null) is recorded between the two launches.
+int launch_step_full(const filt_fields& f, const filt_params& p, int nx, bool with_u,
+                     hipEvent_t mid = nullptr);
 int launch_roS_zm(const filt_fields& f, const filt_params& p, int nx);
 int launch_u_zm(const filt_fields& f, const filt_params& p, int nx, int rank, int size);
 

@@ -610,13 +610,45 @@ int filt_u_roS_P1_S2_calculation(const filt_fields* dev, const filt_params* p, d
     return filt_P1_S2_calculation(dev, p, localNx, rank, size);
 }
 
+int filt_time_step_timed(const filt_fields* dev, const filt_params* p, double t, int localNx,
+                         int rank, int size, int with_u, void* const* events, int* path)
+{
+    auto mark = [&](int q) -> int {
+        if (events && events[q])
+            FILT_HIP_TRY(hipEventRecord((hipEvent_t)events[q], stream()));
+        return FILT_OK;
+    };
+    int rc;
+    const bool full = dev && p && full_step_fusable(*p, localNx, rank, size,
+                                                   dev->media && p->n_media > 1);
+    if (path)
+        *path = full ? 1 : 0;
+    if ((rc = mark(0))) return rc;
+    if (full) {
+        // stage 1 and stages 2-4 in one launch, the stash copy, the boundary
+        if ((rc = launch_check(dev, p, localNx))) return rc;
+        if (!dev->P2 || !dev->Xi1 || !dev->Xi2 || !dev->ro1 || !dev->ro2 || !dev->roS1 ||
+            !dev->roS2 || !dev->roS1_old || !dev->roS2_old ||
+            (with_u && (!dev->u1x || !dev->u1y || !dev->u1z || !dev->u2x || !dev->u2y ||
+                        !dev->u2z)))
+            return set_error(FILT_ERR_INVALID, "null device fields");
+        if ((rc = launch_step_full(*dev, *p, localNx, with_u != 0,
+                                   events ? (hipEvent_t)events[1] : nullptr)))
+            return rc;
+    } else {
+        if ((rc = filt_ro_P2_Xi_calculation(dev, p, localNx, rank, size))) return rc;
+        if ((rc = mark(1))) return rc;
+        if ((rc = filt_u_roS_P1_S2_calculation(dev, p, t, localNx, rank, size, with_u))) return rc;
+    }
+    if ((rc = mark(2))) return rc;
+    if ((rc = filt_boundary_conditions(dev, p, localNx, rank, size))) return rc;
+    return mark(3);
+}
+
 int filt_time_step(const filt_fields* dev, const filt_params* p, double t, int localNx, int rank,
                    int size, int with_u)
 {
-    int rc;
-    if ((rc = filt_ro_P2_Xi_calculation(dev, p, localNx, rank, size))) return rc;
-    if ((rc = filt_u_roS_P1_S2_calculation(dev, p, t, localNx, rank, size, with_u))) return rc;
-    return filt_boundary_conditions(dev, p, localNx, rank, size);
+    return filt_time_step_timed(dev, p, t, localNx, rank, size, with_u, nullptr, nullptr);
 }
 
 }  // extern "C"

@@ -354,13 +354,20 @@ UFn pick_u(int tx, int ty)
 constexpr TileConfig kRoSDefault = {64, 8, 64, 1};
 constexpr TileConfig kUDefault = {64, 8, 64, 1};
 constexpr TileConfig kStepDefault = {64, 8, 16, 1};
+// The whole-step kernel is off by default (g_untiled[3]): on a 256^3 grid its
+// best configuration (64x16, kz 64) ran 0.721 ms per step with its stash
+// copy against 0.726 ms for stage 1 + the stages 2-4 kernel -- its 17 output
+// streams run at 5.3 TB/s at best on MI355X against 5.7 and 6.2 for the two
+// kernels' mixes, which eats the 32 B per cell it saves.
+constexpr TileConfig kFullDefault = {64, 16, 64, 0};
 TileConfig g_roS_cfg = kRoSDefault;
 TileConfig g_u_cfg = kUDefault;
 TileConfig g_step_cfg = kStepDefault;
+TileConfig g_full_cfg = kFullDefault;
 // Per kernel (0 roS, 1 u, 2 the fused step): run the one-cell-per-thread
 // kernels of filt_stages.hip (for 2: the stages one by one) even where the
 // tiled ones apply (filt_set_tile_config with tx = -1).
-bool g_untiled[3] = {false, false, false};
+bool g_untiled[4] = {false, false, false, true};
 
 // Autotuned configurations by grid shape (filt_autotune_stencil).
 std::mutex g_tune_mu;
@@ -369,8 +376,13 @@ std::map<std::tuple<int, int, int>, std::pair<TileConfig, TileConfig>> g_tuned;
 // The fused step offers the tile shapes of the roS kernel.
 bool valid_cfg(const TileConfig& t, int kernel)
 {
-    return t.kz >= 1 && t.kz <= 4096 && (t.order == 0 || t.order == 1) &&
-           (kernel != 1 ? pick_roS(t.tx, t.ty) != nullptr : pick_u(t.tx, t.ty) != nullptr);
+    if (!(t.kz >= 1 && t.kz <= 4096 && (t.order == 0 || t.order == 1)))
+        return false;
+    if (kernel == 3)
+        return full_tile_ok(t.tx, t.ty);
+    if (kernel == 2)
+        return step_tile_ok(t.tx, t.ty);
+    return kernel != 1 ? pick_roS(t.tx, t.ty) != nullptr : pick_u(t.tx, t.ty) != nullptr;
 }
 
 TileConfig cfg_for(int kernel, int nx, int ny, int nz)
@@ -378,6 +390,8 @@ TileConfig cfg_for(int kernel, int nx, int ny, int nz)
     std::lock_guard<std::mutex> lk(g_tune_mu);
     if (kernel == 2)
         return g_step_cfg;
+    if (kernel == 3)
+        return g_full_cfg;
     auto it = g_tuned.find(std::make_tuple(nx, ny, nz));
     if (it != g_tuned.end())
         return kernel == 0 ? it->second.first : it->second.second;
@@ -399,6 +413,7 @@ bool tiled_enabled(int kernel)
 }
 
 TileConfig step_tile_config(int nx, int ny, int nz) { return cfg_for(2, nx, ny, nz); }
+TileConfig full_tile_config(int nx, int ny, int nz) { return cfg_for(3, nx, ny, nz); }
 
 // Launchers used by filt_stages.hip (grids with NX, Ny, Nz >= 3).
 int launch_roS_zm(const filt_fields& f, const filt_params& p, int nx)
@@ -427,22 +442,32 @@ extern "C" int filt_set_tile_config(int kernel, int tx, int ty, int kz, int orde
 {
     return filt::guarded([&]() -> int {
         filt::TileConfig t = {tx, ty, kz, order};
-        if (kernel < 0 || kernel > 2)
-            return filt::set_error(FILT_ERR_INVALID, "kernel must be 0 (roS), 1 (u) or 2 (step)");
+        if (kernel < 0 || kernel > 3)
+            return filt::set_error(FILT_ERR_INVALID,
+                                   "kernel must be 0 (roS), 1 (u), 2 (stages 2-4) or 3 (step)");
         if (tx == -1) {  // the untiled kernels
             std::lock_guard<std::mutex> lk(filt::g_tune_mu);
             filt::g_untiled[kernel] = true;
             return FILT_OK;
         }
-        if (tx == 0)  // back to the built-in default
-            t = kernel == 0 ? filt::kRoSDefault : kernel == 1 ? filt::kUDefault : filt::kStepDefault;
+        if (tx == 0) {  // back to the built-in default
+            t = kernel == 0 ? filt::kRoSDefault : kernel == 1 ? filt::kUDefault
+              : kernel == 2 ? filt::kStepDefault : filt::kFullDefault;
+            if (kernel == 3) {  // which is off for the whole-step kernel
+                std::lock_guard<std::mutex> lk(filt::g_tune_mu);
+                filt::g_full_cfg = t;
+                filt::g_untiled[3] = true;
+                return FILT_OK;
+            }
+        }
         if (!filt::valid_cfg(t, kernel))
             return filt::set_error(FILT_ERR_INVALID, "unsupported tile %dx%d kz=%d order=%d", tx, ty,
                                    kz, order);
         std::lock_guard<std::mutex> lk(filt::g_tune_mu);
-        (kernel == 0 ? filt::g_roS_cfg : kernel == 1 ? filt::g_u_cfg : filt::g_step_cfg) = t;
+        (kernel == 0 ? filt::g_roS_cfg : kernel == 1 ? filt::g_u_cfg
+         : kernel == 2 ? filt::g_step_cfg : filt::g_full_cfg) = t;
         filt::g_untiled[kernel] = false;
-        if (kernel != 2)
+        if (kernel < 2)
             filt::g_tuned.clear();
         return FILT_OK;
     });
@@ -529,8 +554,9 @@ extern "C" int filt_autotune_stencil(const filt_fields* dev, const filt_params* 
 extern "C" int filt_get_tile_config(int kernel, int nx, int ny, int nz, int* tx, int* ty, int* kz,
                                     int* order)
 {
-    if (kernel < 0 || kernel > 2)
-        return filt::set_error(FILT_ERR_INVALID, "kernel must be 0 (roS), 1 (u) or 2 (step)");
+    if (kernel < 0 || kernel > 3)
+        return filt::set_error(FILT_ERR_INVALID,
+                               "kernel must be 0 (roS), 1 (u), 2 (stages 2-4) or 3 (step)");
     const filt::TileConfig t = filt::cfg_for(kernel, nx, ny, nz);
     if (tx) *tx = t.tx;
     if (ty) *ty = t.ty;

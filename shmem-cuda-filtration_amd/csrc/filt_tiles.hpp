@@ -46,6 +46,8 @@ struct TileConfig {
 
 // The fused step's configuration for a grid (filt_stencil.hip).
 TileConfig step_tile_config(int nx, int ny, int nz);
+// The whole-step kernel's configuration (stage 1 with stages 2-4).
+TileConfig full_tile_config(int nx, int ny, int nz);
 
 // Tiles and z-chunks covering an nx * ny * nz grid.
 inline TileMap make_map(int nx, int ny, int nz, const TileConfig& t)

@@ -13,6 +13,8 @@
 //   k_roS_zm      8 in,  4 out   S2, P1, P2, Xi1, Xi2, ro1, ro2, poro -> four roS
 //   k_newton      4 in,  2 out   S2, P1, roS1, roS2 -> P1, S2
 //   k_step_zm     5 in, 12 out   P2, Xi1, Xi2, S2, poro -> six u, four roS, P1, S2
+//   k_step_full   4 in, 17 out   P1, S2, perm, poro -> P2, Xi1, Xi2, ro1, ro2,
+//                                six u, four roS, P1, S2
 //
 // Every mix runs with plain and with non-temporal stores.  Prints one JSON
 // line per mix and store kind.  Usage: stream_mix [cells [mix]] (default
@@ -34,7 +36,7 @@
 
 namespace {
 
-constexpr int kMaxIn = 8, kMaxOut = 12;
+constexpr int kMaxIn = 8, kMaxOut = 17;
 
 struct Streams {
     const double* in[kMaxIn];
@@ -148,6 +150,7 @@ int main(int argc, char** argv)
     if (want("k_roS_zm")) both<8, 4>("k_roS_zm", s, n);
     if (want("k_newton")) both<4, 2>("k_newton", s, n);
     if (want("k_step_zm")) both<5, 12>("k_step_zm", s, n);
+    if (want("k_step_full")) both<4, 17>("k_step_full", s, n);
 
     for (int a = 0; a < kMaxIn; a++)
         CHECK(hipFree((void*)s.in[a]));

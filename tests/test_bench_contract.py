@@ -23,12 +23,13 @@ def test_helpers_and_tables():
     b = load_bench()
     for name in ("cpu_baseline", "load_traffic", "run_plain", "run_fused", "run_c4",
                  "run_serving", "run_slab", "copy_calibration", "mixed_errors", "form_errors",
-                 "oracle_run", "plain_state", "host_info", "main"):
+                 "oracle_run", "plain_state", "host_info", "run_one_form", "main"):
         assert callable(getattr(b, name)), name
-    for s in b.STEP_STAGES + b.SPLIT_STAGES:
+    for s in b.STEP_STAGES + b.TWO_KERNEL_STAGES + b.SPLIT_STAGES:
         assert s in b.PLAIN_BYTES and s in b.KERNEL_NAMES, s
     assert sum(b.PLAIN_BYTES[s] for s in b.SPLIT_STAGES) == 304
-    assert sum(b.PLAIN_BYTES[s] for s in b.STEP_STAGES) == 200
+    assert sum(b.PLAIN_BYTES[s] for s in b.STEP_STAGES) == 168
+    assert sum(b.PLAIN_BYTES[s] for s in b.TWO_KERNEL_STAGES) == 200
 
 
 def test_every_global_name_resolves():

@@ -688,3 +688,62 @@ def test_device_mass_equals_oracle_mass(hip, orc, media, poro, dims):
         d.free()
     mo = orc.mass(g, p)
     assert abs(m[0] - mo[0]) <= 1e-13 * abs(mo[0]) and abs(m[1] - mo[1]) <= 1e-13 * abs(mo[1])
+
+
+def time_steps(hip, p, g, steps, with_u, perm, poro):
+    d = hip.grid(p, perm=perm, poro=poro)
+    try:
+        d.upload(g)
+        for s in range(1, steps + 1):
+            d.time_step(s * p.dt, with_u)
+        out = g.copy()
+        d.download(out)
+        return out
+    finally:
+        d.free()
+
+
+@pytest.mark.parametrize("case", ["fields", "media"])
+@pytest.mark.parametrize("with_u", [1, 0])
+def test_whole_step_kernel_every_config_equals_two_kernel_path(hip, case, with_u):
+    """With the whole-step kernel switched on, filt_time_step runs stage 1 and
+    stages 2-4 as one z-march (k_step_full) for lambda = 2 media, with the
+    cells other blocks read stashed and copied back: for every tile shape,
+    z-chunk depth and block order, every array after four steps equals
+    stage 1 + the stages 2-4 kernel bit for bit (ragged tiles and chunks:
+    37 x 19 x 23)."""
+    lib = hip.lib
+    p = bl_params(37, 19, 23)
+    perm = poro = case == "fields"
+    if perm:
+        p.n_media = 1
+    g = random_state(p, np.random.default_rng(7 + with_u), media=case == "media", perm=perm,
+                     poro=poro)
+    try:
+        assert lib.filt_set_tile_config(3, -1, 0, 0, 0) == 0  # the two-kernel path
+        want = time_steps(hip, p, g, 4, with_u, perm, poro)
+        for tile in TILES + [(32, 16), (64, 16)]:
+            for kz, order in ((1, 0), (5, 1), (16, 0), (64, 1)):
+                assert lib.filt_set_tile_config(3, tile[0], tile[1], kz, order) == 0
+                got = time_steps(hip, p, g, 4, with_u, perm, poro)
+                for n in FIELD_NAMES_ALL:
+                    assert np.array_equal(got[n], want[n], equal_nan=True), (tile, kz, order, n)
+    finally:
+        lib.filt_set_tile_config(3, 0, 0, 0, 0)
+
+
+def test_whole_step_kernel_c1_trajectory_vs_oracle(hip, orc):
+    """Configuration 1 (lambda = 2) through the whole-step kernel, 500 steps,
+    against the oracle."""
+    p = c1_params()
+    g = HostGrid(p)
+    orc.initial_data(g, p)
+    lib = hip.lib
+    try:
+        assert lib.filt_set_tile_config(3, 64, 16, 64, 0) == 0
+        out = time_steps(hip, p, g, 500, 1, False, False)
+    finally:
+        lib.filt_set_tile_config(3, 0, 0, 0, 0)
+    orc.run(g, p, 500)
+    for n in ("P1", "S2", "u1x", "u2y", "roS1", "roS2_old", "Xi1", "ro2", "P2"):
+        assert max_rel(out[n], g[n]) <= 1e-10, n
