@@ -367,8 +367,7 @@ def run_one_form(args):
         e.add(p, g["P1"], g["S2"], perm, poro)
         if args.tiling:
             t, kz, order = args.tiling.split(",")
-            tx, ty = (int(v) for v in t.split("x"))
-            e.set_tiling(int(kz), int(order), (tx, ty))
+            e.set_tiling(int(kz), int(order), tuple(int(v) for v in t.split("x")))
         else:
             e.autotune(3)
         tiling = e.tiling()
@@ -733,8 +732,8 @@ def main():
                     help="only run this fused form on the headline grid and print its line "
                          "(for profiles)")
     ap.add_argument("--tiling", default=None,
-                    help="with --form: fixed tiling TXxTY,kz,order (e.g. 64x8,64,1) instead "
-                         "of the autotuner")
+                    help="with --form: fixed tiling TXxTY[xCY],kz,order (e.g. 64x8x2,64,1) "
+                         "instead of the autotuner")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "

@@ -414,17 +414,20 @@ int filt_ensemble_get(filt_ensemble* e, int id, double* P1, double* S2);
  * sum phi*ro(P)*S*h1*h2*h3, FP64, fixed summation order. */
 int filt_ensemble_mass(filt_ensemble* e, int id, double* out2);
 
-/* Tiling of the fused kernel: tx x ty columns per tile (16x16, 32x8, 64x4,
- * 32x16 or 64x8; 0x0: 16x16), kz planes per z-chunk, order 0 = XCD-grouped,
- * 1 = hardware block order.  Results do not depend on the tiling. */
-int filt_ensemble_set_tiling(filt_ensemble* e, int tx, int ty, int kz, int order);
-int filt_ensemble_get_tiling(const filt_ensemble* e, int* tx, int* ty, int* kz, int* order);
+/* Tiling of the fused kernel: tx x ty columns per tile and cy columns per
+ * thread, stacked in y (16x16/1, 32x8/1, 64x4/1, 32x16/1, 64x8/1, 64x8/2,
+ * 32x16/2, 64x16/2; 0x0: 16x16/1), kz planes per z-chunk, order 0 =
+ * XCD-grouped, 1 = hardware block order.  Results do not depend on the
+ * tiling. */
+int filt_ensemble_set_tiling(filt_ensemble* e, int tx, int ty, int cy, int kz, int order);
+int filt_ensemble_get_tiling(const filt_ensemble* e, int* tx, int* ty, int* cy, int* kz,
+                             int* order);
 /* Autotuner: times each (tile shape, kz, order) candidate -- every tile
  * shape above, kz in {4, 8, 16, 32, 64}, both orders -- over reps steps of
  * the current membership (the members advance by the steps run) and keeps
  * the fastest.  Outputs may be null. */
-int filt_ensemble_autotune(filt_ensemble* e, int reps, int* best_tx, int* best_ty, int* best_kz,
-                           int* best_order, double* best_ms_per_step);
+int filt_ensemble_autotune(filt_ensemble* e, int reps, int* best_tx, int* best_ty, int* best_cy,
+                           int* best_kz, int* best_order, double* best_ms_per_step);
 
 /* Free / total device memory (admission control). */
 int filt_device_free_bytes(size_t* free_bytes, size_t* total_bytes);

@@ -177,9 +177,9 @@ def load_library(path=HIP_LIB):
     _sig(lib, "filt_ensemble_replay", c_int, E)
     _sig(lib, "filt_ensemble_get", c_int, E, c_int, c_void_p, c_void_p)
     _sig(lib, "filt_ensemble_mass", c_int, E, c_int, POINTER(c_double))
-    _sig(lib, "filt_ensemble_set_tiling", c_int, E, c_int, c_int, c_int, c_int)
-    _sig(lib, "filt_ensemble_get_tiling", c_int, E, pi, pi, pi, pi)
-    _sig(lib, "filt_ensemble_autotune", c_int, E, c_int, pi, pi, pi, pi, POINTER(c_double))
+    _sig(lib, "filt_ensemble_set_tiling", c_int, E, c_int, c_int, c_int, c_int, c_int)
+    _sig(lib, "filt_ensemble_get_tiling", c_int, E, pi, pi, pi, pi, pi)
+    _sig(lib, "filt_ensemble_autotune", c_int, E, c_int, pi, pi, pi, pi, pi, POINTER(c_double))
     _sig(lib, "filt_device_free_bytes", c_int, POINTER(c_size_t), POINTER(c_size_t))
     _sig(lib, "filt_serve_create", c_int, POINTER(c_void_p), c_int, c_size_t, c_int, c_int)
     _sig(lib, "filt_serve_destroy", c_int, c_void_p)

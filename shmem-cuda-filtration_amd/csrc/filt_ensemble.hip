@@ -175,11 +175,12 @@ int filt_ensemble_destroy(filt_ensemble* e)
     return FILT_OK;
 }
 
-int filt_ensemble_set_tiling(filt_ensemble* e, int tx, int ty, int kz, int order)
+int filt_ensemble_set_tiling(filt_ensemble* e, int tx, int ty, int cy, int kz, int order)
 {
-    const int tile = (tx == 0 && ty == 0) ? 0 : fused_tile_index(tx, ty);
+    const int tile = (tx == 0 && ty == 0) ? 0 : fused_tile_index(tx, ty, cy);
     if (!e || tile < 0 || kz < 1 || kz > 4096 || (order != 0 && order != 1))
-        return set_error(FILT_ERR_INVALID, "bad tiling %dx%d kz=%d order=%d", tx, ty, kz, order);
+        return set_error(FILT_ERR_INVALID, "bad tiling %dx%d/%d kz=%d order=%d", tx, ty, cy, kz,
+                         order);
     e->tile = tile;
     e->kz = kz;
     e->order = order;
@@ -475,12 +476,14 @@ int filt_device_free_bytes(size_t* free_b, size_t* total_b)
     return FILT_OK;
 }
 
-int filt_ensemble_get_tiling(const filt_ensemble* e, int* tx, int* ty, int* kz, int* order)
+int filt_ensemble_get_tiling(const filt_ensemble* e, int* tx, int* ty, int* cy, int* kz,
+                             int* order)
 {
     if (!e)
         return set_error(FILT_ERR_INVALID, "null ensemble");
     if (tx) *tx = FUSED_TILES[e->tile].tx;
     if (ty) *ty = FUSED_TILES[e->tile].ty;
+    if (cy) *cy = FUSED_TILES[e->tile].cy;
     if (kz) *kz = e->kz;
     if (order) *order = e->order;
     return FILT_OK;
@@ -489,8 +492,8 @@ int filt_ensemble_get_tiling(const filt_ensemble* e, int* tx, int* ty, int* kz, 
 // Time every (tile shape, kz, order) for the current membership: reps steps
 // each after one untimed step (the state advances by the steps run), keep
 // the fastest.
-int filt_ensemble_autotune(filt_ensemble* e, int reps, int* best_tx, int* best_ty, int* best_kz,
-                           int* best_order, double* best_ms)
+int filt_ensemble_autotune(filt_ensemble* e, int reps, int* best_tx, int* best_ty, int* best_cy,
+                           int* best_kz, int* best_order, double* best_ms)
 {
     return filt::guarded([&]() -> int {
         if (!e || reps < 1)
@@ -507,7 +510,7 @@ int filt_ensemble_autotune(filt_ensemble* e, int reps, int* best_tx, int* best_t
             for (int kz : kzs)
                 for (int order = 0; order < 2 && rc == FILT_OK && he == hipSuccess; order++) {
                     rc = filt_ensemble_set_tiling(e, FUSED_TILES[tile].tx, FUSED_TILES[tile].ty,
-                                                  kz, order);
+                                                  FUSED_TILES[tile].cy, kz, order);
                     if (!rc) rc = filt_ensemble_step(e, 1);  // warm
                     if (!rc) he = hipEventRecord(a, stream());
                     if (!rc && he == hipSuccess) rc = filt_ensemble_step(e, reps);
@@ -528,9 +531,11 @@ int filt_ensemble_autotune(filt_ensemble* e, int reps, int* best_tx, int* best_t
             return hip_fail(he, "filt_ensemble_autotune");
         if (rc)
             return rc;
-        rc = filt_ensemble_set_tiling(e, FUSED_TILES[bt].tx, FUSED_TILES[bt].ty, bk, bo);
+        rc = filt_ensemble_set_tiling(e, FUSED_TILES[bt].tx, FUSED_TILES[bt].ty,
+                                      FUSED_TILES[bt].cy, bk, bo);
         if (best_tx) *best_tx = FUSED_TILES[bt].tx;
         if (best_ty) *best_ty = FUSED_TILES[bt].ty;
+        if (best_cy) *best_cy = FUSED_TILES[bt].cy;
         if (best_kz) *best_kz = bk;
         if (best_order) *best_order = bo;
         if (best_ms) *best_ms = best;

@@ -81,10 +81,10 @@ __global__ void k_mass_final(const double* __restrict__ partial, int nblocks,
 
 // ------------------------------------------------------------ host side --
 
-int fused_tile_index(int tx, int ty)
+int fused_tile_index(int tx, int ty, int cy)
 {
     for (int q = 0; q < N_FUSED_TILES; q++)
-        if (FUSED_TILES[q].tx == tx && FUSED_TILES[q].ty == ty)
+        if (FUSED_TILES[q].tx == tx && FUSED_TILES[q].ty == ty && FUSED_TILES[q].cy == cy)
             return q;
     return -1;
 }
@@ -102,12 +102,13 @@ void fused_prepare(FusedGrid& G, int tile)
     if (G.kz < 1)
         G.kz = 32;
     const int tx = FUSED_TILES[tile].tx, ty = FUSED_TILES[tile].ty;
+    const int threads = tx * ty / FUSED_TILES[tile].cy;
     G.nbx = (G.nx - 2 + tx - 1) / tx;
     G.nby = (G.ny - 2 + ty - 1) / ty;
     G.nbz = (G.nz - 2 + G.kz - 1) / G.kz;
     G.n_interior_blocks = (long long)G.nbx * G.nby * G.nbz;
     const long long shell = 2LL * G.ny + 2LL * (G.nz - 2) + (long long)(G.nx - 1) * G.nz;
-    G.n_shell_blocks = (shell + tx * ty - 1) / (tx * ty);
+    G.n_shell_blocks = (shell + threads - 1) / threads;
 }
 
 bool fused_general_lambda(const FusedGrid& g)

@@ -25,15 +25,17 @@ struct FusedGrid {
     filt_params p;
 };
 
-// Tile shapes of the fused kernel (columns in x, y); each is a kernel
-// instantiation, and the autotuner picks among them.
+// Tile shapes of the fused kernel: columns in x and y, columns per thread
+// (stacked in y); each is a kernel instantiation, and the autotuner picks
+// among them.
 struct FusedTile {
-    int tx, ty;
+    int tx, ty, cy;
 };
-constexpr FusedTile FUSED_TILES[] = {{16, 16}, {32, 8}, {64, 4}, {32, 16}, {64, 8}};
+constexpr FusedTile FUSED_TILES[] = {{16, 16, 1}, {32, 8, 1},  {64, 4, 1},  {32, 16, 1},
+                                     {64, 8, 1},  {64, 8, 2},  {32, 16, 2}, {64, 16, 2}};
 constexpr int N_FUSED_TILES = sizeof(FUSED_TILES) / sizeof(FUSED_TILES[0]);
 // Index of a tile shape in FUSED_TILES, -1 if none.
-int fused_tile_index(int tx, int ty);
+int fused_tile_index(int tx, int ty, int cy);
 
 // Fill the derived members (reciprocals, tile counts) from nx, ny, nz, kz, p
 // for tile shape tile (an index into FUSED_TILES).

@@ -26,8 +26,8 @@
 //     old S(i',0,1) for k = 0, with i' = min(i, N-2)
 // which is the closed form of the reference's i -> j -> k sweep.
 //
-// Launch: interior tiles of TX x TY columns (one thread per column) march
-// over z-chunks of kz planes.  Per plane, the stage-1 values of the tile and
+// Launch: interior tiles of TX x TY columns (CY columns per thread, stacked in
+// y) march over z-chunks of kz planes.  Per plane, the stage-1 values of the tile and
 // its one-cell ring go to one of two LDS tiles; the raw inputs (state,
 // statics) of the own column two planes ahead and of the ring one plane
 // ahead are in flight while a plane is updated; the own column's plane k-1
@@ -73,6 +73,30 @@ __device__ __forceinline__ void gst(double* base, unsigned off, double v)
     __builtin_nontemporal_store(v, (__attribute__((address_space(1))) double*)((gchar*)base + off));
 }
 
+// Buffer view of an output array of `bytes` bytes.  A buffer store is range
+// checked by the hardware: a lane whose offset lies past the array stores
+// nothing.  The main stores of a plane are therefore issued by every lane,
+// with an offset of DROP where the lane has no cell, and no branch skips
+// them -- which keeps the number of memory operations after the prefetch
+// loads the same on every path, so the wait for those loads does not also
+// wait for the plane's stores.
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+typedef __attribute__((ext_vector_type(2))) unsigned U32x2;
+constexpr unsigned DROP = 0xFFFFFFF8u;  // past every array (arrays stay below 4 GiB)
+constexpr int BUF_WORD3 = 0x00020000;   // raw buffer, 32-bit data format
+constexpr int VMCNT0 = 0x0F70;          // s_waitcnt vmcnt(0), other counters free
+
+__device__ __forceinline__ Rsrc out_rsrc(double* base, unsigned bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(base, 0, (int)bytes, BUF_WORD3);
+}
+
+__device__ __forceinline__ void bst(Rsrc r, unsigned off, double v)
+{
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U32x2, v), r, (int)off, 0,
+                                          2 /* non-temporal */);
+}
+
 // The media table in LDS, one row of FILT_MAX_MEDIA per quantity: the tile
 // loop looks entries up by each cell's medium.
 enum {
@@ -109,29 +133,50 @@ struct Ctx {
         return q;
     }
     __device__ double tab(int row, int md) const { return med[row * FILT_MAX_MEDIA + md]; }
-    __device__ const double* Pin() const { return G()->P[parity]; }
-    __device__ const double* Sin() const { return G()->S[parity]; }
-    __device__ double* Po() const { return G()->P[parity ^ 1]; }
-    __device__ double* So() const { return G()->S[parity ^ 1]; }
+    // the ping-pong buffers, chosen by a uniform select (an index by parity
+    // would turn into a per-lane load of the pointer)
+    __device__ const double* Pin() const { CG* q = G(); return parity ? q->P[1] : q->P[0]; }
+    __device__ const double* Sin() const { CG* q = G(); return parity ? q->S[1] : q->S[0]; }
+    __device__ double* Po() const { CG* q = G(); return parity ? q->P[0] : q->P[1]; }
+    __device__ double* So() const { CG* q = G(); return parity ? q->S[0] : q->S[1]; }
     __device__ unsigned sz() const { CG* q = G(); return 8u * (unsigned)q->nx * (unsigned)q->ny; }
+    __device__ unsigned bytes() const { CG* q = G(); return sz() * (unsigned)q->nz; }
     __device__ unsigned sy() const { return 8u * (unsigned)G()->nx; }
 };
 
-// Inputs of one cell: state, K (when per-cell), porosity (centre cells).
+// Inputs of one cell: state, K (when per-cell), porosity (centre cells),
+// the medium word as loaded (medium() gives the cell's medium).
 struct Raw {
     double P, S, K, phi;
-    int md;
+    int mdw;
 };
 
+// The cell's medium: 0 unless the grid has a media field with several media
+// (the select is made where the medium is used, not where it is loaded, so
+// that the load stays in flight).
+__device__ __forceinline__ int medium(const Ctx& c, const Raw& q)
+{
+    CG* G = c.G();
+    return (G->media && G->p.n_media > 1) ? q.mdw : 0;
+}
+
+// Every call issues the same loads whichever fields the grid has: a field
+// that is absent is read from the pressure array instead and the value is not
+// used.  No branch around a load, so the memory operations in flight are the
+// same on every path through the plane loop and the compiler's waits for them
+// stay exact (a skipped load makes it wait for everything outstanding).
 __device__ __forceinline__ Raw load_raw(const Ctx& c, unsigned off, bool centre)
 {
     CG* G = c.G();
+    const double* const Pin = c.parity ? G->P[1] : G->P[0];
+    const bool media = G->media && G->p.n_media > 1;
     Raw r;
-    r.md = (G->media && G->p.n_media > 1) ? gld(G->media, off >> 1) : 0;
-    r.P = gld(G->P[c.parity], off);
-    r.S = gld(G->S[c.parity], off);
-    r.K = G->perm ? gld(G->perm, off) : 0.0;
-    r.phi = (centre && G->poro) ? gld(G->poro, off) : 0.0;
+    r.mdw = gld(media ? G->media : (const int*)Pin, off >> 1);
+    r.P = gld(Pin, off);
+    r.S = gld(c.parity ? G->S[1] : G->S[0], off);
+    r.K = gld(G->perm ? G->perm : Pin, off);
+    if (centre)
+        r.phi = gld(G->poro ? G->poro : Pin, off);
     return r;
 }
 
@@ -151,13 +196,19 @@ struct FormFP64 {
     static constexpr bool TOP64 = false;
     template <int SN, int TX>
     static constexpr int tile_bytes() { return SN * SLOT_BYTES; }
-    static constexpr int WAVES = 4;  // waves per SIMD the kernel is built for
+    // waves per SIMD the kernel is built for, with one or two columns per
+    // thread
+    static constexpr int WAVES = 4;
+    static constexpr int WAVES2 = 3;
+    // a plane's inputs are loaded at the top of its iteration: with them in
+    // flight across the update this form needs more than the 128 registers
+    // of four waves per SIMD (measured slower at three waves)
     static constexpr bool PIPELINED = false;
 
     __device__ static Val eval(const Ctx& c, const Raw& q)
     {
         const auto& p = c.G()->p;
-        const int md = q.md;
+        const int md = medium(c, q);
         const double K = c.G()->perm ? q.K : c.tab(M_K, md);
         const double swr = c.tab(M_SWR, md);
         const DivBy d1{1. - swr, c.tab(M_DY, md)};
@@ -227,7 +278,7 @@ struct FormFP64 {
                                      L[QR1 * SN + ym], L[QX1 * SN + yp], L[QR1 * SN + yp], dy);
         const double g2 = upwind_div(dy2 + r2c * g, dy1 + r2c * g, zc.X2, r2c, L[QX2 * SN + ym],
                                      L[QR2 * SN + ym], L[QX2 * SN + yp], L[QR2 * SN + yp], dy);
-        const int md = q.md;
+        const int md = medium(c, q);
         const double phi = c.G()->poro ? q.phi : c.tab(M_M, md);
         const double A1 = roS1 - (p.dt / phi) * (f1 + f2 + f3);
         const double A2 = roS2 - (p.dt / phi) * (g1 + g2 + g3);
@@ -251,7 +302,7 @@ template <bool GEN, class R>
 __device__ __forceinline__ void mobilities(const Ctx& c, const Raw& q, R& L1, R& L2)
 {
     CG& G = *c.G();
-    const int md = q.md;
+    const int md = medium(c, q);
     const double K = c.G()->perm ? q.K : c.tab(M_K, md);
     const double P = q.P;
     double k1, k2d;
@@ -345,7 +396,7 @@ __device__ __forceinline__ void newton_fast(const Ctx& c, int md, double P0, dou
 // dt / phi with an FP64 refined reciprocal
 __device__ __forceinline__ double dt_over_phi(const Ctx& c, const Raw& q)
 {
-    const double phi = c.G()->poro ? q.phi : c.tab(M_M, q.md);
+    const double phi = c.G()->poro ? q.phi : c.tab(M_M, medium(c, q));
     return c.G()->p.dt * rcp1(phi);
 }
 
@@ -367,6 +418,7 @@ struct FormFast {
     template <int SN, int TX>
     static constexpr int tile_bytes() { return SN * SLOT_BYTES; }
     static constexpr int WAVES = GEN ? 3 : 4;
+    static constexpr int WAVES2 = GEN ? 2 : 3;
     static constexpr bool PIPELINED = true;
 
     __device__ static Val eval(const Ctx& c, const Raw& q)
@@ -415,7 +467,7 @@ struct FormFast {
         const double dtphi = dt_over_phi(c, q);
         const double A1 = r1c * (1 - S) - dtphi * F1;
         const double A2 = r2c * S - dtphi * F2;
-        newton_fast(c, q.md, p1, S, A1, A2, nP, nS);
+        newton_fast(c, medium(c, q), p1, S, A1, A2, nP, nS);
     }
 };
 
@@ -448,6 +500,7 @@ struct FormMixed {
     };
     static constexpr int SLOT_BYTES = 16;
     static constexpr int WAVES = GEN ? 3 : 4;
+    static constexpr int WAVES2 = GEN ? 2 : 3;
     static constexpr bool PIPELINED = true;
     static constexpr bool TOP64 = true;
     // per tile: the slots, then FP64 L1 and L2 of the j = 0 ring row
@@ -532,23 +585,22 @@ struct FormMixed {
         const double dtphi = dt_over_phi(c, q);
         const double A1 = r1c * (1 - S) - dtphi * F1;
         const double A2 = r2c * S - dtphi * F2;
-        newton_fast(c, q.md, p1, S, A1, A2, nP, nS);
+        newton_fast(c, medium(c, q), p1, S, A1, A2, nP, nS);
     }
 };
 
 // -------------------------------------------------------------- writing --
 
-// Output of interior cell (i,j,k) at byte offset off and the shell cells
-// that copy it.  i1: i == 1; lastcol: i == nx-2; top: j == ny-2 (with Pt0 =
-// ro1(i,1,0)*g*h2 of the input state); kl, kh: k == 1, k == nz-2.
-__device__ __forceinline__ void write_cell(const Ctx& c, unsigned off, bool i1, bool lastcol,
-                                           bool top, double Pt0, bool kl, bool kh, double nP,
-                                           double nS)
+// The shell cells that copy interior cell (i,j,k) (byte offset off), whose
+// own output the plane's buffer stores write.  i1: i == 1; lastcol: i ==
+// nx-2; top: j == ny-2 (with Pt0 = ro1(i,1,0)*g*h2 of the input state); kl,
+// kh: k == 1, k == nz-2.
+__device__ __forceinline__ void write_copies(const Ctx& c, unsigned off, bool i1, bool lastcol,
+                                             bool top, double Pt0, bool kl, bool kh, double nP,
+                                             double nS)
 {
     double* const Po = c.Po();
     double* const So = c.So();
-    gst(Po, off, nP);
-    gst(So, off, nS);
     if (i1) {
         gst(Po, off - 8, nP);
         gst(So, off - 8, nS);
@@ -618,19 +670,56 @@ __device__ inline void shell_cell(const FusedGrid& G, const double* Pin_, const 
     const int ii = i < nx - 1 ? i : nx - 2;
     const long long c = i + k * sz;
     Po[c] = G.p.P_atm;
-    So[c] = (k == 0) ? Sin[ii + sz] : (in_patch(ii, k < nz - 1 ? k : nz - 2, G.p) ? G.p.S2_gr : 0.0);
+    So[c] = (k == 0) ? Sin[ii + sz]
+                     : (in_patch(ii, k < nz - 1 ? k : nz - 2, G.p) ? G.p.S2_gr : 0.0);
+}
+
+// Update of a thread's CY cells of plane k and their outputs: the cells'
+// own values by buffer stores every lane issues (DROP where a lane has no
+// interior cell), the shell copies of edge cells after them.
+template <class F, int SN, int SX, int TX, int CY>
+__device__ __forceinline__ void write_plane(const Ctx& c, const typename F::Val* zc,
+                                            const char* Lm, const typename F::Val* zp,
+                                            const char* L, const int* s, const Raw* qc,
+                                            const bool* jtop, int tx, const bool* inner,
+                                            const unsigned* col, int k, int nz, bool i1,
+                                            bool lastcol, const bool* top, const double* Pt0)
+{
+    double nP[CY], nS[CY];
+#pragma unroll
+    for (int q = 0; q < CY; q++)
+        if (inner[q])
+            F::template update<SN, SX, TX>(c, zc[q], Lm, zp[q], L, s[q], qc[q], jtop[q], tx,
+                                           nP[q], nS[q]);
+    const unsigned bytes = c.bytes(), plane = (unsigned)k * c.sz();
+    const Rsrc rP = out_rsrc(c.Po(), bytes), rS = out_rsrc(c.So(), bytes);
+#pragma unroll
+    for (int q = 0; q < CY; q++) {
+        const unsigned off = inner[q] ? col[q] + plane : DROP;
+        bst(rP, off, nP[q]);
+        bst(rS, off, nS[q]);
+    }
+    const bool kl = k == 1, kh = k == nz - 2;
+#pragma unroll
+    for (int q = 0; q < CY; q++)
+        if (inner[q] && (i1 || lastcol || top[q] || kl || kh))
+            write_copies(c, col[q] + plane, i1, lastcol, top[q], Pt0[q], kl, kh, nP[q], nS[q]);
 }
 
 // ------------------------------------------------------------- the tile --
-template <class F, int TX, int TY>
+// A tile of TX x TY columns; each thread owns CY of them, stacked in y
+// (register blocking: the per-plane work common to a thread's cells -- loop
+// control, descriptor reads, the barrier -- is paid once for CY cells, and
+// their independent arithmetic interleaves).
+template <class F, int TX, int TY, int CY>
 __device__ __forceinline__ void march(const Ctx& c, const FusedGrid& G, unsigned lb, char* lds)
 {
-    constexpr int NT = TX * TY;
+    constexpr int NT = TX * TY / CY;  // threads
     constexpr int NW = NT / 64;
     constexpr int SX = TX + 2;
     constexpr int SN = SX * (TY + 2);
     constexpr int RING = 2 * TX + 2 * TY;
-    static_assert(NT % 64 == 0 && RING <= NT, "tile shape");
+    static_assert(NT % 64 == 0 && RING <= NT && TY % CY == 0, "tile shape");
     const unsigned nbx = (unsigned)G.nbx, nby = (unsigned)G.nby;
     const int bx = (int)(lb % nbx);
     const unsigned r = lb / nbx;
@@ -642,10 +731,28 @@ __device__ __forceinline__ void march(const Ctx& c, const FusedGrid& G, unsigned
     if (kb >= ke)
         return;  // uniform over the block
     const int t = threadIdx.x;
-    const int tx = t % TX, ty = t / TX;
-    const int i = i0 + tx, j = j0 + ty;
-    const bool in_grid = i < nx && j < ny;
-    const bool inner = i <= nx - 2 && j <= ny - 2;
+    const int tx = t % TX, ty = t / TX;  // this thread: rows CY*ty .. CY*ty+CY-1
+    const int i = i0 + tx;
+    bool in_grid[CY], inner[CY], top[CY], jtop[CY];
+    unsigned col[CY];
+    int s[CY];
+    double Pt0[CY];
+#pragma unroll
+    for (int q = 0; q < CY; q++) {
+        const int j = j0 + CY * ty + q;
+        in_grid[q] = i < nx && j < ny;
+        inner[q] = i <= nx - 2 && j <= ny - 2;
+        top[q] = j == ny - 2;   // shell copies of row M-1 (loop-invariant)
+        jtop[q] = j == 1;       // the row next to the top boundary
+        col[q] = 8u * (unsigned)(i + j * nx);
+        s[q] = (CY * ty + q + 1) * SX + (tx + 1);
+        Pt0[q] = 0.0;
+        if (inner[q] && top[q]) {
+            const double P10 = gld(c.Pin(), 8u * (unsigned)(i + nx));  // (i, 1, 0)
+            CG& Gc = *c.G();
+            Pt0[q] = ro1_of(P10, Gc.p) * Gc.p.g_const * Gc.p.h2;
+        }
+    }
     // ring cell h of this thread: waves w0, w0+1, ... (mod NW) take the ring
     const int w0 = (int)(blockIdx.x % NW);
     const int h = ((((t >> 6) - w0) + NW) % NW) * 64 + (t & 63);
@@ -655,42 +762,38 @@ __device__ __forceinline__ void march(const Ctx& c, const FusedGrid& G, unsigned
     else if (h < 2 * TX + TY) { rx = -1; ry = h - 2 * TX; }
     else if (h < RING) { rx = TX; ry = h - 2 * TX - TY; }
     const bool ring = h < RING && i0 + rx < nx && j0 + ry < ny;
-    const unsigned col = 8u * (unsigned)(i + j * nx);
     const unsigned rcol = 8u * (unsigned)((i0 + rx) + (j0 + ry) * nx);
-    const int s = (ty + 1) * SX + (tx + 1), rs = (ry + 1) * SX + (rx + 1);
-    // shell copies of this column (loop-invariant flags)
-    const bool i1 = i == 1, lastcol = i == nx - 2, top = j == ny - 2;
-    double Pt0 = 0.0;
-    if (inner && top) {
-        const double P10 = gld(c.Pin(), 8u * (unsigned)(i + nx));  // (i, 1, 0)
-        CG& Gc = *c.G();
-        Pt0 = ro1_of(P10, Gc.p) * Gc.p.g_const * Gc.p.h2;
-    }
+    const int rs = (ry + 1) * SX + (rx + 1);
+    const bool i1 = i == 1, lastcol = i == nx - 2;
     char* const T0 = lds;
     char* const T1 = lds + F::template tile_bytes<SN, TX>();
-    // the top tiles' ring cells of row j = 0, and the cells of row j = 1
+    // the top tiles' ring cells of row j = 0
     const bool top_ring = ring && ry == -1 && j0 == 1;
-    const bool jtop = j == 1;
     using Val = typename F::Val;
-    Val zc{}, zp{};
+    Val zc[CY], zp[CY];
     if constexpr (!F::PIPELINED) {
         // Loads of a plane at the top of its iteration: fewer registers live
         // across the update, for a form at the edge of an occupancy step.
-        Raw qc{};
-        if (in_grid) {
-            F::template put<SN>(T1, s, c, F::eval(c, load_raw(c, col + (kb - 1) * c.sz(), false)));
-            qc = load_raw(c, col + kb * c.sz(), true);
-            zc = F::eval(c, qc);
-        }
+        Raw qc[CY];
+#pragma unroll
+        for (int q = 0; q < CY; q++)
+            if (in_grid[q]) {
+                F::template put<SN>(T1, s[q], c,
+                                    F::eval(c, load_raw(c, col[q] + (kb - 1) * c.sz(), false)));
+                qc[q] = load_raw(c, col[q] + kb * c.sz(), true);
+                zc[q] = F::eval(c, qc[q]);
+            }
         for (int k = kb; k < ke; k++) {
             const bool odd = ((k - kb) & 1) != 0;
             char* const L = odd ? T1 : T0;
             const char* const Lm = odd ? T0 : T1;
-            Raw qp{};
-            if (in_grid) {
-                qp = load_raw(c, col + (k + 1) * c.sz(), true);  // k + 1 <= ke <= nz - 1
-                F::template put<SN>(L, s, c, zc);
-            }
+            Raw qp[CY];
+#pragma unroll
+            for (int q = 0; q < CY; q++)
+                if (in_grid[q]) {
+                    qp[q] = load_raw(c, col[q] + (k + 1) * c.sz(), true);  // k+1 <= ke <= nz-1
+                    F::template put<SN>(L, s[q], c, zc[q]);
+                }
             if (ring) {
                 const Raw rr = load_raw(c, rcol + k * c.sz(), false);
                 F::template put<SN>(L, rs, c, F::eval(c, rr));
@@ -698,68 +801,103 @@ __device__ __forceinline__ void march(const Ctx& c, const FusedGrid& G, unsigned
                     if (top_ring)
                         F::template put_top<SN, TX>(L, rx, c, rr);
             }
-            if (in_grid)
-                zp = F::eval(c, qp);
+#pragma unroll
+            for (int q = 0; q < CY; q++)
+                if (in_grid[q])
+                    zp[q] = F::eval(c, qp[q]);
             __syncthreads();
-            if (inner) {
-                double nP, nS;
-                F::template update<SN, SX, TX>(c, zc, Lm, zp, L, s, qc, jtop, tx, nP, nS);
-                write_cell(c, col + k * c.sz(), i1, lastcol, top, Pt0, k == 1, k == nz - 2, nP,
-                           nS);
+            write_plane<F, SN, SX, TX, CY>(c, zc, Lm, zp, L, s, qc, jtop, tx, inner, col, k, nz,
+                                           i1, lastcol, top, Pt0);
+#pragma unroll
+            for (int q = 0; q < CY; q++) {
+                zc[q] = zp[q];
+                qc[q] = qp[q];
             }
-            zc = zp;
-            qc = qp;
         }
         return;
-    }
-    Raw qc{}, qp{}, qn{}, rq{};
-    if (in_grid) {
-        // plane kb-1 goes to the own slot of the tile plane kb reads as k-1
-        F::template put<SN>(T1, s, c, F::eval(c, load_raw(c, col + (kb - 1) * c.sz(), false)));
-        qc = load_raw(c, col + kb * c.sz(), true);
-        zc = F::eval(c, qc);
-        qp = load_raw(c, col + (kb + 1) * c.sz(), true);  // kb + 1 <= ke <= nz - 1
-    }
-    if (ring)
-        rq = load_raw(c, rcol + kb * c.sz(), false);
-    for (int k = kb; k < ke; k++) {
-        const bool odd = ((k - kb) & 1) != 0;
-        char* const L = odd ? T1 : T0;
-        const char* const Lm = odd ? T0 : T1;
-        if (in_grid) {
-            zp = F::eval(c, qp);
-            F::template put<SN>(L, s, c, zc);
+    } else {
+        // Pipelined: qp and rq hold loads in flight across the loop's back
+        // edge; each is consumed at the top of the next iteration and only
+        // then reloaded, never copied while in flight (a copy would make the
+        // wave wait for every memory operation it has outstanding, stores
+        // included, once per plane)
+        Raw qc[CY], qp[CY], rq{};
+#pragma unroll
+        for (int q = 0; q < CY; q++)
+            if (in_grid[q]) {
+                // plane kb-1 goes to the own slot of the tile plane kb reads
+                // as k-1
+                F::template put<SN>(
+                    T1, s[q], c, F::eval(c, load_raw(c, col[q] + (kb - 1) * c.sz(), false)));
+                qc[q] = load_raw(c, col[q] + kb * c.sz(), true);
+                zc[q] = F::eval(c, qc[q]);
+            }
+        // everything loaded so far has arrived before the loop's loads are
+        // issued: the loop is entered with only those in flight (a field of qc
+        // that eval does not read would otherwise still be pending at the loop
+        // header, and the compiler's wait for it would then also cover loads
+        // issued in every later plane)
+        __builtin_amdgcn_s_waitcnt(VMCNT0);
+#pragma unroll
+        for (int q = 0; q < CY; q++)
+            if (in_grid[q])
+                qp[q] = load_raw(c, col[q] + (kb + 1) * c.sz(), true);  // kb+1 <= ke <= nz-1
+        if (ring)
+            rq = load_raw(c, rcol + kb * c.sz(), false);
+        {
+            // two stores that write nothing: the loop is entered with the same
+            // memory operations behind the prefetch loads as at its back edge,
+            // so the first plane's wait for them has the same count as later
+            // planes' (without them, that count would be zero for all planes)
+            const Rsrc rP = out_rsrc(c.Po(), c.bytes()), rS = out_rsrc(c.So(), c.bytes());
+            bst(rP, DROP, 0.0);
+            bst(rS, DROP, 0.0);
         }
-        if (ring) {
-            F::template put<SN>(L, rs, c, F::eval(c, rq));
-            if constexpr (F::TOP64)
-                if (top_ring)
-                    F::template put_top<SN, TX>(L, rx, c, rq);
+        for (int k = kb; k < ke; k++) {
+            const bool odd = ((k - kb) & 1) != 0;
+            char* const L = odd ? T1 : T0;
+            const char* const Lm = odd ? T0 : T1;
+            Raw qnext[CY];  // the next plane's centre inputs, arrived
+#pragma unroll
+            for (int q = 0; q < CY; q++)
+                if (in_grid[q]) {
+                    zp[q] = F::eval(c, qp[q]);
+                    qnext[q] = qp[q];
+                    F::template put<SN>(L, s[q], c, zc[q]);
+                }
+            if (ring) {
+                F::template put<SN>(L, rs, c, F::eval(c, rq));
+                if constexpr (F::TOP64)
+                    if (top_ring)
+                        F::template put_top<SN, TX>(L, rx, c, rq);
+            }
+            if (k + 1 < ke) {  // the next plane is updated: fetch its inputs
+#pragma unroll
+                for (int q = 0; q < CY; q++)
+                    if (in_grid[q])
+                        qp[q] = load_raw(c, col[q] + (k + 2) * c.sz(), true);  // k+2 <= nz-1
+                if (ring)
+                    rq = load_raw(c, rcol + (k + 1) * c.sz(), false);
+            }
+            __syncthreads();
+            write_plane<F, SN, SX, TX, CY>(c, zc, Lm, zp, L, s, qc, jtop, tx, inner, col, k, nz,
+                                           i1, lastcol, top, Pt0);
+#pragma unroll
+            for (int q = 0; q < CY; q++) {
+                zc[q] = zp[q];
+                qc[q] = qnext[q];
+            }
         }
-        if (k + 1 < ke) {  // the next plane is updated: fetch its inputs
-            if (in_grid)
-                qn = load_raw(c, col + (k + 2) * c.sz(), true);  // k + 2 <= ke <= nz - 1
-            if (ring)
-                rq = load_raw(c, rcol + (k + 1) * c.sz(), false);
-        }
-        __syncthreads();
-        if (inner) {
-            double nP, nS;
-            F::template update<SN, SX, TX>(c, zc, Lm, zp, L, s, qc, jtop, tx, nP, nS);
-            write_cell(c, col + k * c.sz(), i1, lastcol, top, Pt0, k == 1, k == nz - 2, nP, nS);
-        }
-        zc = zp;
-        qc = qp;
-        qp = qn;
     }
 }
 
-template <class F, int TX, int TY>
-__global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(F::WAVES))) void k_fused(const FusedGrid* __restrict__ grids,
-                                                  const int* __restrict__ block_grid,
-                                                  long long total_blocks, int parity, int order)
+template <class F, int TX, int TY, int CY>
+__global__ __launch_bounds__(TX* TY / CY) __attribute__((amdgpu_waves_per_eu(
+    CY == 1 ? F::WAVES : F::WAVES2))) void k_fused(
+    const FusedGrid* __restrict__ grids, const int* __restrict__ block_grid, long long total_blocks,
+    int parity, int order)
 {
-    constexpr int NT = TX * TY;
+    constexpr int NT = TX * TY / CY;
     constexpr int SN = (TX + 2) * (TY + 2);
     __shared__ __attribute__((aligned(16))) char lds[2 * F::template tile_bytes<SN, TX>()];
     __shared__ double med[M_ROWS * FILT_MAX_MEDIA];
@@ -787,9 +925,9 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(F::WAVES
             default: v = (2. + p.lambda[q]) / p.lambda[q]; break;
             }
             med[t] = v;
-        } else if (t >= 128 && t < 133) {
+        } else if (t >= 80 && t < 85) {
             const filt_params& p = G.p;
-            const int q = t - 128;
+            const int q = t - 80;
             rh[q] = refined_reciprocal(q == 0 ? p.h1 : q == 1 ? p.h2 : q == 2 ? p.h3
                                        : q == 3 ? p.mu1 : p.mu2);
         }
@@ -799,7 +937,7 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(F::WAVES
         c.med = med;
         c.rh = rh;
         c.parity = parity;
-        march<F, TX, TY>(c, G, (unsigned)lb, lds);
+        march<F, TX, TY, CY>(c, G, (unsigned)lb, lds);
         return;
     }
     const long long q = (lb - G.n_interior_blocks) * NT + t;
@@ -812,22 +950,26 @@ int launch_form(const FusedGrid* g, const int* bg, long long total, int parity, 
                 int tile, int order, hipStream_t s)
 {
     const dim3 grid((unsigned)total);
-#define FILT_FUSED_CASE(TXV, TYV)                                                            \
-    if (FUSED_TILES[tile].tx == TXV && FUSED_TILES[tile].ty == TYV) {                        \
-        if (general)                                                                         \
-            hipLaunchKernelGGL((k_fused<F<true>, TXV, TYV>), grid, dim3(TXV * TYV), 0, s, g, \
-                               bg, total, parity, order);                                    \
-        else                                                                                 \
-            hipLaunchKernelGGL((k_fused<F<false>, TXV, TYV>), grid, dim3(TXV * TYV), 0, s, g, \
-                               bg, total, parity, order);                                    \
-        FILT_LAUNCH_CHECK("k_fused");                                                        \
-        return FILT_OK;                                                                      \
+    const FusedTile& ft = FUSED_TILES[tile];
+#define FILT_FUSED_CASE(TXV, TYV, CYV)                                                         \
+    if (ft.tx == TXV && ft.ty == TYV && ft.cy == CYV) {                                        \
+        if (general)                                                                           \
+            hipLaunchKernelGGL((k_fused<F<true>, TXV, TYV, CYV>), grid, dim3(TXV * TYV / CYV), \
+                               0, s, g, bg, total, parity, order);                             \
+        else                                                                                   \
+            hipLaunchKernelGGL((k_fused<F<false>, TXV, TYV, CYV>), grid,                       \
+                               dim3(TXV * TYV / CYV), 0, s, g, bg, total, parity, order);      \
+        FILT_LAUNCH_CHECK("k_fused");                                                          \
+        return FILT_OK;                                                                        \
     }
-    FILT_FUSED_CASE(16, 16)
-    FILT_FUSED_CASE(32, 8)
-    FILT_FUSED_CASE(64, 4)
-    FILT_FUSED_CASE(32, 16)
-    FILT_FUSED_CASE(64, 8)
+    FILT_FUSED_CASE(16, 16, 1)
+    FILT_FUSED_CASE(32, 8, 1)
+    FILT_FUSED_CASE(64, 4, 1)
+    FILT_FUSED_CASE(32, 16, 1)
+    FILT_FUSED_CASE(64, 8, 1)
+    FILT_FUSED_CASE(64, 8, 2)
+    FILT_FUSED_CASE(32, 16, 2)
+    FILT_FUSED_CASE(64, 16, 2)
 #undef FILT_FUSED_CASE
     return set_error(FILT_ERR_INVALID, "unknown fused tile %d", tile);
 }

@@ -197,27 +197,31 @@ class Ensemble:
     def steps(self, mid):
         return self.lib.filt_ensemble_steps(self.h, mid)
 
-    def set_tiling(self, kz, order=0, tile=(0, 0)):
-        """tile = (tx, ty) columns per tile ((0, 0): 16x16), kz planes per
-        z-chunk, order 0 XCD-grouped / 1 hardware order."""
-        check(self.lib, self.lib.filt_ensemble_set_tiling(self.h, tile[0], tile[1], kz, order),
+    def set_tiling(self, kz, order=0, tile=(0, 0, 1)):
+        """tile = (tx, ty[, cy]): columns per tile and columns per thread
+        ((0, 0): 16x16, one column per thread); kz planes per z-chunk; order
+        0 XCD-grouped / 1 hardware order."""
+        tx, ty = tile[0], tile[1]
+        cy = tile[2] if len(tile) > 2 else 1
+        check(self.lib, self.lib.filt_ensemble_set_tiling(self.h, tx, ty, cy, kz, order),
               "set_tiling")
 
     def tiling(self):
-        v = [ctypes.c_int() for _ in range(4)]
+        v = [ctypes.c_int() for _ in range(5)]
         check(self.lib, self.lib.filt_ensemble_get_tiling(self.h, *[ctypes.byref(x) for x in v]),
               "get_tiling")
-        return {"tile": [v[0].value, v[1].value], "kz": v[2].value, "order": v[3].value}
+        return {"tile": [v[0].value, v[1].value, v[2].value], "kz": v[3].value,
+                "order": v[4].value}
 
     def autotune(self, reps=3):
         """Search tile shape x kz x order; returns the chosen tiling with its
         ms per step."""
-        tx, ty, kz, order = (ctypes.c_int() for _ in range(4))
+        tx, ty, cy, kz, order = (ctypes.c_int() for _ in range(5))
         ms = ctypes.c_double()
         check(self.lib, self.lib.filt_ensemble_autotune(
-            self.h, reps, ctypes.byref(tx), ctypes.byref(ty), ctypes.byref(kz),
+            self.h, reps, ctypes.byref(tx), ctypes.byref(ty), ctypes.byref(cy), ctypes.byref(kz),
             ctypes.byref(order), ctypes.byref(ms)), "filt_ensemble_autotune")
-        return {"tile": [tx.value, ty.value], "kz": kz.value, "order": order.value,
+        return {"tile": [tx.value, ty.value, cy.value], "kz": kz.value, "order": order.value,
                 "ms_per_step": ms.value}
 
     def apply_tiling(self, t):

@@ -29,12 +29,12 @@ def plain_run(hip, p, g, steps):
 FP64, FAST, MIXED = 0, 1, 2
 
 
-def fused_run(hip, p, g, steps, precision=0, graph=False, kz=None, tile=(0, 0)):
+def fused_run(hip, p, g, steps, precision=0, graph=False, kz=None, tile=(0, 0, 1)):
     e = hip.ensemble(precision)
     try:
         mid = e.add(p, g["P1"], g["S2"], g.a.get("perm"), g.a.get("poro"),
                     g.a["media"] if p.n_media > 1 else None)
-        if kz or tile != (0, 0):
+        if kz or tile != (0, 0, 1):
             kz_, order = kz or (32, 0)
             e.set_tiling(kz_, order, tile)
         if graph:
@@ -114,7 +114,8 @@ def errors_vs_oracle(P, S, m, g, mo):
             "total_mass_rel": abs(m[0] + m[1] - mo[0] - mo[1]) / (mo[0] + mo[1])}
 
 
-TILES = [(16, 16), (32, 8), (64, 4), (32, 16), (64, 8)]
+TILES = [(16, 16, 1), (32, 8, 1), (64, 4, 1), (32, 16, 1), (64, 8, 1), (64, 8, 2), (32, 16, 2),
+         (64, 16, 2)]
 
 
 @pytest.mark.parametrize("precision", [FP64, FAST, MIXED])
