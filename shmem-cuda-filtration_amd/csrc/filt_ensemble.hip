@@ -45,6 +45,7 @@ struct filt_ensemble {
     size_t d_block_cap = 0;
     long long total_blocks = 0;
     bool general_lambda = false;  // some member has a medium with lambda != 2
+    bool cell_fields = false;     // every member has per-cell K, porosity, one medium
     double* d_partial = nullptr;
     size_t d_partial_cap = 0;
     double* d_mass = nullptr;
@@ -92,8 +93,11 @@ int sync_tables(filt_ensemble* e)
     }
     e->total_blocks = first;
     e->general_lambda = false;
-    for (const FusedGrid& g : gs)
+    e->cell_fields = !gs.empty();
+    for (const FusedGrid& g : gs) {
         e->general_lambda = e->general_lambda || fused_general_lambda(g);
+        e->cell_fields = e->cell_fields && fused_cell_fields(g);
+    }
     // launches already enqueued read the current tables: let them finish
     // before the tables are rewritten or reallocated
     FILT_HIP_TRY(hipStreamSynchronize(stream()));
@@ -342,7 +346,7 @@ int filt_ensemble_step(filt_ensemble* e, int nsteps)
             return rc;
         for (int s = 0; s < nsteps; s++) {
             rc = fused_launch(e->d_grids, e->d_block_grid, e->total_blocks, e->parity, e->precision,
-                              e->general_lambda, e->tile, e->order, stream());
+                              e->general_lambda, e->cell_fields, e->tile, e->order, stream());
             if (rc)
                 return rc;
             e->parity ^= 1;
@@ -368,7 +372,7 @@ int filt_ensemble_capture(filt_ensemble* e, int nsteps)
             int par = start;
             for (int s = 0; s < nsteps && rc == FILT_OK; s++) {
                 rc = fused_launch(e->d_grids, e->d_block_grid, e->total_blocks, par, e->precision,
-                                  e->general_lambda, e->tile, e->order, stream());
+                                  e->general_lambda, e->cell_fields, e->tile, e->order, stream());
                 par ^= 1;
             }
             hipGraph_t g = nullptr;

@@ -99,6 +99,13 @@ void fused_prepare(FusedGrid& G, int tile)
     G.inv_mu2 = 1.0 / p.mu2;
     for (int q = 0; q < FILT_MAX_MEDIA; q++)
         G.inv_1mswr[q] = 1.0 / (1.0 - p.S_wr[q]);
+    // the same products, in the same order, as the kernels would form (this
+    // file is built without FP contraction)
+    G.nq_a1 = p.ro01 * p.beta1;
+    G.nq_c1 = p.ro02 * p.beta2;
+    G.nq_al4 = 4.0 * (G.nq_a1 * G.nq_c1);
+    G.nq_be0 = p.ro01 * G.nq_c1 + G.nq_a1 * p.ro02;
+    G.nq_ga0 = p.ro01 * p.ro02;
     if (G.kz < 1)
         G.kz = 32;
     const int tx = FUSED_TILES[tile].tx, ty = FUSED_TILES[tile].ty;
@@ -120,6 +127,11 @@ bool fused_general_lambda(const FusedGrid& g)
     return false;
 }
 
+bool fused_cell_fields(const FusedGrid& g)
+{
+    return g.perm && g.poro && !(g.media && g.p.n_media > 1);
+}
+
 bool fused_divisors_ok(const filt_params& p)
 {
     if (!(divby_ok(p.h1) && divby_ok(p.h2) && divby_ok(p.h3) && divby_ok(p.mu1) &&
@@ -133,8 +145,8 @@ bool fused_divisors_ok(const filt_params& p)
 }
 
 int fused_launch(const FusedGrid* dev_grids, const int* dev_block_grid, long long total_blocks,
-                 int parity, int precision, bool general_lambda, int tile, int order,
-                 hipStream_t s)
+                 int parity, int precision, bool general_lambda, bool cell_fields, int tile,
+                 int order, hipStream_t s)
 {
     if (total_blocks <= 0)
         return FILT_OK;
@@ -145,13 +157,13 @@ int fused_launch(const FusedGrid* dev_grids, const int* dev_block_grid, long lon
     switch (precision) {
     case FILT_PREC_FP64:
         return fused_launch_fp64(dev_grids, dev_block_grid, total_blocks, parity, general_lambda,
-                                 tile, order, s);
+                                 cell_fields, tile, order, s);
     case FILT_PREC_FAST:
         return fused_launch_fast(dev_grids, dev_block_grid, total_blocks, parity, general_lambda,
-                                 tile, order, s);
+                                 cell_fields, tile, order, s);
     case FILT_PREC_MIXED:
         return fused_launch_mixed(dev_grids, dev_block_grid, total_blocks, parity,
-                                  general_lambda, tile, order, s);
+                                  general_lambda, cell_fields, tile, order, s);
     }
     return set_error(FILT_ERR_INVALID, "unknown precision %d", precision);
 }

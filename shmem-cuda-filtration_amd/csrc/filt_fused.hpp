@@ -21,6 +21,10 @@ struct FusedGrid {
     long long first_block;       // global index of this grid's first block
     double inv_h[3];             // 1/h1, 1/h2, 1/h3 (fast and mixed forms)
     double inv_mu1, inv_mu2;     // fast and mixed forms
+    // the closed-form Newton's constant coefficients (fast and mixed forms):
+    // ro01*beta1, ro02*beta2, 4*(ro01*beta1)*(ro02*beta2),
+    // ro01*(ro02*beta2) + (ro01*beta1)*ro02, ro01*ro02
+    double nq_a1, nq_c1, nq_al4, nq_be0, nq_ga0;
     double inv_1mswr[FILT_MAX_MEDIA];
     filt_params p;
 };
@@ -45,25 +49,30 @@ void fused_prepare(FusedGrid& G, int tile);
 // general Brooks-Corey powers).
 bool fused_general_lambda(const FusedGrid& g);
 
+// Whether a grid has per-cell K and porosity and a single medium (the kernels
+// then read both fields unconditionally and look up no medium).
+bool fused_cell_fields(const FusedGrid& g);
+
 // Whether the FP64 form can divide by the grid's constant divisors (h1, h2,
 // h3, mu1, mu2, 1 - S_wr) through DivBy: each in [2^-100, 2^101].
 bool fused_divisors_ok(const filt_params& p);
 
 // One step of every grid in the batch: blocks [0, total_blocks) map to grids
 // through dev_block_grid; reads buffer `parity`, writes buffer parity ^ 1.
-// general_lambda: some grid has fused_general_lambda.
+// general_lambda: some grid has fused_general_lambda; cell_fields: every grid
+// has fused_cell_fields.
 int fused_launch(const FusedGrid* dev_grids, const int* dev_block_grid, long long total_blocks,
-                 int parity, int precision, bool general_lambda, int tile, int order,
-                 hipStream_t s);
+                 int parity, int precision, bool general_lambda, bool cell_fields, int tile,
+                 int order, hipStream_t s);
 
 // Per-form launchers (filt_fused_fp64.hip, filt_fused_fast.hip,
 // filt_fused_mixed.hip).
 int fused_launch_fp64(const FusedGrid* g, const int* bg, long long total, int parity, bool general,
-                      int tile, int order, hipStream_t s);
+                      bool fields, int tile, int order, hipStream_t s);
 int fused_launch_fast(const FusedGrid* g, const int* bg, long long total, int parity, bool general,
-                      int tile, int order, hipStream_t s);
+                      bool fields, int tile, int order, hipStream_t s);
 int fused_launch_mixed(const FusedGrid* g, const int* bg, long long total, int parity,
-                       bool general, int tile, int order, hipStream_t s);
+                       bool general, bool fields, int tile, int order, hipStream_t s);
 
 // Deterministic total water / NAPL mass of one grid into dev_out[2].
 int fused_mass(const FusedGrid* dev_grid, const FusedGrid& G, int parity, double* dev_partial,

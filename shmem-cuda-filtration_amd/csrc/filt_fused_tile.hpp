@@ -120,7 +120,11 @@ typedef const __attribute__((address_space(4))) FusedGrid CG;
 // each read is a scalar load (a scalar-cache hit) at the point of use rather
 // than a value kept in SGPRs across the plane loop (the loop would otherwise
 // spill them to VGPR lanes and read each back with v_readlane).
+// FIELDS: every grid of the launch has per-cell K and porosity and one medium
+// (no media lookups, no tests for absent fields).
+template <bool FIELDS_>
 struct Ctx {
+    static constexpr bool FIELDS = FIELDS_;
     CG* g;
     const double* med;  // LDS media table
     const double* rh;   // LDS: refined reciprocals of h1, h2, h3, mu1, mu2
@@ -154,10 +158,31 @@ struct Raw {
 // The cell's medium: 0 unless the grid has a media field with several media
 // (the select is made where the medium is used, not where it is loaded, so
 // that the load stays in flight).
-__device__ __forceinline__ int medium(const Ctx& c, const Raw& q)
+template <class C>
+__device__ __forceinline__ int medium(const C& c, const Raw& q)
 {
+    if constexpr (C::FIELDS)
+        return 0;
     CG* G = c.G();
     return (G->media && G->p.n_media > 1) ? q.mdw : 0;
+}
+
+// The cell's permeability and porosity: per-cell where the grid has the
+// field, the medium's otherwise.
+template <class C>
+__device__ __forceinline__ double perm_of(const C& c, const Raw& q, int md)
+{
+    if constexpr (C::FIELDS)
+        return q.K;
+    return c.G()->perm ? q.K : c.tab(M_K, md);
+}
+
+template <class C>
+__device__ __forceinline__ double poro_of(const C& c, const Raw& q, int md)
+{
+    if constexpr (C::FIELDS)
+        return q.phi;
+    return c.G()->poro ? q.phi : c.tab(M_M, md);
 }
 
 // Every call issues the same loads whichever fields the grid has: a field
@@ -165,12 +190,22 @@ __device__ __forceinline__ int medium(const Ctx& c, const Raw& q)
 // used.  No branch around a load, so the memory operations in flight are the
 // same on every path through the plane loop and the compiler's waits for them
 // stay exact (a skipped load makes it wait for everything outstanding).
-__device__ __forceinline__ Raw load_raw(const Ctx& c, unsigned off, bool centre)
+template <class C>
+__device__ __forceinline__ Raw load_raw(const C& c, unsigned off, bool centre)
 {
     CG* G = c.G();
     const double* const Pin = c.parity ? G->P[1] : G->P[0];
-    const bool media = G->media && G->p.n_media > 1;
     Raw r;
+    if constexpr (C::FIELDS) {
+        r.mdw = 0;
+        r.P = gld(Pin, off);
+        r.S = gld(c.parity ? G->S[1] : G->S[0], off);
+        r.K = gld(G->perm, off);
+        if (centre)
+            r.phi = gld(G->poro, off);
+        return r;
+    }
+    const bool media = G->media && G->p.n_media > 1;
     r.mdw = gld(media ? G->media : (const int*)Pin, off >> 1);
     r.P = gld(Pin, off);
     r.S = gld(c.parity ? G->S[1] : G->S[0], off);
@@ -205,11 +240,12 @@ struct FormFP64 {
     // of four waves per SIMD (measured slower at three waves)
     static constexpr bool PIPELINED = false;
 
-    __device__ static Val eval(const Ctx& c, const Raw& q)
+    template <class C>
+    __device__ static Val eval(const C& c, const Raw& q)
     {
         const auto& p = c.G()->p;
         const int md = medium(c, q);
-        const double K = c.G()->perm ? q.K : c.tab(M_K, md);
+        const double K = perm_of(c, q, md);
         const double swr = c.tab(M_SWR, md);
         const DivBy d1{1. - swr, c.tab(M_DY, md)};
         const double S_e = d1(1. - q.S - swr);
@@ -229,8 +265,8 @@ struct FormFP64 {
         return v;
     }
 
-    template <int SN>
-    __device__ static void put(char* T, int s, const Ctx& c, const Val& v)
+    template <int SN, class C>
+    __device__ static void put(char* T, int s, const C& c, const Val& v)
     {
         double* d = (double*)T;
         const auto& p = c.G()->p;
@@ -243,8 +279,8 @@ struct FormFP64 {
 
     // roS_calculation (cpu.cpp:218-291) then the Newton (cpu.cpp:293-320)
     // of an interior cell, expression for expression as k_step_zm.
-    template <int SN, int SX, int TX>
-    __device__ static void update(const Ctx& c, const Val& zc, const char* Tm, const Val& zp,
+    template <int SN, int SX, int TX, class C>
+    __device__ static void update(const C& c, const Val& zc, const char* Tm, const Val& zp,
                                   const char* T, int s, const Raw& q, bool, int, double& nP,
                                   double& nS)
     {
@@ -279,7 +315,7 @@ struct FormFP64 {
         const double g2 = upwind_div(dy2 + r2c * g, dy1 + r2c * g, zc.X2, r2c, L[QX2 * SN + ym],
                                      L[QR2 * SN + ym], L[QX2 * SN + yp], L[QR2 * SN + yp], dy);
         const int md = medium(c, q);
-        const double phi = c.G()->poro ? q.phi : c.tab(M_M, md);
+        const double phi = poro_of(c, q, md);
         const double A1 = roS1 - (p.dt / phi) * (f1 + f2 + f3);
         const double A2 = roS2 - (p.dt / phi) * (g1 + g2 + g3);
         nP = Pc;
@@ -298,12 +334,12 @@ struct FormFP64 {
 
 // Mobility-density products of one cell.  1 - S_e = S/(1-S_wr) is formed
 // directly, without the cancellation of 1 - S_e at small NAPL saturation.
-template <bool GEN, class R>
-__device__ __forceinline__ void mobilities(const Ctx& c, const Raw& q, R& L1, R& L2)
+template <bool GEN, class R, class C>
+__device__ __forceinline__ void mobilities(const C& c, const Raw& q, R& L1, R& L2)
 {
     CG& G = *c.G();
     const int md = medium(c, q);
-    const double K = c.G()->perm ? q.K : c.tab(M_K, md);
+    const double K = perm_of(c, q, md);
     const double P = q.P;
     double k1, k2d;
     const double lam = c.tab(M_LAM, md);
@@ -342,20 +378,22 @@ __device__ __forceinline__ double rcp1(double d)
 // converges to, for one square root and two reciprocals.  Outside that case
 // (capillary terms, S_e < 2^-10, no finite real root) the iteration runs as
 // the reference's, with refined reciprocals.
-__device__ __forceinline__ void newton_fast(const Ctx& c, int md, double P0, double S0,
+template <class C>
+__device__ __forceinline__ void newton_fast(const C& c, int md, double P0, double S0,
                                             double A1, double A2, double& nP, double& nS)
 {
-    const auto& p = c.G()->p;
+    CG& G = *c.G();
+    const auto& p = G.p;
     const double ro01 = p.ro01, ro02 = p.ro02, b1 = p.beta1, b2 = p.beta2, Pa = p.P_atm;
     const double swr = c.tab(M_SWR, md), pd = c.tab(M_PD, md);
     const double lam = c.tab(M_LAM, md), inv1 = c.tab(M_INV1, md);
     const bool no_cap = (pd == 0.0) && (lam >= 0.1);
     if (no_cap && p.newton_iterations > 0) {
-        const double a1 = ro01 * b1, c1 = ro02 * b2;
-        const double al = a1 * c1;
-        const double be = ro01 * c1 + a1 * ro02 - A2 * a1 - A1 * c1;
-        const double ga = ro01 * ro02 - A2 * ro01 - A1 * ro02;
-        const double disc = be * be - 4.0 * al * ga;
+        // the uniform products come with the grid's descriptor
+        const double a1 = G.nq_a1, c1 = G.nq_c1;
+        const double be = G.nq_be0 - A2 * a1 - A1 * c1;
+        const double ga = G.nq_ga0 - A2 * ro01 - A1 * ro02;
+        const double disc = be * be - G.nq_al4 * ga;
         if (disc >= 0.0) {
             const double x = -2.0 * ga * rcp1(be + copysign(sqrt(disc), be));
             const double Sx = A2 * rcp1(ro02 + c1 * x);
@@ -394,9 +432,10 @@ __device__ __forceinline__ void newton_fast(const Ctx& c, int md, double P0, dou
 }
 
 // dt / phi with an FP64 refined reciprocal
-__device__ __forceinline__ double dt_over_phi(const Ctx& c, const Raw& q)
+template <class C>
+__device__ __forceinline__ double dt_over_phi(const C& c, const Raw& q)
 {
-    const double phi = c.G()->poro ? q.phi : c.tab(M_M, medium(c, q));
+    const double phi = poro_of(c, q, medium(c, q));
     return c.G()->p.dt * rcp1(phi);
 }
 
@@ -421,7 +460,8 @@ struct FormFast {
     static constexpr int WAVES2 = GEN ? 2 : 3;
     static constexpr bool PIPELINED = true;
 
-    __device__ static Val eval(const Ctx& c, const Raw& q)
+    template <class C>
+    __device__ static Val eval(const C& c, const Raw& q)
     {
         Val v;
         v.P = q.P;
@@ -429,8 +469,8 @@ struct FormFast {
         return v;
     }
 
-    template <int SN>
-    __device__ static void put(char* T, int s, const Ctx&, const Val& v)
+    template <int SN, class C>
+    __device__ static void put(char* T, int s, const C&, const Val& v)
     {
         double* d = (double*)T;
         d[s] = v.P;
@@ -438,8 +478,8 @@ struct FormFast {
         d[2 * SN + s] = v.L2;
     }
 
-    template <int SN, int SX, int TX>
-    __device__ static void update(const Ctx& c, const Val& zc, const char* Tm, const Val& zp,
+    template <int SN, int SX, int TX, class C>
+    __device__ static void update(const C& c, const Val& zc, const char* Tm, const Val& zp,
                                   const char* T, int s, const Raw& q, bool, int, double& nP,
                                   double& nS)
     {
@@ -508,8 +548,8 @@ struct FormMixed {
     static constexpr int tile_bytes() { return SN * SLOT_BYTES + TX * 16; }
 
     // FP64 mobilities of the top boundary cell at tile column x
-    template <int SN, int TX>
-    __device__ static void put_top(char* T, int x, const Ctx& c, const Raw& q)
+    template <int SN, int TX, class C>
+    __device__ static void put_top(char* T, int x, const C& c, const Raw& q)
     {
         double* t = (double*)(T + SN * SLOT_BYTES);
         double L1, L2;
@@ -518,7 +558,8 @@ struct FormMixed {
         t[TX + x] = L2;
     }
 
-    __device__ static Val eval(const Ctx& c, const Raw& q)
+    template <class C>
+    __device__ static Val eval(const C& c, const Raw& q)
     {
         Val v;
         v.P = q.P;
@@ -526,8 +567,8 @@ struct FormMixed {
         return v;
     }
 
-    template <int SN>
-    __device__ static void put(char* T, int s, const Ctx&, const Val& v)
+    template <int SN, class C>
+    __device__ static void put(char* T, int s, const C&, const Val& v)
     {
         ((double*)T)[s] = v.P;
         float* f = (float*)(T + SN * 8);
@@ -535,8 +576,8 @@ struct FormMixed {
         f[SN + s] = v.L2;
     }
 
-    template <int SN, int SX, int TX>
-    __device__ static void update(const Ctx& c, const Val& zc, const char* Tm, const Val& zp,
+    template <int SN, int SX, int TX, class C>
+    __device__ static void update(const C& c, const Val& zc, const char* Tm, const Val& zp,
                                   const char* T, int s, const Raw& q, bool jtop, int tx,
                                   double& nP, double& nS)
     {
@@ -595,7 +636,8 @@ struct FormMixed {
 // own output the plane's buffer stores write.  i1: i == 1; lastcol: i ==
 // nx-2; top: j == ny-2 (with Pt0 = ro1(i,1,0)*g*h2 of the input state); kl,
 // kh: k == 1, k == nz-2.
-__device__ __forceinline__ void write_copies(const Ctx& c, unsigned off, bool i1, bool lastcol,
+template <class C>
+__device__ __forceinline__ void write_copies(const C& c, unsigned off, bool i1, bool lastcol,
                                              bool top, double Pt0, bool kl, bool kh, double nP,
                                              double nS)
 {
@@ -677,8 +719,8 @@ __device__ inline void shell_cell(const FusedGrid& G, const double* Pin_, const 
 // Update of a thread's CY cells of plane k and their outputs: the cells'
 // own values by buffer stores every lane issues (DROP where a lane has no
 // interior cell), the shell copies of edge cells after them.
-template <class F, int SN, int SX, int TX, int CY>
-__device__ __forceinline__ void write_plane(const Ctx& c, const typename F::Val* zc,
+template <class F, int SN, int SX, int TX, int CY, class C>
+__device__ __forceinline__ void write_plane(const C& c, const typename F::Val* zc,
                                             const char* Lm, const typename F::Val* zp,
                                             const char* L, const int* s, const Raw* qc,
                                             const bool* jtop, int tx, const bool* inner,
@@ -711,8 +753,8 @@ __device__ __forceinline__ void write_plane(const Ctx& c, const typename F::Val*
 // (register blocking: the per-plane work common to a thread's cells -- loop
 // control, descriptor reads, the barrier -- is paid once for CY cells, and
 // their independent arithmetic interleaves).
-template <class F, int TX, int TY, int CY>
-__device__ __forceinline__ void march(const Ctx& c, const FusedGrid& G, unsigned lb, char* lds)
+template <class F, int TX, int TY, int CY, class C>
+__device__ __forceinline__ void march(const C& c, const FusedGrid& G, unsigned lb, char* lds)
 {
     constexpr int NT = TX * TY / CY;  // threads
     constexpr int NW = NT / 64;
@@ -891,7 +933,7 @@ __device__ __forceinline__ void march(const Ctx& c, const FusedGrid& G, unsigned
     }
 }
 
-template <class F, int TX, int TY, int CY>
+template <class F, int TX, int TY, int CY, bool FIELDS>
 __global__ __launch_bounds__(TX* TY / CY) __attribute__((amdgpu_waves_per_eu(
     CY == 1 ? F::WAVES : F::WAVES2))) void k_fused(
     const FusedGrid* __restrict__ grids, const int* __restrict__ block_grid, long long total_blocks,
@@ -932,7 +974,7 @@ __global__ __launch_bounds__(TX* TY / CY) __attribute__((amdgpu_waves_per_eu(
                                        : q == 3 ? p.mu1 : p.mu2);
         }
         __syncthreads();
-        Ctx c;
+        Ctx<FIELDS> c;
         c.g = (CG*)(uintptr_t)&G;
         c.med = med;
         c.rh = rh;
@@ -944,23 +986,28 @@ __global__ __launch_bounds__(TX* TY / CY) __attribute__((amdgpu_waves_per_eu(
     shell_cell(G, G.P[parity], G.S[parity], G.P[parity ^ 1], G.S[parity ^ 1], q);
 }
 
-// Launch of form F (for lambda = 2 only, and general) at tile shape `tile`.
+// Launch of form F at tile shape `tile`: for lambda = 2 only (with every
+// grid's K and porosity per cell and one medium, or not), or general.
 template <template <bool> class F>
 int launch_form(const FusedGrid* g, const int* bg, long long total, int parity, bool general,
-                int tile, int order, hipStream_t s)
+                bool fields, int tile, int order, hipStream_t s)
 {
     const dim3 grid((unsigned)total);
     const FusedTile& ft = FUSED_TILES[tile];
-#define FILT_FUSED_CASE(TXV, TYV, CYV)                                                         \
-    if (ft.tx == TXV && ft.ty == TYV && ft.cy == CYV) {                                        \
-        if (general)                                                                           \
-            hipLaunchKernelGGL((k_fused<F<true>, TXV, TYV, CYV>), grid, dim3(TXV * TYV / CYV), \
-                               0, s, g, bg, total, parity, order);                             \
-        else                                                                                   \
-            hipLaunchKernelGGL((k_fused<F<false>, TXV, TYV, CYV>), grid,                       \
-                               dim3(TXV * TYV / CYV), 0, s, g, bg, total, parity, order);      \
-        FILT_LAUNCH_CHECK("k_fused");                                                          \
-        return FILT_OK;                                                                        \
+#define FILT_FUSED_CASE(TXV, TYV, CYV)                                                     \
+    if (ft.tx == TXV && ft.ty == TYV && ft.cy == CYV) {                                    \
+        const dim3 block(TXV * TYV / CYV);                                                 \
+        if (general)                                                                       \
+            hipLaunchKernelGGL((k_fused<F<true>, TXV, TYV, CYV, false>), grid, block, 0, s, \
+                               g, bg, total, parity, order);                               \
+        else if (fields)                                                                   \
+            hipLaunchKernelGGL((k_fused<F<false>, TXV, TYV, CYV, true>), grid, block, 0, s, \
+                               g, bg, total, parity, order);                               \
+        else                                                                               \
+            hipLaunchKernelGGL((k_fused<F<false>, TXV, TYV, CYV, false>), grid, block, 0,   \
+                               s, g, bg, total, parity, order);                            \
+        FILT_LAUNCH_CHECK("k_fused");                                                      \
+        return FILT_OK;                                                                    \
     }
     FILT_FUSED_CASE(16, 16, 1)
     FILT_FUSED_CASE(32, 8, 1)

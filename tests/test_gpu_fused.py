@@ -235,6 +235,34 @@ def test_member_same_alone_and_in_mix(hip, precision):
 
 
 @pytest.mark.parametrize("precision", [FP64, FAST, MIXED])
+def test_member_with_fields_same_beside_member_without(hip, precision):
+    """A batch whose members all carry per-cell K and porosity and one medium
+    runs a kernel that reads both fields without tests; a member without them
+    in the batch switches it to the kernel that tests for them.  The member
+    with fields gives the same bits either way."""
+    import importlib
+    W = importlib.import_module("shmem-cuda-filtration_amd").workloads
+    p = W.base_params(26, 18, 22, h=W.INI_H)
+    W.set_patch(p, 7, 9)
+    perm, poro = W.gen_fields(p, 71, sigma=1.5, corr=4.0, clip=2.5)
+    g = W.initial_grid(p, perm, poro)
+    q = W.base_params(20, 14, 16, h=W.INI_H)
+    W.set_patch(q, 5, 6)
+    h = W.initial_grid(q, None, None)
+    alone = fused_run(hip, p, g, 30, precision=precision)
+    e = hip.ensemble(precision)
+    try:
+        a = e.add(p, g["P1"], g["S2"], perm, poro)
+        e.add(q, h["P1"], h["S2"], None, None)
+        e.step(30)
+        P, S = e.get(a)
+        assert np.array_equal(P, alone[0]) and np.array_equal(S, alone[1])
+        assert e.mass(a) == alone[2]
+    finally:
+        e.close()
+
+
+@pytest.mark.parametrize("precision", [FP64, FAST, MIXED])
 def test_lambda2_member_same_beside_reference_media_member(hip, orc, precision):
     """A batch that holds a member with lambda != 2 or P_d != 0 runs the
     kernel with the general Brooks-Corey powers; a lambda = 2, P_d = 0 member
