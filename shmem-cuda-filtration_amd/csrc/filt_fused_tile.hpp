@@ -53,6 +53,7 @@
 #include "filt_device.hpp"
 #include "filt_fused.hpp"
 #include "filt_runtime.hpp"
+#include "filt_zmarch.hpp"
 
 namespace filt {
 namespace fz {
@@ -73,29 +74,11 @@ __device__ __forceinline__ void gst(double* base, unsigned off, double v)
     __builtin_nontemporal_store(v, (__attribute__((address_space(1))) double*)((gchar*)base + off));
 }
 
-// Buffer view of an output array of `bytes` bytes.  A buffer store is range
-// checked by the hardware: a lane whose offset lies past the array stores
-// nothing.  The main stores of a plane are therefore issued by every lane,
-// with an offset of DROP where the lane has no cell, and no branch skips
-// them -- which keeps the number of memory operations after the prefetch
-// loads the same on every path, so the wait for those loads does not also
-// wait for the plane's stores.
-typedef __amdgpu_buffer_rsrc_t Rsrc;
-typedef __attribute__((ext_vector_type(2))) unsigned U32x2;
-constexpr unsigned DROP = 0xFFFFFFF8u;  // past every array (arrays stay below 4 GiB)
-constexpr int BUF_WORD3 = 0x00020000;   // raw buffer, 32-bit data format
-constexpr int VMCNT0 = 0x0F70;          // s_waitcnt vmcnt(0), other counters free
-
-__device__ __forceinline__ Rsrc out_rsrc(double* base, unsigned bytes)
-{
-    return __builtin_amdgcn_make_buffer_rsrc(base, 0, (int)bytes, BUF_WORD3);
-}
-
-__device__ __forceinline__ void bst(Rsrc r, unsigned off, double v)
-{
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U32x2, v), r, (int)off, 0,
-                                          2 /* non-temporal */);
-}
+using zm::DROP;
+using zm::out_rsrc;
+using zm::Rsrc;
+using zm::VMCNT0;
+using zm::bstn;
 
 // The media table in LDS, one row of FILT_MAX_MEDIA per quantity: the tile
 // loop looks entries up by each cell's medium.
@@ -738,8 +721,8 @@ __device__ __forceinline__ void write_plane(const C& c, const typename F::Val* z
 #pragma unroll
     for (int q = 0; q < CY; q++) {
         const unsigned off = inner[q] ? col[q] + plane : DROP;
-        bst(rP, off, nP[q]);
-        bst(rS, off, nS[q]);
+        bstn(rP, off, nP[q]);
+        bstn(rS, off, nS[q]);
     }
     const bool kl = k == 1, kh = k == nz - 2;
 #pragma unroll
@@ -892,8 +875,8 @@ __device__ __forceinline__ void march(const C& c, const FusedGrid& G, unsigned l
             // so the first plane's wait for them has the same count as later
             // planes' (without them, that count would be zero for all planes)
             const Rsrc rP = out_rsrc(c.Po(), c.bytes()), rS = out_rsrc(c.So(), c.bytes());
-            bst(rP, DROP, 0.0);
-            bst(rS, DROP, 0.0);
+            bstn(rP, DROP, 0.0);
+            bstn(rS, DROP, 0.0);
         }
         for (int k = kb; k < ke; k++) {
             const bool odd = ((k - kb) & 1) != 0;

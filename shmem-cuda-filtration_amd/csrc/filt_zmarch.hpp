@@ -39,6 +39,32 @@ __device__ __forceinline__ int ldi(const int* base, unsigned off4)
     return *(const int*)((const char*)base + off4);
 }
 
+// Buffer view of an output array of `bytes` bytes.  A buffer store is range
+// checked by the hardware: a lane whose offset lies past the array stores
+// nothing.  A plane loop's stores are therefore issued by every lane, with
+// an offset of DROP where the lane has nothing to store, and no branch skips
+// them.  That keeps the number of memory operations issued after a load the
+// same on every path through the loop, and with it the compiler's wait for
+// the load exact: a store that some path skips makes the wait cover every
+// memory operation outstanding, the plane's stores included.
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+typedef __attribute__((ext_vector_type(2))) unsigned U32x2;
+constexpr unsigned DROP = 0xFFFFFFF8u;  // past every array (arrays stay below 4 GiB)
+constexpr int BUF_WORD3 = 0x00020000;   // raw buffer, 32-bit data format
+constexpr int VMCNT0 = 0x0F70;          // s_waitcnt vmcnt(0), other counters free
+
+__device__ __forceinline__ Rsrc out_rsrc(double* base, unsigned bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(base, 0, (int)bytes, BUF_WORD3);
+}
+
+// Non-temporal buffer store of one double.
+__device__ __forceinline__ void bstn(Rsrc r, unsigned off, double v)
+{
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U32x2, v), r, (int)off, 0,
+                                          2 /* non-temporal */);
+}
+
 // The halo ring of a TX x TY tile without its corners (a cross stencil never
 // reads them): 2*TX + 2*TY cells, cell h at LDS row ly, column lx and at
 // (di, dj) from the tile origin.  Rows y-1 and y+TY first, then columns x-1
