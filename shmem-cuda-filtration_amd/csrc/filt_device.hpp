@@ -98,10 +98,16 @@ __device__ inline double upwind(double a2, double a1, double Xc, double rc, doub
 // of that happens, so the expansion is exactly the sequence below with the
 // reciprocal computed once per divisor: three FP64 operations per division
 // instead of a v_rcp_f64, eight more FP64 operations and the two scale steps.
-// Zeros (q = n*y keeps n's sign, d > 0) take q; any other numerator outside
-// the range takes the division itself.  A probe over 4.3e9 (numerator,
-// divisor) pairs, specials and significands of all ones included, found no
-// difference on MI355X.
+// The remainder is formed as -(d*q - n): that is n - d*q rounded once, as in
+// the expansion, and for a zero numerator it keeps the zero's sign through the
+// last step (n = +-0 gives q = +-0, d*q - n = +0, and fma(-0, y, +-0) = +-0, as
+// n / d), so zeros need no select; for a nonzero remainder -(d*q - n) and
+// n - d*q round to the same value, and a zero remainder leaves q as it is
+// either way.  Any other numerator outside the range takes the division
+// itself.  tests/hip/divby_check compares the result with n / d on 2^24
+// numerators (specials and significands of all ones included) for each of 64
+// divisors; an earlier probe of the expansion's own remainder form covered
+// 4.3e9 pairs.
 __device__ __forceinline__ double refined_reciprocal(double d)
 {
     double y = __builtin_amdgcn_rcp(d);
@@ -122,12 +128,10 @@ struct DivBy {
     __device__ __forceinline__ double operator()(double n) const
     {
         const double q = n * y;
-        const double r = fma(-d, q, n);
-        double f = fma(r, y, q);
+        const double t = fma(d, q, -n);  // d*q - n, rounded once
+        double f = fma(-t, y, q);
         const unsigned e = ((unsigned)__double2hiint(n) >> 20) & 0x7ffu;
-        const bool zero = n == 0.0;
-        f = zero ? q : f;
-        if (!(e - 423u <= 1200u || zero))  // |n| outside [2^-600, 2^601)
+        if (!(e - 423u <= 1200u || n == 0.0))  // |n| outside [2^-600, 2^601)
             f = div_rare(n, d);
         return f;
     }
