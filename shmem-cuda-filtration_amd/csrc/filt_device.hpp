@@ -261,7 +261,9 @@ __device__ inline void newton_cell(double& P, double& S, double roS1, double roS
 }
 
 // Source patch test of main.cpp:109 / cpu.cpp:346, generalised to bounds.
-__host__ __device__ inline bool in_patch(int I, int k, const filt_params& p)
+// Params: filt_params, or filt_params in the kernel argument segment (KArg).
+template <class Params>
+__host__ __device__ inline bool in_patch(int I, int k, const Params& p)
 {
     return I >= p.src_x0 && I <= p.src_x1 && k >= p.src_z0 && k <= p.src_z1;
 }

@@ -31,7 +31,10 @@ bool tiled_enabled(int kernel);
 // The fused step (filt_step.hip): u, roS and the Newton in one z-march;
 // false when the grid or the parameters need the stages one by one.
 bool step_fusable(const filt_params& p, int nx, int rank, int size);
-int launch_step_zm(const filt_fields& f, const filt_params& p, int nx, bool with_u);
+// bc: also write the boundary cells the closed form of the sweep takes from
+// post-Newton values (after a stage 1 that wrote the others, k_ro_P2_Xi with
+// BC_A in filt_stages.hip); the step then needs no boundary launch.
+int launch_step_zm(const filt_fields& f, const filt_params& p, int nx, bool with_u, bool bc);
 // The whole step but the boundary in one launch (filt_step.hip): stage 1
 // with stages 2-4, then the copy of its stash; for grids step_fusable
 // accepts whose media in use all have lambda = 2 (and DivBy-able mu1, mu2,

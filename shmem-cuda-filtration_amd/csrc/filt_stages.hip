@@ -49,6 +49,14 @@ __device__ inline int media_of(const filt_fields& f, const filt_params& p, long 
 // ---------------------------------------------------------------- stage 1 --
 // cpu.cpp:3-15, 57-74.  Bytes per cell: media 4 + S2 8 + P1 8 in,
 // P2, Xi1, Xi2, ro1, ro2 40 out.
+//
+// BC_A (the step's boundary folded into its kernels, one rank, every extent
+// >= 3): a cell (0, j, k) of the i = 0 face whose source (1, j, k) is a shell
+// cell also takes its post-sweep value, the pre-sweep (1, j, k), after its
+// own stage 1 has read its old value.  Nothing else in this launch writes
+// (1, j, k), and the rest of the sweep, which reads only post-Newton values,
+// is written by the stages 2-4 kernel (k_step_zm with BC).
+template <bool BC_A>
 __global__ __launch_bounds__(TX* TY) void k_ro_P2_Xi(filt_fields f, filt_params p, int nx,
                                                      int rank, int size)
 {
@@ -64,6 +72,12 @@ __global__ __launch_bounds__(TX* TY) void k_ro_P2_Xi(filt_fields f, filt_params 
     __builtin_nontemporal_store(r.Xi2, f.Xi2 + e.c);
     __builtin_nontemporal_store(r.ro1, f.ro1 + e.c);
     __builtin_nontemporal_store(r.ro2, f.ro2 + e.c);
+    if constexpr (BC_A) {
+        if (e.i == 0 && !is_interior(1, e.j, e.k, nx, p.Ny, p.Nz)) {
+            f.P1[e.c] = f.P1[e.c + 1];
+            f.S2[e.c] = f.S2[e.c + 1];
+        }
+    }
 }
 
 // ---------------------------------------------------------------- stage 2 --
@@ -512,8 +526,8 @@ int filt_ro_P2_Xi_calculation(const filt_fields* dev, const filt_params* p, int 
 {
     int rc = launch_check(dev, p, localNx);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_ro_P2_Xi, cell_grid(p, localNx), dim3(TX, TY), 0, stream(), *dev, *p,
-                       localNx, rank, size);
+    hipLaunchKernelGGL(k_ro_P2_Xi<false>, cell_grid(p, localNx), dim3(TX, TY), 0, stream(), *dev,
+                       *p, localNx, rank, size);
     FILT_LAUNCH_CHECK("k_ro_P2_Xi");
     return FILT_OK;
 }
@@ -600,7 +614,7 @@ int filt_u_roS_P1_S2_calculation(const filt_fields* dev, const filt_params* p, d
             (with_u && (!dev->u1x || !dev->u1y || !dev->u1z || !dev->u2x || !dev->u2y ||
                         !dev->u2z)))
             return set_error(FILT_ERR_INVALID, "null device fields");
-        return launch_step_zm(*dev, *p, localNx, with_u != 0);
+        return launch_step_zm(*dev, *p, localNx, with_u != 0, false);
     }
     // the divgrad scheme reads this step's u (cpu.cpp:187-190)
     if ((with_u || p->roS_scheme == FILT_SCHEME_DIVGRAD) &&
@@ -635,6 +649,22 @@ int filt_time_step_timed(const filt_fields* dev, const filt_params* p, double t,
         if ((rc = launch_step_full(*dev, *p, localNx, with_u != 0,
                                    events ? (hipEvent_t)events[1] : nullptr)))
             return rc;
+    } else if (dev && p && step_fusable(*p, localNx, rank, size)) {
+        // stage 1 with the boundary cells that read pre-sweep shell values,
+        // then stages 2-4 with every other boundary cell: no boundary launch
+        if ((rc = launch_check(dev, p, localNx))) return rc;
+        if (!dev->P2 || !dev->Xi1 || !dev->Xi2 || !dev->ro1 || !dev->ro2 || !dev->roS1 ||
+            !dev->roS2 || !dev->roS1_old || !dev->roS2_old ||
+            (with_u && (!dev->u1x || !dev->u1y || !dev->u1z || !dev->u2x || !dev->u2y ||
+                        !dev->u2z)))
+            return set_error(FILT_ERR_INVALID, "null device fields");
+        hipLaunchKernelGGL(k_ro_P2_Xi<true>, cell_grid(p, localNx), dim3(TX, TY), 0, stream(),
+                           *dev, *p, localNx, rank, size);
+        FILT_LAUNCH_CHECK("k_ro_P2_Xi");
+        if ((rc = mark(1))) return rc;
+        if ((rc = launch_step_zm(*dev, *p, localNx, with_u != 0, true))) return rc;
+        if ((rc = mark(2))) return rc;
+        return mark(3);
     } else {
         if ((rc = filt_ro_P2_Xi_calculation(dev, p, localNx, rank, size))) return rc;
         if ((rc = mark(1))) return rc;

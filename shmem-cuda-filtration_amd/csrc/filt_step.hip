@@ -67,7 +67,82 @@ struct StepArgs {
 __device__ __forceinline__ KArg<filt_fields>* fields_arg() { return &kernarg<StepArgs>(0)->f; }
 __device__ __forceinline__ KArg<filt_params>* params_arg() { return &kernarg<StepArgs>(0)->p; }
 
-template <int TX, int TY, bool WITH_U>
+// The boundary cells of the sweep's closed form (cpu.cpp:44-54, 323-403;
+// k_bc_b in filt_stages.hip) that follow from interior cell (i, j, k)'s new
+// P, S or from constants of its column, written by the cell's own thread:
+//   (0, j, k) if i == 1;  (N-1, j, k) if i == N-2;
+//   for column c in {i} and, if i == N-2, also {N-1}:
+//     (c, M-1, k) = (S, P + Pt) if j == M-2, Pt = ro1(i,1,0)*g*h2, with
+//       k = 0 / L-1 as well when k == 1 / L-2;
+//     (c, j, 0) if k == 1, (c, j, L-1) if k == L-2;
+//     (c, 0, k): P_atm and S = src(i, k) if j == 1, with (c, 0, 0) =
+//       (P_atm, pre-sweep S(i, 0, 1)) when k == 1 and (c, 0, L-1) =
+//       (P_atm, src(i, L-2)) when k == L-2.
+// No other thread of the launch reads these cells (the stencils read P2, Xi
+// and ro, and P1, S2 only at interior centres); the pre-sweep S(i, 0, 1) is
+// read by the thread that then overwrites it.
+template <class F, class Pp>
+__device__ inline void boundary_from_cell(F* Fp, const Pp& p, unsigned c, unsigned sy,
+                                          unsigned sz, int i, int j, int k, int nx, int ny,
+                                          int nz, double P, double S, double Pt)
+{
+    double* const P1 = Fp->P1;
+    double* const S2 = Fp->S2;
+    if (i == 1) {
+        stn(P1, c - 8, P);
+        stn(S2, c - 8, S);
+    }
+    const bool last = i == nx - 2;
+    if (last) {
+        stn(P1, c + 8, P);
+        stn(S2, c + 8, S);
+    }
+    const bool kl = k == 1, kh = k == nz - 2;
+    double S01 = 0.0;
+    if (j == 1) {
+        if (kl)
+            S01 = ld(S2, c - sy);  // pre-sweep S(i, 0, 1), before this thread rewrites it
+    }
+    const double Ssrc = j == 1 ? (in_patch(i, k, p) ? p.S2_gr : 0.0) : 0.0;
+    for (int q = 0; q < (last ? 2 : 1); q++) {
+        const unsigned d = 8u * q;  // column i, then N-1
+        if (j == ny - 2) {
+            const double Pb = P + Pt;
+            stn(P1, c + sy + d, Pb);
+            stn(S2, c + sy + d, S);
+            if (kl) {
+                stn(P1, c + sy - sz + d, Pb);
+                stn(S2, c + sy - sz + d, S);
+            }
+            if (kh) {
+                stn(P1, c + sy + sz + d, Pb);
+                stn(S2, c + sy + sz + d, S);
+            }
+        }
+        if (kl) {
+            stn(P1, c - sz + d, P);
+            stn(S2, c - sz + d, S);
+        }
+        if (kh) {
+            stn(P1, c + sz + d, P);
+            stn(S2, c + sz + d, S);
+        }
+        if (j == 1) {
+            stn(P1, c - sy + d, p.P_atm);
+            stn(S2, c - sy + d, Ssrc);
+            if (kl) {
+                stn(P1, c - sy - sz + d, p.P_atm);
+                stn(S2, c - sy - sz + d, S01);
+            }
+            if (kh) {
+                stn(P1, c - sy + sz + d, p.P_atm);
+                stn(S2, c - sy + sz + d, Ssrc);
+            }
+        }
+    }
+}
+
+template <int TX, int TY, bool WITH_U, bool BC>
 __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(4))) void k_step_zm(
     StepArgs args)
 {
@@ -130,6 +205,11 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(4))) voi
         for (int a = 0; a < NL; a++) hn[a] = ld(A[a], hcol + kb * sz);
     const int cx = tx + 1, cy = ty + 1;
     const double g = p.g_const;
+    // the bottom boundary's pressure offset ro1(i,1,0)*g*h2 (this step's
+    // stage 1), for the cells of row M-2
+    double Pt = 0.0;
+    if (BC && valid && inner && j == ny - 2)
+        Pt = ld(f.ro1, 8u * (unsigned)(i + nx)) * g * p.h2;
     // reciprocals of h1, h2, h3, 2h1, 2h2, 2h3 for DivBy; read after the
     // first barrier of the loop
     __shared__ double rh[6];
@@ -292,6 +372,8 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(4))) voi
             newton_cell(P, Sv, A1, A2, med[0][md], med[1][md], med[2][md], *Q);
             stn(F->P1, c, P);
             stn(F->S2, c, Sv);
+            if constexpr (BC)
+                boundary_from_cell(F, *Q, c, sy, sz, i, j, k, nx, ny, nz, P, Sv, Pt);
         }
 #pragma unroll
         for (int a = 0; a < NL; a++) {
@@ -646,18 +728,17 @@ __global__ __launch_bounds__(256) void k_unstash(FullArgs args, int tx, int ty)
     }
 }
 
-template <bool WITH_U>
 using StepFn = void (*)(StepArgs);
 
-template <bool WITH_U>
-StepFn<WITH_U> pick_step(int tx, int ty)
+template <bool WITH_U, bool BC>
+StepFn pick_step(int tx, int ty)
 {
-    if (tx == 64 && ty == 4) return k_step_zm<64, 4, WITH_U>;
-    if (tx == 32 && ty == 8) return k_step_zm<32, 8, WITH_U>;
-    if (tx == 16 && ty == 16) return k_step_zm<16, 16, WITH_U>;
-    if (tx == 64 && ty == 8) return k_step_zm<64, 8, WITH_U>;
-    if (tx == 32 && ty == 16) return k_step_zm<32, 16, WITH_U>;
-    if (tx == 64 && ty == 16) return k_step_zm<64, 16, WITH_U>;
+    if (tx == 64 && ty == 4) return k_step_zm<64, 4, WITH_U, BC>;
+    if (tx == 32 && ty == 8) return k_step_zm<32, 8, WITH_U, BC>;
+    if (tx == 16 && ty == 16) return k_step_zm<16, 16, WITH_U, BC>;
+    if (tx == 64 && ty == 8) return k_step_zm<64, 8, WITH_U, BC>;
+    if (tx == 32 && ty == 16) return k_step_zm<32, 16, WITH_U, BC>;
+    if (tx == 64 && ty == 16) return k_step_zm<64, 16, WITH_U, BC>;
     return nullptr;
 }
 
@@ -692,7 +773,7 @@ bool step_fusable(const filt_params& p, int nx, int rank, int size)
 }
 
 bool full_tile_ok(int tx, int ty) { return pick_full<true, false>(tx, ty) != nullptr; }
-bool step_tile_ok(int tx, int ty) { return pick_step<true>(tx, ty) != nullptr; }
+bool step_tile_ok(int tx, int ty) { return pick_step<true, false>(tx, ty) != nullptr; }
 
 bool full_step_fusable(const filt_params& p, int nx, int rank, int size, bool per_cell_media)
 {
@@ -755,7 +836,7 @@ int launch_step_full(const filt_fields& f, const filt_params& p, int nx, bool wi
 // Launches of k_step_zm since the library was loaded (filt_step_zm_launches).
 static std::atomic<long long> g_step_launches{0};
 
-int launch_step_zm(const filt_fields& f, const filt_params& p, int nx, bool with_u)
+int launch_step_zm(const filt_fields& f, const filt_params& p, int nx, bool with_u, bool bc)
 {
     const TileConfig t = step_tile_config(nx, p.Ny, p.Nz);
     StepArgs a;
@@ -765,13 +846,11 @@ int launch_step_zm(const filt_fields& f, const filt_params& p, int nx, bool with
     a.kz = t.kz;
     a.map = make_map(nx, p.Ny, p.Nz, t);
     const unsigned blocks = (unsigned)a.map.count();
-    if (with_u) {
-        hipLaunchKernelGGL(pick_step<true>(t.tx, t.ty), dim3(blocks), dim3(t.tx, t.ty), 0,
-                           stream(), a);
-    } else {
-        hipLaunchKernelGGL(pick_step<false>(t.tx, t.ty), dim3(blocks), dim3(t.tx, t.ty), 0,
-                           stream(), a);
-    }
+    const StepFn fn = with_u ? (bc ? pick_step<true, true>(t.tx, t.ty)
+                                   : pick_step<true, false>(t.tx, t.ty))
+                             : (bc ? pick_step<false, true>(t.tx, t.ty)
+                                   : pick_step<false, false>(t.tx, t.ty));
+    hipLaunchKernelGGL(fn, dim3(blocks), dim3(t.tx, t.ty), 0, stream(), a);
     FILT_LAUNCH_CHECK("k_step_zm");
     g_step_launches.fetch_add(1, std::memory_order_relaxed);
     return FILT_OK;
