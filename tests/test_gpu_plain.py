@@ -619,6 +619,22 @@ def test_fused_step_equals_stage_calls(hip, params, dims, with_u):
     fused_vs_staged(hip, p, g, 3, with_u)
 
 
+@pytest.mark.parametrize("dims", [(9, 7, 6), (3, 5, 3), (21, 3, 8)])
+def test_folded_boundary_with_patch_on_the_edges(hip, dims):
+    """filt_time_step writes the boundary from its stage-1 and stages 2-4
+    kernels; here the source patch covers the first and last columns and
+    planes (the j = 0 row rules at i = 1, N-2 and k = 0, L-1) and the grid
+    has per-cell K and porosity: every array equals the five stage calls
+    (with the separate boundary kernels) bit for bit."""
+    nx, ny, nz = dims
+    p = ref_params(nx, ny, nz, source=1)
+    p.src_x0, p.src_x1, p.src_z0, p.src_z1 = 0, nx - 1, 0, nz - 1
+    g = random_state(p, np.random.default_rng(nx * ny + nz), perm=True, poro=True)
+    fused_vs_staged(hip, p, g, 3, 1, perm=True, poro=True)
+    p.src_x0, p.src_x1, p.src_z0, p.src_z1 = 1, 1, nz - 2, nz - 1
+    fused_vs_staged(hip, p, g, 2, 0, perm=True, poro=True)
+
+
 def test_divby_equals_ieee_division():
     """The fused step's division by a grid spacing (DivBy, filt_device.hpp)
     against n / d, bit for bit, on 2^24 numerators x 64 divisors."""
