@@ -344,6 +344,21 @@ __device__ __forceinline__ void mobilities(const C& c, const Raw& q, R& L1, R& L
     L2 = (R)(K * k2d * G.inv_mu2 * r2);
 }
 
+// Square root of d > 0 from the hardware reciprocal square root and two
+// coupled Newton-Raphson steps (s -> sqrt(d), h -> 1/(2 sqrt(d))), within an
+// ulp or so.  No range scaling: 0, infinity and NaN give NaN, which the
+// caller's finiteness test sends down its general path.
+__device__ __forceinline__ double sqrt1(double d)
+{
+    const double y = __builtin_amdgcn_rsq(d);
+    double s = d * y, h = 0.5 * y;
+    const double r = fma(-s, h, 0.5);
+    s = fma(s, r, s);
+    h = fma(h, r, h);
+    const double e = fma(-s, s, d);
+    return fma(e, h, s);
+}
+
 // Reciprocal with one Newton-Raphson refinement (within an ulp or two).
 __device__ __forceinline__ double rcp1(double d)
 {
@@ -378,7 +393,7 @@ __device__ __forceinline__ void newton_fast(const C& c, int md, double P0, doubl
         const double ga = G.nq_ga0 - A2 * ro01 - A1 * ro02;
         const double disc = be * be - G.nq_al4 * ga;
         if (disc >= 0.0) {
-            const double x = -2.0 * ga * rcp1(be + copysign(sqrt(disc), be));
+            const double x = -2.0 * ga * rcp1(be + copysign(sqrt1(disc), be));
             const double Sx = A2 * rcp1(ro02 + c1 * x);
             if (isfinite(x) && isfinite(Sx) && (1 - Sx - swr) * inv1 >= 0.0009765625) {
                 nP = Pa + x;
