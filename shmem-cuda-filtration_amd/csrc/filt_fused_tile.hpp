@@ -359,7 +359,8 @@ __device__ __forceinline__ double sqrt1(double d)
     return fma(e, h, s);
 }
 
-// Reciprocal with one Newton-Raphson refinement (within an ulp or two).
+// Reciprocal from the hardware estimate with two Newton-Raphson refinements
+// (within an ulp).  One refinement was measured no faster in the fused step.
 __device__ __forceinline__ double rcp1(double d)
 {
     double r = __builtin_amdgcn_rcp(d);
