@@ -110,7 +110,7 @@ void fused_prepare(FusedGrid& G, int tile)
         G.kz = 32;
     const int tx = FUSED_TILES[tile].tx, ty = FUSED_TILES[tile].ty;
     const int threads = tx * ty / FUSED_TILES[tile].cy;
-    G.nbx = (G.nx - 2 + tx - 1) / tx;
+    G.nbx = (G.nx - 1 + tx - 1) / tx;  // x-tiles from x = 0 over x <= nx - 2
     G.nby = (G.ny - 2 + ty - 1) / ty;
     G.nbz = (G.nz - 2 + G.kz - 1) / G.kz;
     G.n_interior_blocks = (long long)G.nbx * G.nby * G.nbz;

@@ -36,7 +36,8 @@ struct FusedTile {
     int tx, ty, cy;
 };
 constexpr FusedTile FUSED_TILES[] = {{16, 16, 1}, {32, 8, 1},  {64, 4, 1},  {32, 16, 1},
-                                     {64, 8, 1},  {64, 8, 2},  {32, 16, 2}, {64, 16, 2}};
+                                     {64, 8, 1},  {64, 8, 2},  {32, 16, 2}, {64, 16, 2},
+                                     {128, 4, 1}};
 constexpr int N_FUSED_TILES = sizeof(FUSED_TILES) / sizeof(FUSED_TILES[0]);
 // Index of a tile shape in FUSED_TILES, -1 if none.
 int fused_tile_index(int tx, int ty, int cy);

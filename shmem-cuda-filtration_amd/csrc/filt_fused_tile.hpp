@@ -26,8 +26,9 @@
 //     old S(i',0,1) for k = 0, with i' = min(i, N-2)
 // which is the closed form of the reference's i -> j -> k sweep.
 //
-// Launch: interior tiles of TX x TY columns (CY columns per thread, stacked in
-// y) march over z-chunks of kz planes.  Per plane, the stage-1 values of the tile and
+// Launch: tiles of TX x TY columns covering the interior (CY columns per
+// thread, stacked in y; x-tiles start at x = 0, on the rows' 128-byte lines)
+// march over z-chunks of kz planes.  Per plane, the stage-1 values of the tile and
 // its one-cell ring go to one of two LDS tiles; the raw inputs (state,
 // statics) of the own column two planes ahead and of the ring one plane
 // ahead are in flight while a plane is updated; the own column's plane k-1
@@ -767,7 +768,10 @@ __device__ __forceinline__ void march(const C& c, const FusedGrid& G, unsigned l
     const int by = (int)(r % nby);
     const int bz = (int)(r / nby);
     const int nx = G.nx, ny = G.ny, nz = G.nz;
-    const int i0 = 1 + bx * TX, j0 = 1 + by * TY;
+    // x-tiles start at x = 0, so a tile's rows begin on 128-byte lines when
+    // the grid's rows do: the column x = 0 is evaluated as a neighbour but
+    // not updated
+    const int i0 = bx * TX, j0 = 1 + by * TY;
     const int kb = 1 + bz * G.kz, ke = min(kb + G.kz, nz - 1);
     if (kb >= ke)
         return;  // uniform over the block
@@ -782,7 +786,7 @@ __device__ __forceinline__ void march(const C& c, const FusedGrid& G, unsigned l
     for (int q = 0; q < CY; q++) {
         const int j = j0 + CY * ty + q;
         in_grid[q] = i < nx && j < ny;
-        inner[q] = i <= nx - 2 && j <= ny - 2;
+        inner[q] = i >= 1 && i <= nx - 2 && j <= ny - 2;
         top[q] = j == ny - 2;   // shell copies of row M-1 (loop-invariant)
         jtop[q] = j == 1;       // the row next to the top boundary
         col[q] = 8u * (unsigned)(i + j * nx);
@@ -802,7 +806,7 @@ __device__ __forceinline__ void march(const C& c, const FusedGrid& G, unsigned l
     else if (h < 2 * TX) { rx = h - TX; ry = TY; }
     else if (h < 2 * TX + TY) { rx = -1; ry = h - 2 * TX; }
     else if (h < RING) { rx = TX; ry = h - 2 * TX - TY; }
-    const bool ring = h < RING && i0 + rx < nx && j0 + ry < ny;
+    const bool ring = h < RING && i0 + rx >= 0 && i0 + rx < nx && j0 + ry < ny;
     const unsigned rcol = 8u * (unsigned)((i0 + rx) + (j0 + ry) * nx);
     const int rs = (ry + 1) * SX + (rx + 1);
     const bool i1 = i == 1, lastcol = i == nx - 2;
@@ -1016,6 +1020,7 @@ int launch_form(const FusedGrid* g, const int* bg, long long total, int parity, 
     FILT_FUSED_CASE(64, 8, 2)
     FILT_FUSED_CASE(32, 16, 2)
     FILT_FUSED_CASE(64, 16, 2)
+    FILT_FUSED_CASE(128, 4, 1)
 #undef FILT_FUSED_CASE
     return set_error(FILT_ERR_INVALID, "unknown fused tile %d", tile);
 }

@@ -115,7 +115,7 @@ def errors_vs_oracle(P, S, m, g, mo):
 
 
 TILES = [(16, 16, 1), (32, 8, 1), (64, 4, 1), (32, 16, 1), (64, 8, 1), (64, 8, 2), (32, 16, 2),
-         (64, 16, 2)]
+         (64, 16, 2), (128, 4, 1)]
 
 
 @pytest.mark.parametrize("precision", [FP64, FAST, MIXED])
