@@ -758,6 +758,9 @@ def main():
     ap.add_argument("--form", choices=("fp64", "fast", "mixed"),
                     help="only run this fused form on the headline grid and print its line "
                          "(for profiles)")
+    ap.add_argument("--leg", choices=("c4", "serving"),
+                    help="only run this leg (the batched ensemble or the serving loop) and "
+                         "print its entry (for profiles)")
     ap.add_argument("--tiling", default=None,
                     help="with --form: fixed tiling TXxTY[xCY],kz,order (e.g. 64x8x2,64,1) "
                          "instead of the autotuner")
@@ -771,6 +774,9 @@ def main():
         return
     if args.form:
         print(json.dumps(run_one_form(args)), flush=True)
+        return
+    if args.leg:
+        print(json.dumps(run_c4(args) if args.leg == "c4" else run_serving(args)), flush=True)
         return
 
     p, perm, poro = PKG.workloads.c2_workload(args.n)
