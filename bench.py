@@ -376,7 +376,9 @@ def run_fused(args, p, perm, poro, precision, steps=None, every=None):
     info = {"value": cells / (ms * 1e-3), "ms_per_step": ms, "tiling": tiling,
             "bytes_per_cell_update": FUSED_BYTES[precision],
             "hbm_GBps": FUSED_BYTES[precision] * cells / (ms * 1e-3) / 1e9,
-            "hbm_frac": FUSED_BYTES[precision] * cells / (ms * 1e-3) / HBM_PEAK}
+            "hbm_frac": FUSED_BYTES[precision] * cells / (ms * 1e-3) / HBM_PEAK,
+            "tiled_mix_ceiling": tiled_mix_ceiling("k_fused", (p.NX, p.Ny, p.Nz),
+                                                   tiling["tile"][:2], tiling["kz"], ms)}
     return info, (P, S, m, series)
 
 
@@ -703,6 +705,28 @@ def stream_mix_ceiling(kernel, cells):
             "GBps": best["TBps"] * 1e3, "nontemporal_stores": best["nontemporal_stores"]}
 
 
+def tiled_mix_ceiling(kernel, grid, tile, kz, ms=None):
+    """Time of a pure stream with the kernel's mix of FP64 arrays and the
+    z-march's access pattern: tiles of tile[0] x tile[1] columns from x = 0,
+    kz planes per block, no halo, no arithmetic (stream_mix's tiled mode).
+    With the kernel's own time ms, also the kernel's share of it."""
+    import subprocess
+    exe = os.path.join(ROOT, "tests", "hip", "bin", "stream_mix")
+    if not os.path.exists(exe):
+        return None
+    args = [exe, "tiled", *[str(v) for v in grid], str(tile[0]), str(tile[1]), str(kz), kernel]
+    out = subprocess.run(args, capture_output=True, text=True, timeout=120, check=True).stdout
+    rows = [json.loads(x) for x in out.splitlines() if x.startswith("{")]
+    if not rows:
+        return None
+    r = rows[0]
+    res = {"mix": f"{r['in']} in / {r['out']} out FP64 arrays", "tile": list(tile), "kz": kz,
+           "bytes_per_cell": r["bytes_per_cell"], "ms": r["ms"], "GBps": r["TBps"] * 1e3}
+    if ms:
+        res["kernel_frac_of_mix"] = r["ms"] / ms
+    return res
+
+
 def copy_calibration(n_bytes=1 << 30, reps=5):
     """The library's streaming copy kernel over n_bytes: the achievable HBM
     ceiling, and the reference point that scales the PMC byte counters."""
@@ -850,6 +874,9 @@ def main():
     mix_ceiling = stream_mix_ceiling(KERNEL_NAMES[dom], cells)
     if mix_ceiling:
         mix_ceiling["kernel_frac_of_mix"] = mix_ceiling["ms"] / dom_ms
+    dom_tiles = pr["tiles"].get(dom)
+    tiled_ceiling = (tiled_mix_ceiling(KERNEL_NAMES[dom], (p.NX, p.Ny, p.Nz), dom_tiles["tile"],
+                                       dom_tiles["kz"], dom_ms) if dom_tiles else None)
     step_bytes = sum(PLAIN_BYTES[s] for s in step_names) * cells
     line = {
         "metric": "cell-updates/s",
@@ -888,6 +915,7 @@ def main():
             "algorithmic_bytes_per_launch": dom_bytes,
             "kernel_ms": dom_ms,
             "stream_mix_ceiling": mix_ceiling,
+            "tiled_mix_ceiling": tiled_ceiling,
         },
         "stages_ms": stage_ms,
         "stages_ms_from": f"{pr['breakdown_steps']} steps after the timed region, with a HIP "
