@@ -226,14 +226,16 @@ __device__ inline void newton_cell(double& P, double& S, double roS1, double roS
                 break;
             double Pn, Sn;
             newton_step(P, S, 0.0, 0.0, roS1, roS2, p, Pn, Sn);
-            if (same_bits(Pn, P) && same_bits(Sn, S))
-                return;  // fixed point: (P, S) is the result
-            if (w >= 2 && same_bits(Pn, Pq) && same_bits(Sn, Sq)) {
-                // 2-cycle: x_{w+1} = x_{w-1}, x_{w+2} = x_w, ...
-                if (((n - w) & 1) == 0) {
-                    P = Pn;
-                    S = Sn;
-                }
+            // both exit tests evaluated in full (bitwise &, |), so a lane's
+            // exit is one mask test rather than a chain of branches
+            const bool fix = same_bits(Pn, P) & same_bits(Sn, S);
+            const bool cyc = (w >= 2) & same_bits(Pn, Pq) & same_bits(Sn, Sq);
+            if (fix | cyc) {
+                // fixed point: (P, S) is the result.  2-cycle: x_{w+1} =
+                // x_{w-1}, x_{w+2} = x_w, ..., so the count left picks one.
+                const bool take = !fix & (((n - w) & 1) == 0);
+                P = take ? Pn : P;
+                S = take ? Sn : S;
                 return;
             }
             Pq = P;
