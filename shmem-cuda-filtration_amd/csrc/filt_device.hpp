@@ -228,12 +228,12 @@ __device__ inline void newton_cell(double& P, double& S, double roS1, double roS
             newton_step(P, S, 0.0, 0.0, roS1, roS2, p, Pn, Sn);
             // both exit tests evaluated in full (bitwise &, |), so a lane's
             // exit is one mask test rather than a chain of branches
-            const bool fix = same_bits(Pn, P) & same_bits(Sn, S);
-            const bool cyc = (w >= 2) & same_bits(Pn, Pq) & same_bits(Sn, Sq);
+            const int fix = (int)same_bits(Pn, P) & (int)same_bits(Sn, S);
+            const int cyc = (int)(w >= 2) & (int)same_bits(Pn, Pq) & (int)same_bits(Sn, Sq);
             if (fix | cyc) {
                 // fixed point: (P, S) is the result.  2-cycle: x_{w+1} =
                 // x_{w-1}, x_{w+2} = x_w, ..., so the count left picks one.
-                const bool take = !fix & (((n - w) & 1) == 0);
+                const bool take = !fix && ((n - w) & 1) == 0;
                 P = take ? Pn : P;
                 S = take ? Sn : S;
                 return;
