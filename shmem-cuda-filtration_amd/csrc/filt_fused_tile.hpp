@@ -394,14 +394,15 @@ __device__ __forceinline__ void newton_fast(const C& c, int md, double P0, doubl
         const double be = G.nq_be0 - A2 * a1 - A1 * c1;
         const double ga = G.nq_ga0 - A2 * ro01 - A1 * ro02;
         const double disc = be * be - G.nq_al4 * ga;
-        if (disc >= 0.0) {
-            const double x = -2.0 * ga * rcp1(be + copysign(sqrt1(disc), be));
-            const double Sx = A2 * rcp1(ro02 + c1 * x);
-            if (isfinite(x) && isfinite(Sx) && (1 - Sx - swr) * inv1 >= 0.0009765625) {
-                nP = Pa + x;
-                nS = Sx;
-                return;
-            }
+        // computed whatever the sign of disc (a negative one gives NaN) and
+        // the tests combined bitwise: one branch instead of a chain
+        const double x = -2.0 * ga * rcp1(be + copysign(sqrt1(disc), be));
+        const double Sx = A2 * rcp1(ro02 + c1 * x);
+        if ((int)(disc >= 0.0) & (int)isfinite(x) & (int)isfinite(Sx) &
+            (int)((1 - Sx - swr) * inv1 >= 0.0009765625)) {
+            nP = Pa + x;
+            nS = Sx;
+            return;
         }
     }
     double P = P0, Sv = S0;
