@@ -71,6 +71,25 @@ def test_fused_fp64_equals_plain_bitwise(hip, dims):
     assert np.array_equal(Sf, Sp)
 
 
+@pytest.mark.parametrize("dims", [(65, 9, 7), (66, 6, 5), (129, 5, 6), (130, 4, 4), (4, 7, 9)])
+def test_fused_fp64_equals_plain_at_tile_row_ends(hip, dims):
+    """x-tiles start at x = 0, so the last interior column x = nx - 2 falls on
+    the first or second column of a tile (nx = 65, 66 with 64-wide tiles;
+    129, 130 with 64- and 128-wide), and a 4-wide grid has one interior
+    column: the fused FP64 form equals the plain path bit for bit for every
+    tile width there."""
+    p = c1_params()
+    p.NX, p.Ny, p.Nz = dims
+    p.src_x0, p.src_x1 = max(0, dims[0] // 2 - 2), dims[0] // 2 + 1
+    p.src_z0, p.src_z1 = max(0, dims[2] // 2 - 2), dims[2] // 2 + 1
+    g = random_state(p, np.random.default_rng(sum(dims)), media=False)
+    Pp, Sp = plain_run(hip, p, g, 6)
+    for tile in ((64, 4, 1), (64, 8, 1), (128, 4, 1), (32, 8, 1), (16, 16, 1), (64, 8, 2)):
+        Pf, Sf, _ = fused_run(hip, p, g, 6, kz=(5, 1), tile=tile)
+        assert np.array_equal(Pf, Pp, equal_nan=True), tile
+        assert np.array_equal(Sf, Sp, equal_nan=True), tile
+
+
 def test_fused_fp64_equals_plain_per_cell_fields(hip):
     p, g = c2_like(40)
     Pp, Sp = plain_run(hip, p, g, 30)
