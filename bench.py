@@ -11,7 +11,7 @@ Beside the headline the JSON line carries the other configurations:
   stage_calls / defines_shims  the five defines.h functions called one by one,
                  and the reference driver's calls through the defines.h shims;
   forms          the fused FP64, fast and mixed forms on the same grid;
-  c2_oracle      the plain path and every form against the oracle port run
+  oracle_check   the plain path and every form against the oracle port run
                  on all host threads this process may use (cell for cell);
   slab_*         one rank's share (512 x 512 x 256 and its transpose): plain
                  vs fused FP64 / fast / mixed, un-batched; the 512 x 512 x 256
@@ -24,7 +24,7 @@ Beside the headline the JSON line carries the other configurations:
                  continuous-batching server, FP64 vs mixed;
   cpu_baseline   the reference's own CPU path (oracle/_ref) on one thread on
                  a bounded sample of the headline workload, and on
-                 configuration 1 in full; the oracle port on all threads.
+                 the homogeneous 32^3 cube in full; the oracle port on all threads.
 Progress goes to stderr; the one JSON line to stdout.
 """
 import argparse
@@ -389,7 +389,7 @@ def run_one_form(args):
     hiprt = importlib.import_module(PKG.__name__ + ".hiprt")
     be = PKG.device.Backend()
     prec = {"fp64": 0, "fast": 1, "mixed": 2}[args.form]
-    p, perm, poro = PKG.workloads.c2_workload(args.n)
+    p, perm, poro = PKG.workloads.lognormal_workload(args.n)
     g = PKG.workloads.initial_grid(p, perm, poro)
     e = be.ensemble(prec)
     try:
@@ -442,7 +442,7 @@ def run_slab(args, nx, ny, nz, with_oracle):
     import types
     W = PKG.workloads
     t0 = time.perf_counter()
-    p, perm, poro = W.c3_workload(nx, ny, nz)
+    p, perm, poro = W.slab_workload(nx, ny, nz)
     gen_s = time.perf_counter() - t0
     steps = args.slab_steps
     sub = types.SimpleNamespace(steps=steps, warmup=3, autotune=False)
@@ -480,8 +480,8 @@ def run_slab(args, nx, ny, nz, with_oracle):
 
 # ---------------------------------------------------------- ensembles --
 
-def run_c4(args):
-    """Batched ensemble: args.c4 realizations with uneven grids, args.c4_steps
+def run_ensemble(args):
+    """Batched ensemble: args.ensemble realizations with uneven grids, args.ensemble_steps
     steps.  Plain FP64 path run realization by realization (all stages, all
     arrays, default tiles) against the batched forms (every member in one
     launch per step, the whole run one hipGraph, tiling from the autotuner)."""
@@ -492,10 +492,10 @@ def run_c4(args):
     stream = lib.filt_get_stream()
     W = PKG.workloads
     t0 = time.perf_counter()
-    members = W.c4_members(args.c4)
+    members = W.ensemble_members(args.ensemble)
     gen_s = time.perf_counter() - t0
     cells = sum(m["params"].NX * m["params"].Ny * m["params"].Nz for m in members)
-    steps = args.c4_steps
+    steps = args.ensemble_steps
     base_dt = W.base_params(4, 4, 4).dt
     lowered = [m for m in members if m["dt"] < base_dt]
     out = {"realizations": len(members), "cells": cells, "steps": steps,
@@ -509,7 +509,7 @@ def run_c4(args):
            "members": [{"shape": list(m["shape"]), "sigma": round(m["sigma"], 4),
                         "clip": m["clip"], "dt": m["dt"], "r": round(m["stability_r"], 4)}
                        for m in members]}
-    log(f"C4: {len(members)} members generated ({gen_s:.1f} s); plain per realization")
+    log(f"ensemble: {len(members)} members generated ({gen_s:.1f} s); plain per realization")
     grids = []
     for m in members:
         g = W.initial_grid(m["params"], m["perm"], m["poro"])
@@ -533,7 +533,7 @@ def run_c4(args):
     results = {}
     for prec, form in FORMS.items():
         name = "batched_" + form.replace("fused_", "")
-        log(f"C4: {name}")
+        log(f"ensemble: {name}")
         e = be.ensemble(prec)
         for m in members:
             g = W.initial_grid(m["params"], m["perm"], m["poro"])
@@ -598,7 +598,7 @@ def run_serving(args):
     same request stream; the fast form too with --serve-fast."""
     serving = importlib.import_module(PKG.__name__ + ".serving")
     be = PKG.device.Backend()
-    stream = serving.c5_stream(rate=args.serve_rate, duration=args.serve_duration)
+    stream = serving.request_stream(rate=args.serve_rate, duration=args.serve_duration)
     out = {"rate_per_s": args.serve_rate, "window_s": args.serve_duration,
            "requests": len(stream),
            "offered_cell_updates_per_s": sum(s[1][0] * s[1][1] * s[1][2] * s[2] for s in stream)
@@ -613,19 +613,19 @@ def run_serving(args):
 
 # ----------------------------------------------------------------- CPU --
 
-def cpu_baseline(args, c2_port=None):
+def cpu_baseline(args, port_full=None):
     """The reference's own CPU path (oracle/_ref, compiled from its sources;
     i-outer / k-inner loops, one thread, as its 1-rank CPU build runs):
       value: two time steps of the headline 256^3 grid (its rate per cell
              does not depend on the step count), about 20 s;
-      c1:    configuration 1 in full (32^3, 500 steps).
+      cube32_full: the homogeneous 32^3 cube in full (500 steps).
     The compiled reference cannot express per-cell fields, a 4 x 4 patch or
     Sn = 0.5: its samples use the configurations' grid, spacing, time step,
     fluids and Brooks-Corey values through its media table, and its own
     5 x 5 patch with Sn = 0.4 (the cost per cell is the same).  Beside it the
     oracle port (loop-interchanged, OpenMP) on all threads this process may
-    use: on configuration 1 in full (bitwise equal to the reference there)
-    and on the headline workload in full (c2_port, from the oracle check)."""
+    use: on the 32^3 cube in full (bitwise equal to the reference there)
+    and on the headline workload in full (port_full, from the oracle check)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import oracle
@@ -659,7 +659,7 @@ def cpu_baseline(args, c2_port=None):
     log(f"cpu_baseline: {kind} on 2 steps of {n}^3, 1 thread")
     dt2, _ = timed(impl, p2, 2)
     p1 = ref_expressible(32, 32, 32, (1 / 31, 1 / 31, 1 / 31))
-    log(f"cpu_baseline: {kind} on configuration 1 (32^3, 500 steps), 1 thread")
+    log(f"cpu_baseline: {kind} on the homogeneous 32^3 cube (500 steps), 1 thread")
     dt1, g_ref1 = timed(impl, p1, 500)
     port = oracle.Oracle()
     port.set_threads(host_threads())
@@ -671,16 +671,16 @@ def cpu_baseline(args, c2_port=None):
                      f"{info['cpu_model']}; the reference's rate per cell does not depend on "
                      f"the step count",
            "host": info,
-           "c1_full": {"value": 32 ** 3 * 500 / dt1, "cores": 1, "kind": kind,
-                       "sample": f"configuration 1 in full: 32^3, 500 steps, {dt1:.1f} s "
+           "cube32_full": {"value": 32 ** 3 * 500 / dt1, "cores": 1, "kind": kind,
+                       "sample": f"the homogeneous 32^3 cube in full: 500 steps, {dt1:.1f} s "
                                  "(reference 5x5 patch, Sn 0.4)"},
            "best_effort_port": {
-               "c1_full": {"value": 32 ** 3 * 500 / dtp1, "cores": host_threads(),
+               "cube32_full": {"value": 32 ** 3 * 500 / dtp1, "cores": host_threads(),
                            "seconds": dtp1,
                            "bitwise_equal_to_reference": bool(
                                np.array_equal(g_port1["P1"], g_ref1["P1"]) and
                                np.array_equal(g_port1["S2"], g_ref1["S2"]))},
-               "c2_full": c2_port}}
+               "lognormal_full": port_full}}
     return out
 
 
@@ -762,9 +762,9 @@ def main():
     ap.add_argument("--n", type=int, default=256, help="grid edge")
     ap.add_argument("--no-cpu", action="store_true",
                     help="skip the CPU legs: the baseline and the oracle checks")
-    ap.add_argument("--c4", type=int, default=96,
+    ap.add_argument("--ensemble", type=int, default=96,
                     help="realizations of the batched-ensemble leg (0: skip)")
-    ap.add_argument("--c4-steps", type=int, default=300)
+    ap.add_argument("--ensemble-steps", type=int, default=300)
     ap.add_argument("--slab-steps", type=int, default=100,
                     help="steps of the 512x512x256 and 256x512x512 slab legs (0: skip)")
     ap.add_argument("--serve-duration", type=float, default=60.0,
@@ -782,7 +782,7 @@ def main():
     ap.add_argument("--form", choices=("fp64", "fast", "mixed"),
                     help="only run this fused form on the headline grid and print its line "
                          "(for profiles)")
-    ap.add_argument("--leg", choices=("c4", "serving"),
+    ap.add_argument("--leg", choices=("ensemble", "serving"),
                     help="only run this leg (the batched ensemble or the serving loop) and "
                          "print its entry (for profiles)")
     ap.add_argument("--tiling", default=None,
@@ -800,16 +800,16 @@ def main():
         print(json.dumps(run_one_form(args)), flush=True)
         return
     if args.leg:
-        print(json.dumps(run_c4(args) if args.leg == "c4" else run_serving(args)), flush=True)
+        print(json.dumps(run_ensemble(args) if args.leg == "ensemble" else run_serving(args)), flush=True)
         return
 
-    p, perm, poro = PKG.workloads.c2_workload(args.n)
+    p, perm, poro = PKG.workloads.lognormal_workload(args.n)
     log(f"plain {args.n}^3: {args.steps} timed steps")
     pr = run_plain(args, p, perm, poro)
     wall, stage_ms = pr["wall"], pr["stage_ms"]
     cells = p.NX * p.Ny * p.Nz
     value = cells * args.steps / wall
-    forms, states, split, shims, c2_oracle, c2_port = {}, {}, None, None, None, None
+    forms, states, split, shims, oracle_check, port_full = {}, {}, None, None, None, None
     one_kernel = None
     if not args.plain_only:
         states["plain"] = plain_state(p, perm, poro, args.steps)
@@ -850,10 +850,10 @@ def main():
         if not args.no_cpu:
             log(f"oracle port on {args.n}^3, {args.steps} steps, {host_threads()} threads")
             ref, secs, mref, _ = oracle_run(p, perm, poro, args.steps)
-            c2_port = {"value": cells * args.steps / secs, "cores": host_threads(),
+            port_full = {"value": cells * args.steps / secs, "cores": host_threads(),
                        "seconds": secs, "sample": f"the headline workload in full: {args.n}^3, "
                                                   f"{args.steps} steps, per-cell K and porosity"}
-            c2_oracle = {name: form_errors(P, S, m, ref["P1"], ref["S2"], mref,
+            oracle_check = {name: form_errors(P, S, m, ref["P1"], ref["S2"], mref,
                                            f"oracle port after {args.steps} steps")
                          for name, (P, S, m, _) in states.items()}
         if args.slab_steps > 0:
@@ -861,8 +861,8 @@ def main():
                                        ((256, 512, 512), False)):
                 log("slab %dx%dx%d" % shape)
                 forms["slab_%dx%dx%d" % shape] = run_slab(args, *shape, with_oracle)
-        if args.c4 > 0:
-            forms["batched_ensemble"] = run_c4(args)
+        if args.ensemble > 0:
+            forms["batched_ensemble"] = run_ensemble(args)
         if args.serve_duration > 0:
             forms["serving"] = run_serving(args)
     step_names = pr["names"]
@@ -928,11 +928,11 @@ def main():
         "stage_calls": split,
         "plain_one_kernel": one_kernel,
         "defines_shims": shims,
-        "c2_oracle": c2_oracle,
+        "oracle_check": oracle_check,
         "forms": forms,
     }
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(args, c2_port)
+        line["cpu_baseline"] = cpu_baseline(args, port_full)
     log("done")
     print(json.dumps(line), flush=True)
 

@@ -76,7 +76,7 @@ class Server:
             pass
 
 
-def c5_stream(rate=20.0, duration=60.0, seed=5005, edge_lo=32, edge_hi=192,
+def request_stream(rate=20.0, duration=60.0, seed=5005, edge_lo=32, edge_hi=192,
               steps_lo=100, steps_hi=2000):
     """[(arrival_s, shape, nsteps, member_seed)] of the serving configuration."""
     rng = np.random.default_rng(seed)

@@ -55,18 +55,18 @@ def set_patch(p, x0, z0, size=4):
     return p
 
 
-def c1_params():
+def cube32_params():
     """32^3 over a 1 m cube, homogeneous, 500 steps."""
     return base_params(32, 32, 32)
 
 
-def c2_params(n=256):
+def lognormal_params(n=256):
     """256^3, per-cell log-normal K (sigma 1, correlation 8 cells) and
     porosity U[0.35, 0.45], 200 steps."""
     return base_params(n, n, n, h=INI_H)
 
 
-def c3_params(nx=512, ny=512, nz=256):
+def slab_params(nx=512, ny=512, nz=256):
     """One rank's share: 512 x 512 x 256, sigma 1.5, correlation 16 cells."""
     return base_params(nx, ny, nz, h=INI_H)
 
@@ -109,25 +109,25 @@ def stability(p, perm=None, poro=None):
                                      None if poro is None else poro.ctypes.data, n)
 
 
-def c2_workload(n=256, seed=2024):
-    p = c2_params(n)
+def lognormal_workload(n=256, seed=2024):
+    p = lognormal_params(n)
     perm, poro = gen_fields(p, seed, sigma=1.0, corr=8.0, clip=2.5)
     return p, perm, poro
 
 
-def c3_workload(nx=512, ny=512, nz=256, seed=3033):
+def slab_workload(nx=512, ny=512, nz=256, seed=3033):
     """The slab with ln K clipped at +-2 sigma: its 10 m water column makes
     water ~11x denser at the bottom, and at +-2.5 sigma the guard reads 2.65
     (unstable); at +-2 sigma it reads 1.25."""
-    p = c3_params(nx, ny, nz)
+    p = slab_params(nx, ny, nz)
     perm, poro = gen_fields(p, seed, sigma=1.5, corr=16.0, clip=2.0)
     return p, perm, poro
 
 
-C4_EDGES = (48, 64, 96, 128, 160)
+ENSEMBLE_EDGES = (48, 64, 96, 128, 160)
 
 
-def ensemble_member(rng, edges=C4_EDGES, corr=8.0, seed=0, shape=None, threads=0,
+def ensemble_member(rng, edges=ENSEMBLE_EDGES, corr=8.0, seed=0, shape=None, threads=0,
                     clip=2.0, r_max=1.5):
     """One realization of the batched-ensemble / serving configurations: each
     axis drawn from `edges` (or the given shape), sigma ~ U[0.5, 2],
@@ -154,7 +154,7 @@ def ensemble_member(rng, edges=C4_EDGES, corr=8.0, seed=0, shape=None, threads=0
             "stability_r": r, "dt": p.dt, "shape": shape}
 
 
-def c4_members(n=96, seed=4096):
+def ensemble_members(n=96, seed=4096):
     """Configuration 4: n realizations with edges from {48,64,96,128,160}."""
     rng = np.random.default_rng(seed)
     return [ensemble_member(rng, seed=seed * 1000 + q) for q in range(n)]

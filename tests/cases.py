@@ -27,7 +27,7 @@ def ref_params(nx, ny, nz, source=1, **kw):
     return abi.finalize_patch_py(p)
 
 
-def c1_params(n=32, beta1=1.0e-4, beta2=1.0e-8):
+def cube32_params(n=32, beta1=1.0e-4, beta2=1.0e-8):
     """Configuration 1 (32^3 over a 1 m cube, h = 1/31): K = 6.64e-11,
     phi = 0.4, lambda = 2, S_wr = 0.1, P_d = 0, Sn = 0.5 on the 4x4 patch
     I, k in [14, 17], dt = 1e-4.  The stated compressibility 4.4e-10 is

@@ -86,7 +86,7 @@ def trajectory(ref, dims, source, steps, name, media_seed=None, keep=("P1", "S2"
     save(name, meta_of(p, steps=steps, media_seed=media_seed), arrays)
 
 
-def stated_c1_divergence(ref):
+def stated_cube32_divergence(ref):
     """The configuration-1 compressibility 4.4e-10 on the reference itself
     (h = 1/31, dt = 1e-4, 32^3, its own 5x5 patch and S2_gr 0.4)."""
     p = ref_params(32, 32, 32, source=2, h1=1 / 31, h2=1 / 31, h3=1 / 31, dt=1e-4,
@@ -101,7 +101,7 @@ def stated_c1_divergence(ref):
         with np.errstate(invalid="ignore"):
             pmax.append(np.nanmax(np.abs(g["P1"])) if np.isfinite(g["P1"]).any() else np.nan)
     finite = [bool(np.isfinite(g["P1"]).all())]
-    save("stated_c1_divergence", meta_of(p), {"pmax": np.array(pmax), "finite_end": np.array(finite)})
+    save("stated_cube32_divergence", meta_of(p), {"pmax": np.array(pmax), "finite_end": np.array(finite)})
 
 
 def main():
@@ -116,7 +116,7 @@ def main():
     trajectory(ref, (9, 8, 1), 1, 30, "traj_9x8x1", keep=("P1", "S2", "u1x", "u2y"))
     trajectory(ref, (6, 7, 2), 1, 30, "traj_6x7x2", keep=("P1", "S2", "u2y", "u2z"))
     trajectory(ref, (2, 8, 8), 1, 30, "traj_2x8x8", keep=("P1", "S2", "u2y"))
-    stated_c1_divergence(ref)
+    stated_cube32_divergence(ref)
 
 
 if __name__ == "__main__":

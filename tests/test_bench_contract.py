@@ -21,7 +21,7 @@ def load_bench():
 
 def test_helpers_and_tables():
     b = load_bench()
-    for name in ("cpu_baseline", "load_traffic", "run_plain", "run_fused", "run_c4",
+    for name in ("cpu_baseline", "load_traffic", "run_plain", "run_fused", "run_ensemble",
                  "run_serving", "run_slab", "copy_calibration", "mixed_errors", "form_errors",
                  "oracle_run", "plain_state", "host_info", "run_one_form", "main"):
         assert callable(getattr(b, name)), name

@@ -36,7 +36,7 @@ def params_of(meta):
 def test_fixtures_present():
     names = {os.path.basename(f)[:-4] for f in glob.glob(os.path.join(GOLDEN, "*.npz"))}
     assert {"stage_kat_7x6x5", "stage_kat_9x8x7", "boundary_kat_6x5x7", "traj_15cube_s3",
-            "traj_16cube_two_media", "stated_c1_divergence"} <= names
+            "traj_16cube_two_media", "stated_cube32_divergence"} <= names
 
 
 @pytest.mark.parametrize("name", ["stage_kat_7x6x5", "stage_kat_9x8x7", "stage_kat_5x5x1",
@@ -91,7 +91,7 @@ def test_trajectory(orc, name):
 def test_stated_compressibility_diverges(orc):
     """With beta = 4.4e-10 (the configurations' figure) the reference scheme
     blows up within a dozen steps; the oracle reproduces that run exactly."""
-    d, meta = load("stated_c1_divergence")
+    d, meta = load("stated_cube32_divergence")
     p = params_of(meta)
     g = HostGrid(p)
     orc.initial_data(g, p)

@@ -1,19 +1,19 @@
-"""The configurations at their stated sizes, on the GPU against the oracle.
+"""The workloads at their stated sizes, on the GPU against the oracle.
 
-C2  256^3, log-normal K (sigma 1, correlation 8 cells), porosity U[0.35, 0.45],
+Log-normal cube: 256^3, log-normal K (sigma 1, correlation 8 cells), porosity U[0.35, 0.45],
     200 steps: the plain path (filt_time_step, u included) cell for cell
     against the oracle on all host threads the process may use: P1 and S2
     relative L-inf <= 1e-10, water and NAPL mass <= 1e-12 relative; the fast
     and mixed forms against the same oracle state to their bars (S2 L-inf
     <= 1e-4, P1 relative L-inf <= 1e-6, every mass <= 1e-9).
-C3  one rank's share, 512 x 512 x 256 (sigma 1.5, correlation 16 cells),
+Slab: one rank's share, 512 x 512 x 256 (sigma 1.5, correlation 16 cells),
     100 steps: the plain path, the fast and the mixed forms against the
     oracle, per-field L-inf and the cumulative mass-balance error every 10
     steps (printed; held to the same bars at every point).
-C4  96 realizations of uneven shape (edges from {48, 64, 96, 128, 160}),
+Ensemble: 96 realizations of uneven shape (edges from {48, 64, 96, 128, 160}),
     300 steps batched in one graph, every form: each member equals its run
     alone bit for bit; the three smallest match the oracle.
-C5  Poisson arrivals at 20 requests/s for 60 s through filt_serve_* (mixed
+Serving: Poisson arrivals at 20 requests/s for 60 s through filt_serve_* (mixed
     form): a fixed sample of 64 requests equals the same requests run alone
     bit for bit.
 The oracle runs on the CPU in the same process; it is the checker only.
@@ -78,11 +78,11 @@ def run_form(hip, precision, p, g, steps, every=None):
         e.close()
 
 
-# ---------------------------------------------------------------- C2 --
+# ------------------------------------------------ log-normal 256^3 cube --
 
 @pytest.fixture(scope="module")
-def c2(orc):
-    p, perm, poro = W.c2_workload(256)
+def lognormal256(orc):
+    p, perm, poro = W.lognormal_workload(256)
     g0 = W.initial_grid(p, perm, poro)
     ref = g0.copy()
     orc.set_threads(host_threads())
@@ -94,8 +94,8 @@ def c2(orc):
 
 
 @pytest.mark.timeout(900)
-def test_c2_plain_path_cell_for_cell(hip, orc, c2):
-    p, g0, ref, mref = c2
+def test_lognormal_256cube_plain_path_cell_for_cell(hip, orc, lognormal256):
+    p, g0, ref, mref = lognormal256
     d = hip.grid(p, perm=True, poro=True)
     try:
         d.upload(g0)
@@ -106,7 +106,7 @@ def test_c2_plain_path_cell_for_cell(hip, orc, c2):
     finally:
         d.free()
     e = errors(P, S, m, ref["P1"], ref["S2"], mref)
-    print("C2 plain 256^3 200 steps vs oracle:", json.dumps(e))
+    print("log-normal cube: plain 256^3 200 steps vs oracle:", json.dumps(e))
     assert e["P1_rel_linf"] <= 1e-10 and e["S2_rel_linf"] <= 1e-10
     assert e["water_mass_rel"] <= 1e-12 and e["napl_mass_rel"] <= 1e-12
     # the device mass sum equals the oracle's sum over the same state
@@ -118,19 +118,19 @@ def test_c2_plain_path_cell_for_cell(hip, orc, c2):
 
 @pytest.mark.timeout(900)
 @pytest.mark.parametrize("precision", [FAST, MIXED])
-def test_c2_fast_and_mixed_forms_vs_oracle(hip, c2, precision):
-    p, g0, ref, mref = c2
+def test_lognormal_256cube_fast_and_mixed_forms_vs_oracle(hip, lognormal256, precision):
+    p, g0, ref, mref = lognormal256
     P, S, m, _ = run_form(hip, precision, p, g0, 200)
     e = errors(P, S, m, ref["P1"], ref["S2"], mref)
-    print(f"C2 precision {precision} 256^3 200 steps vs oracle:", json.dumps(e))
+    print(f"log-normal cube: precision {precision} 256^3 200 steps vs oracle:", json.dumps(e))
     assert within(e, BARS), e
 
 
-# ---------------------------------------------------------------- C3 --
+# ------------------------------------------------- 512 x 512 x 256 slab --
 
 @pytest.fixture(scope="module")
-def c3(orc):
-    p, perm, poro = W.c3_workload()
+def slab(orc):
+    p, perm, poro = W.slab_workload()
     g0 = W.initial_grid(p, perm, poro)
     ref = g0.copy()
     series = []
@@ -153,8 +153,8 @@ def mass_series_errors(series, ref_series):
 
 
 @pytest.mark.timeout(900)
-def test_c3_slab_plain_path_vs_oracle_with_mass_series(hip, c3):
-    p, g0, ref, ref_series = c3
+def test_slab_plain_path_vs_oracle_with_mass_series(hip, slab):
+    p, g0, ref, ref_series = slab
     d = hip.grid(p, perm=True, poro=True)
     series = []
     try:
@@ -167,36 +167,36 @@ def test_c3_slab_plain_path_vs_oracle_with_mass_series(hip, c3):
         d.free()
     e = errors(P, S, series[-1], ref["P1"], ref["S2"], ref_series[-1])
     ms = mass_series_errors(series, ref_series)
-    print("C3 plain 512x512x256 100 steps vs oracle:", json.dumps(e))
-    print("C3 plain cumulative mass-balance error:", json.dumps(ms))
+    print("slab: plain 512x512x256 100 steps vs oracle:", json.dumps(e))
+    print("slab: plain cumulative mass-balance error:", json.dumps(ms))
     assert e["P1_rel_linf"] <= 1e-10 and e["S2_rel_linf"] <= 1e-10
     assert all(r["water"] <= 1e-12 and r["napl"] <= 1e-12 for r in ms)
 
 
 @pytest.mark.timeout(900)
 @pytest.mark.parametrize("precision", [FAST, MIXED])
-def test_c3_slab_fast_and_mixed_vs_oracle_with_mass_series(hip, c3, precision):
-    p, g0, ref, ref_series = c3
+def test_slab_fast_and_mixed_vs_oracle_with_mass_series(hip, slab, precision):
+    p, g0, ref, ref_series = slab
     P, S, m, series = run_form(hip, precision, p, g0, 100, every=10)
     e = errors(P, S, m, ref["P1"], ref["S2"], ref_series[-1])
     ms = mass_series_errors(series, ref_series)
-    print(f"C3 precision {precision} 512x512x256 100 steps vs oracle:", json.dumps(e))
-    print(f"C3 precision {precision} cumulative mass-balance error:", json.dumps(ms))
+    print(f"slab: precision {precision} 512x512x256 100 steps vs oracle:", json.dumps(e))
+    print(f"slab: precision {precision} cumulative mass-balance error:", json.dumps(ms))
     assert within(e, BARS), e
     assert all(r["water"] <= 1e-9 and r["napl"] <= 1e-9 and r["total"] <= 1e-9 for r in ms)
 
 
-# ---------------------------------------------------------------- C4 --
+# ----------------------------------------------- 96-realization ensemble --
 
 @pytest.fixture(scope="module")
-def c4_members():
-    return W.c4_members(96)
+def ensemble_members():
+    return W.ensemble_members(96)
 
 
 @pytest.mark.timeout(900)
 @pytest.mark.parametrize("precision", [FP64, FAST, MIXED])
-def test_c4_all_members_batched_equal_alone(hip, orc, c4_members, precision):
-    members = c4_members
+def test_ensemble_96_members_batched_equal_alone(hip, orc, ensemble_members, precision):
+    members = ensemble_members
     assert len(members) == 96
     assert all(m["clip"] == 2.0 and m["stability_r"] <= 1.5 for m in members)
     grids = [W.initial_grid(m["params"], m["perm"], m["poro"]) for m in members]
@@ -230,7 +230,7 @@ def test_c4_all_members_batched_equal_alone(hip, orc, c4_members, precision):
             orc.run(ref, p, 300, with_u=0)
             (Pb, Sb), mb = batched[q]
             err = errors(Pb, Sb, mb, ref["P1"], ref["S2"], orc.mass(ref, p))
-            print(f"C4 member {q} {members[q]['shape']} precision {precision} vs oracle:",
+            print(f"ensemble member {q} {members[q]['shape']} precision {precision} vs oracle:",
                   json.dumps(err))
             if precision == FP64:
                 assert err["P1_rel_linf"] <= 1e-10 and err["S2_rel_linf"] <= 1e-10
@@ -241,15 +241,15 @@ def test_c4_all_members_batched_equal_alone(hip, orc, c4_members, precision):
         orc.set_threads(1)
 
 
-# ---------------------------------------------------------------- C5 --
+# -------------------------------------------------- 60 s serving stream --
 
 @pytest.mark.timeout(900)
-def test_c5_serving_sample_equal_alone(hip):
-    stream = serving.c5_stream(rate=20.0, duration=60.0)
+def test_serving_60s_sample_equal_alone(hip):
+    stream = serving.request_stream(rate=20.0, duration=60.0)
     step = max(1, len(stream) // 64)
     keep = list(range(0, len(stream), step))[:64]
     stats, res = serving.run(hip, MIXED, stream, keep=keep)
-    print("C5 serving (mixed):", json.dumps(stats))
+    print("serving stream (mixed):", json.dumps(stats))
     assert stats["requests"] == len(stream)
     assert stats["max_active"] >= 2
     assert len(res) == len(keep)

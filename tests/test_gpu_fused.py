@@ -10,7 +10,7 @@ Batching: a member's result is the same alone and in a mix, bit for bit."""
 import numpy as np
 import pytest
 
-from cases import HostGrid, c1_params, max_rel, random_state, ref_params
+from cases import HostGrid, cube32_params, max_rel, random_state, ref_params
 
 pytestmark = pytest.mark.gpu
 
@@ -48,17 +48,17 @@ def fused_run(hip, p, g, steps, precision=0, graph=False, kz=None, tile=(0, 0, 1
         e.close()
 
 
-def c2_like(n, seed=5, sigma=1.0):
+def lognormal_like(n, seed=5, sigma=1.0):
     import importlib
     W = importlib.import_module("shmem-cuda-filtration_amd").workloads
-    p = W.c2_params(n)
+    p = W.lognormal_params(n)
     perm, poro = W.gen_fields(p, seed, sigma=sigma, corr=4.0, clip=2.5)
     return p, W.initial_grid(p, perm, poro)
 
 
 @pytest.mark.parametrize("dims", [(32, 32, 32), (37, 19, 23), (3, 3, 3), (18, 5, 40)])
 def test_fused_fp64_equals_plain_bitwise(hip, dims):
-    p = c1_params()
+    p = cube32_params()
     p.NX, p.Ny, p.Nz = dims
     p.src_x0, p.src_x1 = max(0, dims[0] // 2 - 2), dims[0] // 2 + 1
     p.src_z0, p.src_z1 = max(0, dims[2] // 2 - 2), dims[2] // 2 + 1
@@ -78,7 +78,7 @@ def test_fused_fp64_equals_plain_at_tile_row_ends(hip, dims):
     129, 130 with 64- and 128-wide), and a 4-wide grid has one interior
     column: the fused FP64 form equals the plain path bit for bit for every
     tile width there."""
-    p = c1_params()
+    p = cube32_params()
     p.NX, p.Ny, p.Nz = dims
     p.src_x0, p.src_x1 = max(0, dims[0] // 2 - 2), dims[0] // 2 + 1
     p.src_z0, p.src_z1 = max(0, dims[2] // 2 - 2), dims[2] // 2 + 1
@@ -91,7 +91,7 @@ def test_fused_fp64_equals_plain_at_tile_row_ends(hip, dims):
 
 
 def test_fused_fp64_equals_plain_per_cell_fields(hip):
-    p, g = c2_like(40)
+    p, g = lognormal_like(40)
     Pp, Sp = plain_run(hip, p, g, 30)
     for kz in ((1, 0), (7, 1), (64, 0)):
         Pf, Sf, _ = fused_run(hip, p, g, 30, kz=kz)
@@ -111,8 +111,8 @@ def test_fused_fp64_one_step_random_state_reference_media(hip, orc):
     assert max_rel(Sf, want["S2"]) <= 1e-12
 
 
-def test_fused_fp64_c1_trajectory_vs_oracle(hip, orc):
-    p = c1_params()
+def test_fused_fp64_cube32_trajectory_vs_oracle(hip, orc):
+    p = cube32_params()
     g = HostGrid(p)
     orc.initial_data(g, p)
     Pf, Sf, mf = fused_run(hip, p, g, 500, graph=True)
@@ -141,7 +141,7 @@ TILES = [(16, 16, 1), (32, 8, 1), (64, 4, 1), (32, 16, 1), (64, 8, 1), (64, 8, 2
 def test_every_tile_shape_gives_the_same_bits(hip, precision):
     """The autotuner may pick any tile shape, z-chunk and block order: all
     give the same bits (each cell's arithmetic does not depend on them)."""
-    p, g = c2_like(41)
+    p, g = lognormal_like(41)
     P0, S0, m0 = fused_run(hip, p, g, 20, precision=precision)
     for tile in TILES:
         for kz in ((5, 0), (64, 1)):
@@ -150,23 +150,23 @@ def test_every_tile_shape_gives_the_same_bits(hip, precision):
             assert m == m0
 
 
-def test_mixed_c1_trajectory_error(hip, orc):
-    """Mixed form against the FP64 oracle on configuration 1, 500 steps."""
-    p = c1_params()
+def test_mixed_cube32_trajectory_error(hip, orc):
+    """Mixed form against the FP64 oracle on the homogeneous 32^3 cube, 500 steps."""
+    p = cube32_params()
     g = HostGrid(p)
     orc.initial_data(g, p)
     Pm, Sm, mm = fused_run(hip, p, g, 500, precision=MIXED)
     orc.run(g, p, 500, with_u=0)
     e = errors_vs_oracle(Pm, Sm, mm, g, orc.mass(g, p))
-    print("mixed C1 500 steps vs oracle:", e)
+    print("mixed 32^3 cube 500 steps vs oracle:", e)
     assert e["S2_linf"] <= 1e-4 and e["P1_rel_linf"] <= 1e-6
     assert e["water_mass_rel"] <= 1e-9 and e["total_mass_rel"] <= 1e-9
     assert e["napl_mass_rel"] <= 1e-9
 
 
-def test_fast_c1_trajectory_error(hip, orc):
-    """Fast form against the FP64 oracle on configuration 1, 500 steps."""
-    p = c1_params()
+def test_fast_cube32_trajectory_error(hip, orc):
+    """Fast form against the FP64 oracle on the homogeneous 32^3 cube, 500 steps."""
+    p = cube32_params()
     g = HostGrid(p)
     orc.initial_data(g, p)
     Pm, Sm, mm = fused_run(hip, p, g, 500, precision=FAST)
@@ -176,7 +176,7 @@ def test_fast_c1_trajectory_error(hip, orc):
     mo = orc.mass(g, p)
     e1, e2 = abs(mm[0] - mo[0]) / mo[0], abs(mm[1] - mo[1]) / mo[1]
     et = abs(mm[0] + mm[1] - mo[0] - mo[1]) / (mo[0] + mo[1])
-    print(f"fast C1 500 steps: S2 Linf {eS:.3e}, P1 rel Linf {eP:.3e}, "
+    print(f"fast 32^3 cube 500 steps: S2 Linf {eS:.3e}, P1 rel Linf {eP:.3e}, "
           f"water mass {e1:.3e}, NAPL mass {e2:.3e}, total mass {et:.3e}")
     assert eS <= 1e-4
     assert eP <= 1e-6
@@ -186,7 +186,7 @@ def test_fast_c1_trajectory_error(hip, orc):
 
 @pytest.mark.parametrize("precision", [FAST, MIXED])
 def test_heterogeneous_error(hip, orc, precision):
-    p, g = c2_like(48, seed=9, sigma=1.5)
+    p, g = lognormal_like(48, seed=9, sigma=1.5)
     Pm, Sm, mm = fused_run(hip, p, g, 100, precision=precision)
     orc.set_threads(8)
     orc.run(g, p, 100, with_u=0)
@@ -343,7 +343,7 @@ def test_continuous_batching_add_remove_between_steps(hip):
 
 
 def test_graph_replays_match_eager_steps(hip):
-    p, g = c2_like(26)
+    p, g = lognormal_like(26)
     Pe, Se, _ = fused_run(hip, p, g, 12)
     e = hip.ensemble(0)
     try:

@@ -13,7 +13,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from cases import FIELD_NAMES, FIELD_NAMES_ALL, HostGrid, c1_params, max_rel, random_state, ref_params
+from cases import FIELD_NAMES, FIELD_NAMES_ALL, HostGrid, cube32_params, max_rel, random_state, ref_params
 
 pytestmark = pytest.mark.gpu
 
@@ -115,9 +115,9 @@ def trajectory(hip, orc, p, steps, setup=None, with_u=1):
     return out, a
 
 
-def test_trajectory_c1_500_steps(hip, orc):
+def test_trajectory_cube32_500_steps(hip, orc):
     """Configuration 1 shape: 32^3, 4x4 source patch, Sn = 0.5, 500 steps."""
-    p = c1_params()
+    p = cube32_params()
     out, want = trajectory(hip, orc, p, 500, with_u=0)
     assert max_rel(out["P1"], want["P1"]) <= 1e-10
     assert max_rel(out["S2"], want["S2"]) <= 1e-10
@@ -748,10 +748,10 @@ def test_whole_step_kernel_every_config_equals_two_kernel_path(hip, case, with_u
         lib.filt_set_tile_config(3, 0, 0, 0, 0)
 
 
-def test_whole_step_kernel_c1_trajectory_vs_oracle(hip, orc):
+def test_whole_step_kernel_cube32_trajectory_vs_oracle(hip, orc):
     """Configuration 1 (lambda = 2) through the whole-step kernel, 500 steps,
     against the oracle."""
-    p = c1_params()
+    p = cube32_params()
     g = HostGrid(p)
     orc.initial_data(g, p)
     lib = hip.lib

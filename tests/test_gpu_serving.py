@@ -26,7 +26,7 @@ def alone(hip, precision, req, nsteps):
 
 @pytest.mark.parametrize("precision", [0, 1, 2])
 def test_results_same_alone_and_in_the_mix(hip, precision):
-    stream = serving.c5_stream(rate=40.0, duration=0.4, seed=7, edge_lo=16, edge_hi=40,
+    stream = serving.request_stream(rate=40.0, duration=0.4, seed=7, edge_lo=16, edge_hi=40,
                                steps_lo=10, steps_hi=60)
     assert len(stream) >= 8
     stats, res = serving.run(hip, precision, stream, workers=4, keep=range(len(stream)),

@@ -144,12 +144,12 @@ def test_field_generator_statistics_and_determinism(pkg):
 
 
 def test_stability_guard(pkg):
-    from cases import c1_params
+    from cases import cube32_params
     W = pkg.workloads
-    stated = c1_params(beta1=4.4e-10, beta2=4.4e-10)
+    stated = cube32_params(beta1=4.4e-10, beta2=4.4e-10)
     r = W.stability(stated)
     assert 300 < r < 600  # ~435: 4*dt*K/(mu phi beta)*3/h^2
-    assert W.stability(c1_params()) < 2e-2
+    assert W.stability(cube32_params()) < 2e-2
     p = W.base_params(16, 16, 16, h=W.INI_H)
     perm = np.full((16, 16, 16), 6.64e-11)
     perm[3, 4, 5] = 6.64e-11 * 1000
@@ -159,7 +159,7 @@ def test_stability_guard(pkg):
 
 def test_workload_initial_state(pkg):
     W = pkg.workloads
-    p = W.c1_params()
+    p = W.cube32_params()
     g = W.initial_grid(p)
     assert g["S2"].sum() == pytest.approx(16 * 0.5)
     assert np.all(g["S2"][14:18, 0, 14:18] == 0.5)
@@ -172,7 +172,7 @@ def test_slab_workload_passes_the_guard(pkg, shape):
     orientation: with ln K clipped at +-2 sigma the guard stays below 2 (at
     +-2.5 sigma it would read ~2.6)."""
     W = pkg.workloads
-    p, perm, poro = W.c3_workload(*shape)
+    p, perm, poro = W.slab_workload(*shape)
     assert perm.shape == shape[::-1]
     r = W.stability(p, perm, poro)
     assert 0.5 < r < 1.5
