@@ -694,8 +694,12 @@ def stream_mix_ceiling(kernel, cells):
     exe = os.path.join(ROOT, "tests", "hip", "bin", "stream_mix")
     if not os.path.exists(exe):
         return None
-    out = subprocess.run([exe, str(cells), kernel], capture_output=True, text=True,
-                         timeout=120, check=True).stdout
+    try:
+        out = subprocess.run([exe, str(cells), kernel], capture_output=True, text=True,
+                             timeout=120, check=True).stdout
+    except (subprocess.SubprocessError, OSError) as err:  # context only: never fatal
+        log(f"stream_mix {kernel}: {err}")
+        return None
     rows = [json.loads(x) for x in out.splitlines() if x.startswith("{")]
     if not rows:
         return None
@@ -715,7 +719,12 @@ def tiled_mix_ceiling(kernel, grid, tile, kz, ms=None):
     if not os.path.exists(exe):
         return None
     args = [exe, "tiled", *[str(v) for v in grid], str(tile[0]), str(tile[1]), str(kz), kernel]
-    out = subprocess.run(args, capture_output=True, text=True, timeout=120, check=True).stdout
+    try:
+        out = subprocess.run(args, capture_output=True, text=True, timeout=120,
+                             check=True).stdout
+    except (subprocess.SubprocessError, OSError) as err:  # context only: never fatal
+        log(f"stream_mix tiled {kernel}: {err}")
+        return None
     rows = [json.loads(x) for x in out.splitlines() if x.startswith("{")]
     if not rows:
         return None
