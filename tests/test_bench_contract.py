@@ -49,6 +49,8 @@ def test_every_global_name_resolves():
                     local.add(n.name)
             elif isinstance(n, (ast.Import, ast.ImportFrom)):
                 local |= {(a.asname or a.name).split(".")[0] for a in n.names}
+            elif isinstance(n, ast.ExceptHandler) and n.name:
+                local.add(n.name)
             elif isinstance(n, ast.comprehension):
                 for t in ast.walk(n.target):
                     if isinstance(t, ast.Name):
