@@ -646,6 +646,28 @@ def test_divby_equals_ieee_division():
     assert r.returncode == 0 and res["mismatches"] == 0, line
 
 
+def test_stream_mix_ceilings_flat_and_tiled():
+    """The bench's ceiling tool (tests/hip/bin/stream_mix): a flat and a tiled
+    stream of the dominant kernel's array mix on a small grid each print one
+    JSON line with the mix's byte count and a positive time; a malformed
+    tiled request is refused with status 2."""
+    exe = os.path.join(ROOT, "tests", "hip", "bin", "stream_mix")
+    assert os.path.exists(exe), "build the tools first (make -C tests/hip)"
+    flat = subprocess.run([exe, str(64 * 32 * 16), "k_step_zm"], capture_output=True, text=True,
+                          timeout=120)
+    assert flat.returncode == 0, flat.stderr
+    rows = [json.loads(x) for x in flat.stdout.splitlines() if x.startswith("{")]
+    assert len(rows) == 2 and all(r["bytes_per_cell"] == 136 and r["ms"] > 0 for r in rows)
+    tiled = subprocess.run([exe, "tiled", "64", "32", "16", "64", "8", "4", "k_step_zm"],
+                           capture_output=True, text=True, timeout=120)
+    assert tiled.returncode == 0, tiled.stderr
+    rows = [json.loads(x) for x in tiled.stdout.splitlines() if x.startswith("{")]
+    assert len(rows) == 1 and rows[0]["tiled"] == [64, 8, 4] and rows[0]["ms"] > 0
+    bad = subprocess.run([exe, "tiled", "64", "32", "16", "100", "1", "4"], capture_output=True,
+                         text=True, timeout=60)
+    assert bad.returncode == 2
+
+
 @pytest.mark.parametrize("seed", [0, 1, 2])
 def test_fused_step_extreme_magnitudes_and_specials(hip, seed):
     """The fused step divides by h through a reciprocal for numerators in
