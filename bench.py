@@ -22,6 +22,12 @@ Beside the headline the JSON line carries the other configurations:
                  realization;
   serving        Poisson arrivals at 20 requests/s for 60 s through the
                  continuous-batching server, FP64 vs mixed;
+  divgrad        the reference's tau-relaxed divgrad scheme in place of the
+                 upwind roS stage: the five stage calls per step, the divgrad
+                 kernel's roofline, 3 steps against the oracle port;
+  io             checkpoint (save/save.dat) and Tecplot output from a 128^3
+                 grid on the device, and a snapshot written in the background
+                 while the steps go on;
   cpu_baseline   the reference's own CPU path (oracle/_ref) on one thread on
                  a bounded sample of the headline workload, and on
                  the homogeneous 32^3 cube in full; the oracle port on all threads.
@@ -67,6 +73,9 @@ SPLIT_STAGES = ("ro_P2_Xi", "u", "roS", "newton", "boundary")  # the five stage 
 KERNEL_NAMES = {"ro_P2_Xi": "k_ro_P2_Xi", "u": "k_u_zm", "roS": "k_roS_zm", "newton": "k_newton",
                 "u_roS_newton": "k_step_zm", "step_full": "k_step_full", "unstash": "k_unstash",
                 "boundary": "k_bc_a+k_bc_b"}
+# k_roS_divgrad: ro1, ro2, S2, poro, u1x, u1y, u2x, u2y, roS1_old, roS2_old in;
+# roS1_old, roS2_old, roS1, roS2 out
+DIVGRAD_BYTES = 112
 FORMS = {0: "fused_fp64", 1: "fast", 2: "mixed"}
 # P, S in + out (32) + perm, poro in FP64 (16), every form
 FUSED_BYTES = {0: 48, 1: 48, 2: 48}
@@ -590,6 +599,119 @@ def run_ensemble(args):
     return out
 
 
+def run_divgrad(args, p, perm, poro):
+    """The reference's tau-relaxed divgrad scheme (assign_rS, cpu.cpp:162-216)
+    in place of the upwind roS stage, on the headline grid: the five stage
+    calls per step (the stage path; the one-kernel fusion is upwind only),
+    timed like the stage-call leg, with the divgrad kernel's roofline; then
+    3 steps from the initial state against the oracle port, cell for cell."""
+    import copy
+    import numpy as np
+    q = copy.copy(p)
+    q.roS_scheme = PKG.abi.FILT_SCHEME_DIVGRAD
+    K = max(1, args.steps // 4)
+    r = run_plain(args, q, perm, poro, split=True, steps=K)
+    cells = q.NX * q.Ny * q.Nz
+    ms = r["stage_ms"]["roS"]
+    out = {"value": cells * K / r["wall"], "unit": "cell-updates/s",
+           "ms_per_step": r["wall"] * 1e3 / K, "steps": K, "stages_ms": r["stage_ms"],
+           "state_finite": r["finite"],
+           "bytes_per_cell_update": sum(PLAIN_BYTES[s] for s in SPLIT_STAGES) -
+           PLAIN_BYTES["roS"] + DIVGRAD_BYTES,
+           "k_roS_divgrad": {"bytes_per_cell": DIVGRAD_BYTES, "ms": ms,
+                             "GBps": DIVGRAD_BYTES * cells / (ms * 1e-3) / 1e9,
+                             "frac": DIVGRAD_BYTES * cells / (ms * 1e-3) / HBM_PEAK}}
+    if not args.no_cpu:
+        n = 3
+        be = PKG.device.Backend()
+        d = be.grid(q, perm=True, poro=True)
+        try:
+            d.upload(PKG.workloads.initial_grid(q, perm, poro))
+            d.run(n)
+            d.sync()
+            P1, S2 = d.get("P1"), d.get("S2")
+        finally:
+            d.free()
+        ref, secs, _, _ = oracle_run(q, perm, poro, n)
+        out["oracle_check"] = {
+            "steps": n, "oracle_seconds": secs,
+            "P1_bit_equal": bool(np.array_equal(P1.view(np.uint64), ref["P1"].view(np.uint64))),
+            "S2_bit_equal": bool(np.array_equal(S2.view(np.uint64), ref["S2"].view(np.uint64))),
+            "P1_rel_linf": float(np.max(np.abs(P1 - ref["P1"])) / np.max(np.abs(ref["P1"]))),
+            "S2_linf": float(np.max(np.abs(S2 - ref["S2"])))}
+    return out
+
+
+def run_io(args, n=128, steps=20):
+    """Checkpoint and plot output (save/save.dat and the Tecplot files of
+    main.cpp:319-569) from a grid resident on the device, n^3 cells: the
+    synchronous writes, and a snapshot written by its background thread
+    while the time steps go on.  Files go to a temporary folder that is
+    removed afterwards."""
+    import shutil
+    import tempfile
+    hiprt = importlib.import_module(PKG.__name__ + ".hiprt")
+    be = PKG.device.Backend()
+    p, perm, poro = PKG.workloads.lognormal_workload(n)
+    d = be.grid(p, perm=True, poro=True)
+    tmp = tempfile.mkdtemp(prefix="filt_io_")
+    out = {"grid": [p.NX, p.Ny, p.Nz]}
+    try:
+        d.upload(PKG.workloads.initial_grid(p, perm, poro))
+        d.run(5)
+        d.sync()
+
+        def files_bytes(path):
+            if os.path.isfile(path):
+                return os.path.getsize(path)
+            return sum(os.path.getsize(os.path.join(a, f)) for a, _, fs in os.walk(path)
+                       for f in fs)
+
+        save = os.path.join(tmp, "save.dat")
+        t0 = time.perf_counter()
+        d.save(save, 5)
+        ts = time.perf_counter() - t0
+        b = files_bytes(save)
+        out["save"] = {"seconds": ts, "bytes": b, "MBps": b / ts / 1e6}
+        plots = os.path.join(tmp, "plots")
+        t0 = time.perf_counter()
+        d.save_plots(plots, 5 * p.dt)
+        tp = time.perf_counter() - t0
+        b = files_bytes(plots)
+        out["plots"] = {"seconds": tp, "bytes": b, "MBps": b / tp / 1e6}
+        # time steps alone, then with a snapshot written in the background
+        stream = be.lib.filt_get_stream()
+        a, c = hiprt.Event(), hiprt.Event()
+        a.record(stream)
+        d.run(steps, step0=5)
+        c.record(stream)
+        c.sync()
+        alone = a.elapsed_ms(c) / steps
+        snap = PKG.device.Snapshot(d)
+        try:
+            t0 = time.perf_counter()
+            snap.take(d, 25, 25 * p.dt)
+            d.sync()
+            take_ms = (time.perf_counter() - t0) * 1e3
+            t0 = time.perf_counter()
+            snap.write_async(os.path.join(tmp, "save_async.dat"), os.path.join(tmp, "plots_async"))
+            a.record(stream)
+            d.run(steps, step0=25)
+            c.record(stream)
+            c.sync()
+            during = a.elapsed_ms(c) / steps
+            snap.wait()
+            tw = time.perf_counter() - t0
+        finally:
+            snap.close()
+        out["snapshot"] = {"take_ms": take_ms, "write_seconds": tw,
+                           "step_ms_alone": alone, "step_ms_during_write": during}
+    finally:
+        d.free()
+        shutil.rmtree(tmp, ignore_errors=True)
+    return out
+
+
 def run_serving(args):
     """Serving configuration: Poisson arrivals (rate args.serve_rate per s for
     args.serve_duration s), edges log-uniform in [32, 192], steps U[100, 2000],
@@ -791,9 +913,10 @@ def main():
     ap.add_argument("--form", choices=("fp64", "fast", "mixed"),
                     help="only run this fused form on the headline grid and print its line "
                          "(for profiles)")
-    ap.add_argument("--leg", choices=("ensemble", "serving"),
-                    help="only run this leg (the batched ensemble or the serving loop) and "
-                         "print its entry (for profiles)")
+    ap.add_argument("--leg", choices=("ensemble", "serving", "divgrad", "io"),
+                    help="only run this leg (the batched ensemble, the serving loop, the "
+                         "divgrad scheme or the checkpoint and plot output) and print its "
+                         "entry (for profiles)")
     ap.add_argument("--tiling", default=None,
                     help="with --form: fixed tiling TXxTY[xCY],kz,order (e.g. 64x8x2,64,1) "
                          "instead of the autotuner")
@@ -809,7 +932,12 @@ def main():
         print(json.dumps(run_one_form(args)), flush=True)
         return
     if args.leg:
-        print(json.dumps(run_ensemble(args) if args.leg == "ensemble" else run_serving(args)), flush=True)
+        if args.leg == "divgrad":
+            p, perm, poro = PKG.workloads.lognormal_workload(args.n)
+            res = run_divgrad(args, p, perm, poro)
+        else:
+            res = {"ensemble": run_ensemble, "serving": run_serving, "io": run_io}[args.leg](args)
+        print(json.dumps(res), flush=True)
         return
 
     p, perm, poro = PKG.workloads.lognormal_workload(args.n)
@@ -865,6 +993,10 @@ def main():
             oracle_check = {name: form_errors(P, S, m, ref["P1"], ref["S2"], mref,
                                            f"oracle port after {args.steps} steps")
                          for name, (P, S, m, _) in states.items()}
+        log("divgrad scheme")
+        forms["divgrad"] = run_divgrad(args, p, perm, poro)
+        log("checkpoint and plot output")
+        forms["io"] = run_io(args)
         if args.slab_steps > 0:
             for shape, with_oracle in (((512, 512, 256), not args.no_cpu),
                                        ((256, 512, 512), False)):
