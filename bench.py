@@ -599,6 +599,17 @@ def run_ensemble(args):
     return out
 
 
+def auxiliary(name, fn, *a):
+    """A leg beside the headline whose failure (a file system the box does
+    not let us write, say) is reported in its entry instead of ending the
+    run."""
+    try:
+        return fn(*a)
+    except Exception as err:  # noqa: BLE001 -- recorded, the headline goes on
+        log(f"{name}: {type(err).__name__}: {err}")
+        return {"error": f"{type(err).__name__}: {err}"}
+
+
 def run_divgrad(args, p, perm, poro):
     """The reference's tau-relaxed divgrad scheme (assign_rS, cpu.cpp:162-216)
     in place of the upwind roS stage, on the headline grid: the five stage
@@ -994,9 +1005,9 @@ def main():
                                            f"oracle port after {args.steps} steps")
                          for name, (P, S, m, _) in states.items()}
         log("divgrad scheme")
-        forms["divgrad"] = run_divgrad(args, p, perm, poro)
+        forms["divgrad"] = auxiliary("divgrad", run_divgrad, args, p, perm, poro)
         log("checkpoint and plot output")
-        forms["io"] = run_io(args)
+        forms["io"] = auxiliary("io", run_io, args)
         if args.slab_steps > 0:
             for shape, with_oracle in (((512, 512, 256), not args.no_cpu),
                                        ((256, 512, 512), False)):

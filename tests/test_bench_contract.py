@@ -40,6 +40,9 @@ def test_every_global_name_resolves():
     known = set(dir(b)) | set(dir(builtins))
     for fn in [n for n in tree.body if isinstance(n, ast.FunctionDef)]:
         local = {a.arg for a in fn.args.args + fn.args.kwonlyargs}
+        for extra in (fn.args.vararg, fn.args.kwarg):
+            if extra is not None:
+                local.add(extra.arg)
         for n in ast.walk(fn):
             if isinstance(n, ast.Name) and isinstance(n.ctx, ast.Store):
                 local.add(n.id)
