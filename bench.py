@@ -73,7 +73,7 @@ SPLIT_STAGES = ("ro_P2_Xi", "u", "roS", "newton", "boundary")  # the five stage 
 KERNEL_NAMES = {"ro_P2_Xi": "k_ro_P2_Xi", "u": "k_u_zm", "roS": "k_roS_zm", "newton": "k_newton",
                 "u_roS_newton": "k_step_zm", "step_full": "k_step_full", "unstash": "k_unstash",
                 "boundary": "k_bc_a+k_bc_b"}
-# k_roS_divgrad: ro1, ro2, S2, poro, u1x, u1y, u2x, u2y, roS1_old, roS2_old in;
+# k_roS_divgrad_zm: ro1, ro2, S2, poro, u1x, u1y, u2x, u2y, roS1_old, roS2_old in;
 # roS1_old, roS2_old, roS1, roS2 out
 DIVGRAD_BYTES = 112
 FORMS = {0: "fused_fp64", 1: "fast", 2: "mixed"}
@@ -629,9 +629,17 @@ def run_divgrad(args, p, perm, poro):
            "state_finite": r["finite"],
            "bytes_per_cell_update": sum(PLAIN_BYTES[s] for s in SPLIT_STAGES) -
            PLAIN_BYTES["roS"] + DIVGRAD_BYTES,
-           "k_roS_divgrad": {"bytes_per_cell": DIVGRAD_BYTES, "ms": ms,
-                             "GBps": DIVGRAD_BYTES * cells / (ms * 1e-3) / 1e9,
-                             "frac": DIVGRAD_BYTES * cells / (ms * 1e-3) / HBM_PEAK}}
+           "k_roS_divgrad_zm": {"bytes_per_cell": DIVGRAD_BYTES, "ms": ms,
+                                "GBps": DIVGRAD_BYTES * cells / (ms * 1e-3) / 1e9,
+                                "frac": DIVGRAD_BYTES * cells / (ms * 1e-3) / HBM_PEAK,
+                                "tiles": r["tiles"]["roS"]}}
+    flat = stream_mix_ceiling("k_roS_divgrad_zm", cells)
+    if flat:
+        flat["kernel_frac_of_mix"] = flat["ms"] / ms
+    tl = r["tiles"]["roS"]
+    out["k_roS_divgrad_zm"]["stream_mix_ceiling"] = flat
+    out["k_roS_divgrad_zm"]["tiled_mix_ceiling"] = tiled_mix_ceiling(
+        "k_roS_divgrad_zm", (q.NX, q.Ny, q.Nz), tl["tile"], tl["kz"], ms)
     if not args.no_cpu:
         n = 3
         be = PKG.device.Backend()

@@ -48,6 +48,11 @@ bool step_tile_ok(int tx, int ty);
 int launch_step_full(const filt_fields& f, const filt_params& p, int nx, bool with_u,
                      hipEvent_t mid = nullptr);
 int launch_roS_zm(const filt_fields& f, const filt_params& p, int nx);
+// The divgrad scheme's roS stage, z-marched with the roS kernel's tile
+// configuration; divgrad_tiled_ok tells whether its eight uniform divisors
+// (h^2, 2h, dt^2, 2dt^2) are in DivBy's range.
+bool divgrad_tiled_ok(const filt_params& p);
+int launch_roS_divgrad_zm(const filt_fields& f, const filt_params& p, int nx, double t);
 int launch_u_zm(const filt_fields& f, const filt_params& p, int nx, int rank, int size);
 
 // The no-allocation mass sum's scratch address on the current device

@@ -553,6 +553,8 @@ int filt_roS_calculation(const filt_fields* dev, const filt_params* p, double t,
     if (p->roS_scheme == FILT_SCHEME_DIVGRAD) {
         if (!dev->u1x || !dev->u1y || !dev->u2x || !dev->u2y || !dev->roS1_old || !dev->roS2_old)
             return set_error(FILT_ERR_INVALID, "the divgrad scheme reads u and roS_old");
+        if (use_tiled(p, localNx, 0) && divgrad_tiled_ok(*p))
+            return launch_roS_divgrad_zm(*dev, *p, localNx, t);
         hipLaunchKernelGGL(k_roS_divgrad, cell_grid(p, localNx), dim3(TX, TY), 0, stream(), *dev,
                            *p, localNx, rank, size, t);
         FILT_LAUNCH_CHECK("k_roS_divgrad");

@@ -186,6 +186,210 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(4))) voi
     }
 }
 
+// -------------------------------------------------------- roS, divgrad ----
+// cpu.cpp:162-216 (assign_rS) for interior cells: the tau-relaxed scheme.
+// Its x/y stencil operands are six products, each formed once per cell as
+// the plane is staged: a1 = ro1*(1-S2) and a2 = ro2*S2 (diffusion), b1x =
+// ro1*u1x and b2x = ro2*u2x (x transport), b1y = ro1*u1y and b2y = ro2*u2y
+// (y transport).  Each is the product the reference forms at that neighbour,
+// so the bits are the same.  Along z the own column's a1, a2 and b2y of plane
+// k+1 are formed from registers; those of plane k-1, with b1x, are read back
+// from the thread's own slots of the other LDS tile (no other thread writes
+// them); the other term of the water z transport, ro1*u1x at (i+1, j, k+1),
+// is loaded one plane ahead for every cell.  The divisions by h^2, 2h, dt^2
+// and 2dt^2 go through DivBy, their divisors and reciprocals read from LDS
+// where they are used (held in registers across the loop they would take 32
+// VGPRs); the two by the cell's porosity are IEEE divisions.  Ring cells of
+// the two rows (h < 2*TX) carry the y transport products, those of the two
+// columns the x ones.  PORO: porosity per cell, else by media index.  At 4
+// waves per SIMD one value is kept in scratch across the rare-numerator
+// calls of DivBy.
+template <int TX, int TY, bool PORO>
+__global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(4))) void k_roS_divgrad_zm(
+    filt_fields f, filt_params p, int nx, TileMap map, int kz, double t)
+{
+    using L = Lds<TX, TY>;
+    enum { A1, A2, B1X, B2X, B1Y, B2Y, NA };
+    __shared__ double shb[2][NA * L::SN];
+    // for DivBy: reciprocals of h1^2, h2^2, h3^2, 2h1, 2h2, 2h3, dt^2, 2dt^2,
+    // then the divisors themselves
+    __shared__ double rr[16];
+    int bx, by, bz;
+    map.decode(blockIdx.x, bx, by, bz);
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int i0 = bx * TX, j0 = by * TY;
+    const int i = i0 + tx, j = j0 + ty;
+    const int ny = p.Ny, nz = p.Nz;
+    const bool valid = i < nx && j < ny;
+    const bool inner = i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2;
+    const unsigned sy = 8u * nx, sz = 8u * nx * ny;  // byte strides
+    const int kb = max(bz * kz, 1), ke = min(bz * kz + kz, nz - 1);
+    if (kb >= ke)
+        return;  // uniform over the block
+    const unsigned col = 8u * i + sy * j;
+    // (i+1, j), clamped into the grid for the lanes whose value is unused
+    const unsigned scol = 8u * min(i + 1, nx - 1) + sy * j;
+    int hslot;
+    unsigned hcol;
+    const bool hv = ring_slot<TX, TY>(tx, ty, i0, j0, nx, ny, sy, hslot, hcol);
+    // a ring column cell (x transport), else a row cell (y transport); the
+    // ring cells of one wave are all of one kind for tiles with TX >= 32
+    const bool hx = tx + TX * ty >= 2 * TX;
+    const int hb = hslot + (hx ? B1X : B1Y) * L::SN;  // its transport products
+    const bool by_media = !PORO && f.media && p.n_media > 1;
+    // Own column: ro1, ro2, S2, u2y at planes k, k+1 and (in flight) k+2;
+    // u1x, u1y, u2x and roS1_old, roS2_old, porosity at k and (in flight)
+    // k+1; ro1, u1x at (i+1, j, k+1) and (in flight) k+2.  Ring cell of the
+    // next plane: ro1, ro2, S2 and its two u components.
+    double zc[4] = {}, zp[4] = {}, zq[4] = {}, xc[3] = {}, xn[3] = {};
+    double pc[3] = {}, pn[3] = {}, sc[2] = {}, sn[2] = {}, hn[5] = {};
+    int mdc = 0, mdn = 0;
+    if (valid) {
+        const unsigned c = col + kb * sz;
+        // plane kb-1's products to the own slots of the tile plane kb reads
+        // as its predecessor
+        const double r1m = ld(f.ro1, c - sz), r2m = ld(f.ro2, c - sz), Sm = ld(f.S2, c - sz);
+        shb[1][L::at(A1, ty + 1, tx + 1)] = r1m * (1. - Sm);
+        shb[1][L::at(A2, ty + 1, tx + 1)] = r2m * Sm;
+        shb[1][L::at(B1X, ty + 1, tx + 1)] = r1m * ld(f.u1x, c - sz);
+        shb[1][L::at(B2Y, ty + 1, tx + 1)] = r2m * ld(f.u2y, c - sz);
+        zc[0] = ld(f.ro1, c), zc[1] = ld(f.ro2, c), zc[2] = ld(f.S2, c), zc[3] = ld(f.u2y, c);
+        zp[0] = ld(f.ro1, c + sz), zp[1] = ld(f.ro2, c + sz), zp[2] = ld(f.S2, c + sz);
+        zp[3] = ld(f.u2y, c + sz);  // kb + 1 <= ke <= nz - 1
+        xc[0] = ld(f.u1x, c), xc[1] = ld(f.u1y, c), xc[2] = ld(f.u2x, c);
+        pc[0] = ld(f.roS1_old, c), pc[1] = ld(f.roS2_old, c);
+        if (PORO)
+            pc[2] = ld(f.poro, c);
+        else if (by_media)
+            mdc = ldi(f.media, c >> 1);
+        sc[0] = ld(f.ro1, scol + (kb + 1) * sz), sc[1] = ld(f.u1x, scol + (kb + 1) * sz);
+    }
+    if (hv) {
+        const unsigned h = hcol + kb * sz;
+        hn[0] = ld(f.ro1, h), hn[1] = ld(f.ro2, h), hn[2] = ld(f.S2, h);
+        if (hx)
+            hn[3] = ld(f.u1x, h), hn[4] = ld(f.u2x, h);
+        else
+            hn[3] = ld(f.u1y, h), hn[4] = ld(f.u2y, h);
+    }
+    if (tx == 0 && ty == 0) {
+        const double dt = p.dt;
+        rr[0] = refined_reciprocal(p.h1 * p.h1);
+        rr[1] = refined_reciprocal(p.h2 * p.h2);
+        rr[2] = refined_reciprocal(p.h3 * p.h3);
+        rr[3] = refined_reciprocal(2. * p.h1);
+        rr[4] = refined_reciprocal(2. * p.h2);
+        rr[5] = refined_reciprocal(2. * p.h3);
+        rr[6] = refined_reciprocal(dt * dt);
+        rr[7] = refined_reciprocal(2. * dt * dt);
+        rr[8] = p.h1 * p.h1, rr[9] = p.h2 * p.h2, rr[10] = p.h3 * p.h3;
+        rr[11] = 2. * p.h1, rr[12] = 2. * p.h2, rr[13] = 2. * p.h3;
+        rr[14] = dt * dt, rr[15] = 2. * dt * dt;
+    }
+    const int cx = tx + 1, cy = ty + 1;
+    for (int k = kb; k < ke; k++) {
+        double* const sh = shb[(k - kb) & 1];
+        const double* const shm = shb[(k - kb + 1) & 1];  // own slots: plane k-1
+        const unsigned c = col + k * sz;
+        const double b1xs = sc[0] * sc[1];  // ro1*u1x at (i+1, j, k+1)
+        if (valid) {
+            sh[L::at(A1, cy, cx)] = zc[0] * (1. - zc[2]);
+            sh[L::at(A2, cy, cx)] = zc[1] * zc[2];
+            sh[L::at(B1X, cy, cx)] = zc[0] * xc[0];
+            sh[L::at(B2X, cy, cx)] = zc[1] * xc[2];
+            sh[L::at(B1Y, cy, cx)] = zc[0] * xc[1];
+            sh[L::at(B2Y, cy, cx)] = zc[1] * zc[3];
+        }
+        if (hv) {
+            sh[hslot + A1 * L::SN] = hn[0] * (1. - hn[2]);
+            sh[hslot + A2 * L::SN] = hn[1] * hn[2];
+            sh[hb] = hn[0] * hn[3];
+            sh[hb + L::SN] = hn[1] * hn[4];
+        }
+        if (k + 1 < ke) {  // next plane is evaluated: fetch its operands
+            if (valid) {
+                const unsigned c1 = c + sz, c2 = c + 2 * sz;  // k+2 <= ke
+                zq[0] = ld(f.ro1, c2), zq[1] = ld(f.ro2, c2), zq[2] = ld(f.S2, c2);
+                zq[3] = ld(f.u2y, c2);
+                xn[0] = ld(f.u1x, c1), xn[1] = ld(f.u1y, c1), xn[2] = ld(f.u2x, c1);
+                pn[0] = ld(f.roS1_old, c1), pn[1] = ld(f.roS2_old, c1);
+                if (PORO)
+                    pn[2] = ld(f.poro, c1);
+                else if (by_media)
+                    mdn = ldi(f.media, c1 >> 1);
+                sn[0] = ld(f.ro1, scol + (k + 2) * sz), sn[1] = ld(f.u1x, scol + (k + 2) * sz);
+            }
+            if (hv) {
+                const unsigned h = hcol + (k + 1) * sz;
+                hn[0] = ld(f.ro1, h), hn[1] = ld(f.ro2, h), hn[2] = ld(f.S2, h);
+                if (hx)
+                    hn[3] = ld(f.u1x, h), hn[4] = ld(f.u2x, h);
+                else
+                    hn[3] = ld(f.u1y, h), hn[4] = ld(f.u2y, h);
+            }
+        }
+        __syncthreads();
+        if (valid && inner) {
+            // an index the compiler cannot see through keeps the LDS reads
+            // of the divisors here rather than hoisted out of the loop
+            int z0 = 0;
+            asm volatile("" : "+v"(z0));
+            const double* const q = rr + z0;
+            const DivBy dxx{q[8], q[0]}, dyy{q[9], q[1]}, dzz{q[10], q[2]};
+            const DivBy dx2{q[11], q[3]}, dy2{q[12], q[4]}, dz2{q[13], q[5]};
+            const double r1c = zc[0], r2c = zc[1], Sc = zc[2];
+            const double roS1 = r1c * (1 - Sc);
+            const double roS2 = r2c * Sc;
+            const double m = PORO ? pc[2] : p.m[mdc];
+            const double k1 = m * p.l_w * p.c / 2.;
+            const double k2 = m * p.l_n * p.c / 2.;
+            const double t1 = 2 * r1c * (1. - Sc), t2 = 2 * r2c * Sc;
+            double dg1 = dzz(k1 * (zp[0] * (1. - zp[2]) - t1 + shm[L::at(A1, cy, cx)]));
+            double dg2 = dzz(k2 * (zp[1] * zp[2] - t2 + shm[L::at(A2, cy, cx)]));
+            const double Tz1 = dz2(b1xs - shm[L::at(B1X, cy, cx)]);
+            const double Tz2 = dz2(zp[1] * zp[3] - shm[L::at(B2Y, cy, cx)]);
+            const int xp = cx + 1, xm = cx - 1, yp = cy + 1, ym = cy - 1;
+            dg1 += k1 * (dxx(sh[L::at(A1, cy, xp)] - t1 + sh[L::at(A1, cy, xm)]) +
+                         dyy(sh[L::at(A1, yp, cx)] - t1 + sh[L::at(A1, ym, cx)]));
+            dg2 += k2 * (dxx(sh[L::at(A2, cy, xp)] - t2 + sh[L::at(A2, cy, xm)]) +
+                         dyy(sh[L::at(A2, yp, cx)] - t2 + sh[L::at(A2, ym, cx)]));
+            const double Tx1 = dx2(sh[L::at(B1X, cy, xp)] - sh[L::at(B1X, cy, xm)]);
+            const double Ty1 = dy2(sh[L::at(B1Y, yp, cx)] - sh[L::at(B1Y, ym, cx)]);
+            const double Tx2 = dx2(sh[L::at(B2X, cy, xp)] - sh[L::at(B2X, cy, xm)]);
+            const double Ty2 = dy2(sh[L::at(B2Y, yp, cx)] - sh[L::at(B2Y, ym, cx)]);
+            const double dt = p.dt, tau = p.tau;
+            double A1v, A2v;
+            if (t < 2 * dt) {
+                A1v = roS1 + (dt / m) * (dg1 - Tx1 - Ty1 - Tz1);
+                A2v = roS2 + (dt / m) * (dg2 - Tx2 - Ty2 - Tz2);
+            } else {
+                const DivBy dtt{q[14], q[6]}, d2tt{q[15], q[7]};
+                const double w = (2. * dt * dt) / (m * (dt + 2. * tau));
+                A1v = w * (dg1 - Tx1 - Ty1 - Tz1 + dtt(2. * roS1 * m * tau) +
+                           d2tt(pc[0] * m * (dt - 2. * tau)));
+                A2v = w * (dg2 - Tx2 - Ty2 - Tz2 + dtt(2. * roS2 * m * tau) +
+                           d2tt(pc[1] * m * (dt - 2. * tau)));
+            }
+            stn(f.roS1_old, c, roS1);
+            stn(f.roS2_old, c, roS2);
+            stn(f.roS1, c, A1v);
+            stn(f.roS2, c, A2v);
+        }
+#pragma unroll
+        for (int a = 0; a < 4; a++) {
+            zc[a] = zp[a];
+            zp[a] = zq[a];
+        }
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            xc[a] = xn[a];
+            pc[a] = pn[a];
+        }
+        sc[0] = sn[0], sc[1] = sn[1];
+        mdc = mdn;
+    }
+}
+
 // ------------------------------------------------------------------ u ----
 // cpu.cpp:77-160 for every active cell; stencil operands P1, P2 only.
 template <int TX, int TY>
@@ -337,6 +541,23 @@ RoSFn pick_roS(int tx, int ty)
     return nullptr;
 }
 
+using DivgradFn = void (*)(filt_fields, filt_params, int, TileMap, int, double);
+
+template <int TX, int TY>
+DivgradFn divgrad_of(bool poro)
+{
+    return poro ? k_roS_divgrad_zm<TX, TY, true> : k_roS_divgrad_zm<TX, TY, false>;
+}
+
+DivgradFn pick_divgrad(int tx, int ty, bool poro)
+{
+    if (tx == 64 && ty == 4) return divgrad_of<64, 4>(poro);
+    if (tx == 32 && ty == 8) return divgrad_of<32, 8>(poro);
+    if (tx == 16 && ty == 16) return divgrad_of<16, 16>(poro);
+    if (tx == 64 && ty == 8) return divgrad_of<64, 8>(poro);
+    return nullptr;
+}
+
 UFn pick_u(int tx, int ty)
 {
     if (tx == 64 && ty == 4) return k_u_zm<64, 4>;
@@ -423,6 +644,24 @@ int launch_roS_zm(const filt_fields& f, const filt_params& p, int nx)
     hipLaunchKernelGGL(pick_roS(t.tx, t.ty), dim3((unsigned)m.count()), dim3(t.tx, t.ty), 0,
                        stream(), f, p, nx, m, t.kz);
     FILT_LAUNCH_CHECK("k_roS_zm");
+    return FILT_OK;
+}
+
+bool divgrad_tiled_ok(const filt_params& p)
+{
+    const double dt = p.dt;
+    return divby_ok(p.h1 * p.h1) && divby_ok(p.h2 * p.h2) && divby_ok(p.h3 * p.h3) &&
+           divby_ok(2. * p.h1) && divby_ok(2. * p.h2) && divby_ok(2. * p.h3) &&
+           divby_ok(dt * dt) && divby_ok(2. * dt * dt);
+}
+
+int launch_roS_divgrad_zm(const filt_fields& f, const filt_params& p, int nx, double t)
+{
+    const TileConfig c = cfg_for(0, nx, p.Ny, p.Nz);
+    const TileMap m = make_map(nx, p.Ny, p.Nz, c);
+    hipLaunchKernelGGL(pick_divgrad(c.tx, c.ty, f.poro != nullptr), dim3((unsigned)m.count()),
+                       dim3(c.tx, c.ty), 0, stream(), f, p, nx, m, c.kz, t);
+    FILT_LAUNCH_CHECK("k_roS_divgrad_zm");
     return FILT_OK;
 }
 

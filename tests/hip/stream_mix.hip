@@ -11,6 +11,8 @@
 //   k_ro_P2_Xi    3 in,  5 out   S2, P1, perm -> P2, Xi1, Xi2, ro1, ro2
 //   k_u_zm        6 in,  6 out   Xi1, Xi2, ro1, ro2, P1, P2 -> six u
 //   k_roS_zm      8 in,  4 out   S2, P1, P2, Xi1, Xi2, ro1, ro2, poro -> four roS
+//   k_roS_divgrad_zm 10 in, 4 out  ro1, ro2, S2, poro, u1x, u1y, u2x, u2y,
+//                                roS1_old, roS2_old -> roS1_old, roS2_old, roS1, roS2
 //   k_newton      4 in,  2 out   S2, P1, roS1, roS2 -> P1, S2
 //   k_step_zm     5 in, 12 out   P2, Xi1, Xi2, S2, poro -> six u, four roS, P1, S2
 //   k_step_full   4 in, 17 out   P1, S2, perm, poro -> P2, Xi1, Xi2, ro1, ro2,
@@ -43,7 +45,7 @@
 
 namespace {
 
-constexpr int kMaxIn = 8, kMaxOut = 17;
+constexpr int kMaxIn = 10, kMaxOut = 17;
 
 struct Streams {
     const double* in[kMaxIn];
@@ -227,6 +229,7 @@ int main(int argc, char** argv)
         if (want("k_ro_P2_Xi")) run_tiled<3, 5>("k_ro_P2_Xi", s, tl);
         if (want("k_u_zm")) run_tiled<6, 6>("k_u_zm", s, tl);
         if (want("k_roS_zm")) run_tiled<8, 4>("k_roS_zm", s, tl);
+        if (want("k_roS_divgrad_zm")) run_tiled<10, 4>("k_roS_divgrad_zm", s, tl);
         if (want("k_newton")) run_tiled<4, 2>("k_newton", s, tl);
         if (want("k_step_zm")) run_tiled<5, 12>("k_step_zm", s, tl);
         if (want("k_step_full")) run_tiled<4, 17>("k_step_full", s, tl);
@@ -248,6 +251,7 @@ int main(int argc, char** argv)
         if (want("k_ro_P2_Xi")) both<3, 5>("k_ro_P2_Xi", s, n);
         if (want("k_u_zm")) both<6, 6>("k_u_zm", s, n);
         if (want("k_roS_zm")) both<8, 4>("k_roS_zm", s, n);
+        if (want("k_roS_divgrad_zm")) both<10, 4>("k_roS_divgrad_zm", s, n);
         if (want("k_newton")) both<4, 2>("k_newton", s, n);
         if (want("k_step_zm")) both<5, 12>("k_step_zm", s, n);
         if (want("k_step_full")) both<4, 17>("k_step_full", s, n);

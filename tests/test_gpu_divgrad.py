@@ -17,7 +17,8 @@ def divgrad_params(*dims, **kw):
     return p
 
 
-@pytest.mark.parametrize("dims", [(7, 6, 5), (70, 9, 6), (9, 8, 1), (33, 17, 11)])
+@pytest.mark.parametrize("dims", [(7, 6, 5), (70, 9, 6), (9, 8, 1), (33, 17, 11), (3, 3, 3),
+                                  (130, 7, 70)])
 @pytest.mark.parametrize("tstep", [1, 4])
 def test_divgrad_stage_bitwise(hip, orc, dims, tstep):
     p = divgrad_params(*dims)
@@ -33,6 +34,70 @@ def test_divgrad_stage_bitwise(hip, orc, dims, tstep):
     orc.stage(3, g, p, t=tstep * p.dt)
     for n in FIELD_NAMES_ALL:
         assert np.array_equal(out[n], g[n]), (n, dims, tstep)
+
+
+TILES = [(64, 4), (32, 8), (16, 16), (64, 8)]
+
+
+@pytest.mark.parametrize("tile", TILES)
+@pytest.mark.parametrize("kz,order", [(1, 0), (5, 1), (64, 0)])
+def test_divgrad_tiled_every_config_bitwise(hip, orc, tile, kz, order):
+    """The z-marching divgrad kernel at every tile shape, z-chunk and block
+    order, with porosity per cell and by the media map, on both sides of the
+    time test (plain Euler while t < 2 dt): the oracle's bits (partial tiles
+    in x and y, chunk ends not aligned with the grid)."""
+    lib = hip.lib
+    p = divgrad_params(67, 19, 23)
+    try:
+        assert lib.filt_set_tile_config(0, tile[0], tile[1], kz, order) == 0
+        for poro in (False, True):
+            for tstep in (1, 4):
+                rng = np.random.default_rng(kz * 1000 + tile[0] * 10 + order + 2 * tstep + poro)
+                g = random_state(p, rng, poro=poro)
+                d = hip.grid(p, poro=poro)
+                try:
+                    d.upload(g)
+                    d.stage(3, t=tstep * p.dt)
+                    out = g.copy()
+                    d.download(out)
+                finally:
+                    d.free()
+                orc.stage(3, g, p, t=tstep * p.dt)
+                for n in FIELD_NAMES_ALL:
+                    assert np.array_equal(out[n], g[n]), (n, tile, kz, order, poro, tstep)
+    finally:
+        lib.filt_set_tile_config(0, 0, 0, 0, 0)
+
+
+def test_divgrad_untiled_switch_and_divisor_range(hip, orc):
+    """The one-cell-per-thread kernel (tiled kernels switched off, or a
+    divisor h^2 below the range of the tiled kernel's exact division) gives
+    the same bits as the tiled one."""
+    lib = hip.lib
+    outs = []
+    for mode in ("tiled", "untiled", "tiny_h"):
+        kw = {"h1": 1e-20} if mode == "tiny_h" else {}
+        p = divgrad_params(40, 12, 9, **kw)
+        g = random_state(p, np.random.default_rng(5), poro=True)
+        try:
+            if mode == "untiled":
+                assert lib.filt_set_tile_config(0, -1, 0, 0, 0) == 0
+            d = hip.grid(p, poro=True)
+            try:
+                d.upload(g)
+                d.stage(3, t=4 * p.dt)
+                out = g.copy()
+                d.download(out)
+            finally:
+                d.free()
+        finally:
+            lib.filt_set_tile_config(0, 0, 0, 0, 0)
+        orc.stage(3, g, p, t=4 * p.dt)
+        for n in FIELD_NAMES_ALL:
+            assert np.array_equal(out[n], g[n]), (n, mode)
+        outs.append(out)
+    for n in ("roS1", "roS2", "roS1_old", "roS2_old"):
+        assert np.array_equal(outs[0][n], outs[1][n]), n
 
 
 @pytest.mark.parametrize("dims,steps", [((20, 16, 12), 60), ((10, 9, 1), 30)])
