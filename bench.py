@@ -153,7 +153,9 @@ def run_plain(args, p, perm, poro, split=False, steps=None):
     time step as filt_time_step runs it: stage 1, then stages 2-4 in one
     z-march, with the boundary conditions folded into those two launches
     where the grid allows it (one rank, every extent >= 3; the third stage
-    interval then holds no launch).  split=True: the five defines.h stage
+    interval then holds no launch), the stages 2-4 kernel's tile chosen for
+    the grid by the library's autotuner before the warmup.  split=True: the
+    five defines.h stage
     functions called one after another, as the reference's own driver calls
     them."""
     hiprt = importlib.import_module(PKG.__name__ + ".hiprt")
@@ -184,19 +186,29 @@ def run_plain(args, p, perm, poro, split=False, steps=None):
                                                     ctypes.byref(path)), "path probe")
         names = STEP_STAGES if path.value == 1 else TWO_KERNEL_STAGES
         calls = None
+        if path.value == 0:
+            # the library's autotuner picks the stages 2-4 kernel's tile for
+            # this grid (it times the kernel's variant without the folded
+            # boundary, so the time step's own kernel launches only here)
+            step_best = ctypes.c_double()
+            PKG.abi.check(lib, lib.filt_autotune_step(fref, pref, nx, 3, ctypes.byref(step_best)),
+                          "step autotune")
     best = (ctypes.c_double * 2)()
     if args.autotune and split:
         # stage 1 fills the inputs the stencil kernels read, then the
         # library's autotuner picks tile shapes for this grid
         PKG.abi.check(lib, stage1(0.0), "stage 1")
         PKG.abi.check(lib, lib.filt_autotune_stencil(fref, pref, nx, 5, best), "autotune")
-    tuned = {"autotuned": bool(args.autotune and split)}
+    tuned = {"autotuned": bool(args.autotune and split),
+             "step_autotuned": bool(not split and path.value == 0)}
     for kern, name in ((0, "roS"), (1, "u"), (2, "u_roS_newton"), (3, "step_full")):
         v = [ctypes.c_int() for _ in range(4)]
         lib.filt_get_tile_config(kern, nx, p.Ny, p.Nz, *[ctypes.byref(x) for x in v])
         tuned[name] = {"tile": [v[0].value, v[1].value], "kz": v[2].value, "order": v[3].value}
         if args.autotune and split and kern < 2:
             tuned[name]["tuned_ms"] = best[kern]
+        if kern == 2 and not split and path.value == 0:
+            tuned[name]["tuned_ms"] = step_best.value
     def step_timed(t, ev):
         arr = (ctypes.c_void_p * 4)(*[e.h.value for e in ev])
         PKG.abi.check(lib, lib.filt_time_step_timed(fref, pref, t, nx, 0, 1, 1, arr, None),

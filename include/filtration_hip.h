@@ -319,7 +319,7 @@ int filt_save_exchange_data(double* dev_array, double* dev_buffer,
  * 32x8, 16x16 or 64x8), kz planes per z-chunk, order 0 = XCD-grouped tile
  * numbering, 1 = hardware order.  Used by grids with every extent >= 3.
  * The whole-step kernel also takes 32x16 and 64x16 (1024 threads), and
- * so does the stages 2-4 kernel.  tx = 0 restores the built-in default
+ * so does the stages 2-4 kernel, which takes 128x4, 128x8 and 256x4 too.  tx = 0 restores the built-in default
  * (roS 64x8, kz 64, order 1; u 64x8, kz 64, order 1; stages 2-4 64x8, kz 16,
  * order 1; the whole step off); tx = -1 switches the kernel off: the
  * one-cell-per-thread kernels for 0-1, the stages one by one for 2, stage 1
@@ -341,6 +341,23 @@ int filt_get_tile_config(int kernel, int nx, int ny, int nz, int* tx, int* ty, i
  * and best_ms[0] is 0.  filt_set_tile_config clears tuned entries. */
 int filt_autotune_stencil(const filt_fields* dev, const filt_params* p, int localNx, int reps,
                           double* best_ms);
+
+/* Autotuner of the stages 2-4 kernel (kernel 2 of filt_set_tile_config):
+ * times its nine tile shapes (64x4, 32x8, 16x16, 64x8, 32x16, 64x16, 128x4,
+ * 128x8, 256x4) x kz in {8, 16, 32, 64, Nz} x both block orders on the given
+ * device grid (after stage 1 has filled P2, Xi, ro), reps launches each
+ * (median), and keeps the fastest for grids of this shape; filt_time_step
+ * and filt_u_roS_P1_S2_calculation then use it.  S2 is saved to a temporary
+ * device array and restored before each launch; on return the grid holds
+ * what one filt_u_roS_P1_S2_calculation call gives.  *best_ms (when
+ * non-null) receives the chosen configuration's time.  FILT_ERR_UNSUPPORTED
+ * where the kernel does not apply (several ranks, an extent below 3, arrays
+ * of 4 GiB or more, the divgrad scheme, spacings outside [2^-100, 2^101], or
+ * the fusion switched off).  filt_set_tile_config(2, ...) clears the tuned
+ * entries.  No counterpart in the reference, whose launch shape is a fixed
+ * 16x4x4 block (defines.h:18-20). */
+int filt_autotune_step(const filt_fields* dev, const filt_params* p, int localNx, int reps,
+                       double* best_ms);
 
 /* -------------------------------------- fused / batched / mixed forms -- */
 /*

@@ -160,6 +160,7 @@ def load_library(path=HIP_LIB):
     _sig(lib, "filt_set_tile_config", c_int, c_int, c_int, c_int, c_int, c_int)
     _sig(lib, "filt_get_tile_config", c_int, c_int, c_int, c_int, c_int, pi, pi, pi, pi)
     _sig(lib, "filt_autotune_stencil", c_int, F, P, c_int, c_int, POINTER(c_double))
+    _sig(lib, "filt_autotune_step", c_int, F, P, c_int, c_int, POINTER(c_double))
     E = c_void_p
     _sig(lib, "filt_ensemble_create", c_int, POINTER(c_void_p), c_int)
     _sig(lib, "filt_ensemble_destroy", c_int, E)

@@ -593,6 +593,8 @@ bool g_untiled[4] = {false, false, false, true};
 // Autotuned configurations by grid shape (filt_autotune_stencil).
 std::mutex g_tune_mu;
 std::map<std::tuple<int, int, int>, std::pair<TileConfig, TileConfig>> g_tuned;
+// ... and of the stages 2-4 kernel (filt_autotune_step).
+std::map<std::tuple<int, int, int>, TileConfig> g_step_tuned;
 
 // The fused step offers the tile shapes of the roS kernel.
 bool valid_cfg(const TileConfig& t, int kernel)
@@ -609,8 +611,10 @@ bool valid_cfg(const TileConfig& t, int kernel)
 TileConfig cfg_for(int kernel, int nx, int ny, int nz)
 {
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    if (kernel == 2)
-        return g_step_cfg;
+    if (kernel == 2) {
+        auto it = g_step_tuned.find(std::make_tuple(nx, ny, nz));
+        return it != g_step_tuned.end() ? it->second : g_step_cfg;
+    }
     if (kernel == 3)
         return g_full_cfg;
     auto it = g_tuned.find(std::make_tuple(nx, ny, nz));
@@ -708,6 +712,8 @@ extern "C" int filt_set_tile_config(int kernel, int tx, int ty, int kz, int orde
         filt::g_untiled[kernel] = false;
         if (kernel < 2)
             filt::g_tuned.clear();
+        if (kernel == 2)
+            filt::g_step_tuned.clear();
         return FILT_OK;
     });
 }
@@ -786,6 +792,97 @@ extern "C" int filt_autotune_stencil(const filt_fields* dev, const filt_params* 
             best_ms[0] = roS_ok ? best_t[0] : 0.0;
             best_ms[1] = best_t[1];
         }
+        return FILT_OK;
+    });
+}
+
+// Time every candidate tile configuration of the stages 2-4 kernel on the
+// given grid and keep the fastest for grids of this shape.  The kernel reads
+// S2 (the Newton's start) and rewrites it in place, so S2 is saved to a
+// temporary array and restored before every timed launch; P1 is only
+// written (the Newton starts from P2), and u, roS, roS_old follow from the
+// inputs.  At the end S2 is restored once more and the chosen configuration
+// runs once, so the grid holds what one filt_u_roS_P1_S2_calculation gives.
+extern "C" int filt_autotune_step(const filt_fields* dev, const filt_params* p, int localNx,
+                                  int reps, double* best_ms)
+{
+    return filt::guarded([&]() -> int {
+        using namespace filt;
+        int rc = check_shape(p, localNx);
+        if (rc)
+            return rc;
+        if (!dev || !dev->P1 || !dev->S2)
+            return set_error(FILT_ERR_INVALID, "null device fields");
+        const int nx = localNx, ny = p->Ny, nz = p->Nz;
+        if (!step_fusable(*p, nx, 0, 1))
+            return set_error(FILT_ERR_UNSUPPORTED, "grid does not use the stages 2-4 kernel");
+        if (reps < 1)
+            reps = 3;
+        const size_t bytes = sizeof(double) * (size_t)nx * ny * nz;
+        struct Buf {
+            double* p = nullptr;
+            ~Buf()
+            {
+                if (p) {
+                    (void)hipStreamSynchronize(stream());
+                    (void)hipFree(p);
+                }
+            }
+        } save;
+        FILT_HIP_TRY(hipMalloc((void**)&save.p, bytes));
+        FILT_HIP_TRY(hipMemcpyAsync(save.p, dev->S2, bytes, hipMemcpyDeviceToDevice, stream()));
+        struct Ev {
+            hipEvent_t e = nullptr;
+            ~Ev()
+            {
+                if (e) (void)hipEventDestroy(e);
+            }
+        } e0, e1;
+        FILT_HIP_TRY(hipEventCreate(&e0.e));
+        FILT_HIP_TRY(hipEventCreate(&e1.e));
+        static const int shapes[][2] = {{64, 4},  {32, 8},  {16, 16}, {64, 8}, {32, 16},
+                                        {64, 16}, {128, 4}, {128, 8}, {256, 4}};
+        std::vector<int> kzs = {8, 16, 32, 64};
+        if (nz > 64)
+            kzs.push_back(nz);
+        TileConfig best = cfg_for(2, nx, ny, nz);
+        double best_t = 1e30;
+        for (auto& s : shapes)
+            for (int kz : kzs)
+                for (int order = 0; order < 2; order++) {
+                    const TileConfig t = {s[0], s[1], kz, order};
+                    std::vector<float> ms;
+                    for (int r = 0; r <= reps; r++) {
+                        FILT_HIP_TRY(hipMemcpyAsync(dev->S2, save.p, bytes,
+                                                    hipMemcpyDeviceToDevice, stream()));
+                        FILT_HIP_TRY(hipEventRecord(e0.e, stream()));
+                        rc = launch_step_zm_cfg(*dev, *p, nx, true, false, t);
+                        if (rc)
+                            return rc;
+                        FILT_HIP_TRY(hipEventRecord(e1.e, stream()));
+                        FILT_HIP_TRY(hipEventSynchronize(e1.e));
+                        float v = 0;
+                        FILT_HIP_TRY(hipEventElapsedTime(&v, e0.e, e1.e));
+                        if (r > 0)  // the first launch warms caches and code
+                            ms.push_back(v);
+                    }
+                    std::sort(ms.begin(), ms.end());
+                    const double med = ms[ms.size() / 2];
+                    if (med < best_t) {
+                        best_t = med;
+                        best = t;
+                    }
+                }
+        FILT_HIP_TRY(hipMemcpyAsync(dev->S2, save.p, bytes, hipMemcpyDeviceToDevice, stream()));
+        rc = launch_step_zm_cfg(*dev, *p, nx, true, false, best);
+        if (rc)
+            return rc;
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            g_step_tuned[std::make_tuple(nx, ny, nz)] = best;
+        }
+        if (best_ms)
+            *best_ms = best_t;
         return FILT_OK;
     });
 }

@@ -739,6 +739,9 @@ StepFn pick_step(int tx, int ty)
     if (tx == 64 && ty == 8) return k_step_zm<64, 8, WITH_U, BC>;
     if (tx == 32 && ty == 16) return k_step_zm<32, 16, WITH_U, BC>;
     if (tx == 64 && ty == 16) return k_step_zm<64, 16, WITH_U, BC>;
+    if (tx == 128 && ty == 4) return k_step_zm<128, 4, WITH_U, BC>;
+    if (tx == 128 && ty == 8) return k_step_zm<128, 8, WITH_U, BC>;
+    if (tx == 256 && ty == 4) return k_step_zm<256, 4, WITH_U, BC>;
     return nullptr;
 }
 
@@ -838,7 +841,12 @@ static std::atomic<long long> g_step_launches{0};
 
 int launch_step_zm(const filt_fields& f, const filt_params& p, int nx, bool with_u, bool bc)
 {
-    const TileConfig t = step_tile_config(nx, p.Ny, p.Nz);
+    return launch_step_zm_cfg(f, p, nx, with_u, bc, step_tile_config(nx, p.Ny, p.Nz));
+}
+
+int launch_step_zm_cfg(const filt_fields& f, const filt_params& p, int nx, bool with_u, bool bc,
+                       const TileConfig& t)
+{
     StepArgs a;
     a.f = f;
     a.p = p;

@@ -14,6 +14,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "../../include/filtration_hip.h"
+
 namespace filt {
 
 struct TileMap {
@@ -44,8 +46,13 @@ struct TileConfig {
     int order;   // TileMap order
 };
 
-// The fused step's configuration for a grid (filt_stencil.hip).
+// The fused step's configuration for a grid (filt_stencil.hip): autotuned
+// for this grid shape (filt_autotune_step), else the set or built-in one.
 TileConfig step_tile_config(int nx, int ny, int nz);
+// The stages 2-4 kernel with an explicit tile configuration (filt_step.hip),
+// one that step_tile_ok accepts.
+int launch_step_zm_cfg(const filt_fields& f, const filt_params& p, int nx, bool with_u, bool bc,
+                       const TileConfig& t);
 // The whole-step kernel's configuration (stage 1 with stages 2-4).
 TileConfig full_tile_config(int nx, int ny, int nz);
 

@@ -694,17 +694,75 @@ def test_fused_step_per_cell_fields_and_trajectory(hip, orc):
         assert max_rel(out[n], a[n]) <= 1e-10, n
 
 
-@pytest.mark.parametrize("tile", TILES)
+STEP_TILES = TILES + [(32, 16), (64, 16), (128, 4), (128, 8), (256, 4)]
+
+
+@pytest.mark.parametrize("tile", STEP_TILES)
 @pytest.mark.parametrize("kz,order", [(1, 0), (5, 1), (16, 0), (64, 1)])
 def test_fused_step_every_tile_config(hip, tile, kz, order):
+    """Every tile shape, z-chunk and block order of the stages 2-4 kernel
+    (141 columns: two or more tiles in x, the last one partial)."""
     lib = hip.lib
-    p = ref_params(41, 22, 19, source=2)
+    p = ref_params(141, 22, 19, source=2)
     g = random_state(p, np.random.default_rng(kz + tile[1]))
     try:
         assert lib.filt_set_tile_config(2, tile[0], tile[1], kz, order) == 0
         fused_vs_staged(hip, p, g, 2, 1)
     finally:
         lib.filt_set_tile_config(2, 0, 0, 0, 0)
+
+
+def test_step_autotuner_keeps_one_call_state(hip):
+    """filt_autotune_step times every configuration of the stages 2-4
+    kernel, keeps the fastest for the grid shape and leaves the grid as one
+    filt_u_roS_P1_S2_calculation call would (S2, which the kernel rewrites,
+    is restored before every timed launch); later time steps take the tuned
+    configuration and keep the bits of the stage calls."""
+    import ctypes
+    lib = hip.lib
+    p = ref_params(70, 41, 37, source=2)
+    g = random_state(p, np.random.default_rng(31))
+    a, b = hip.grid(p), hip.grid(p)
+    try:
+        a.upload(g)
+        b.upload(g)
+        a.stage(1)
+        b.stage(1)
+        best = ctypes.c_double()
+        assert lib.filt_autotune_step(ctypes.byref(a.f), ctypes.byref(a.params), a.nx, 2,
+                                      ctypes.byref(best)) == 0
+        assert best.value > 0
+        v = [ctypes.c_int() for _ in range(4)]
+        assert lib.filt_get_tile_config(2, 70, 41, 37, *[ctypes.byref(x) for x in v]) == 0
+        assert (v[0].value, v[1].value) in STEP_TILES and v[3].value in (0, 1)
+        assert lib.filt_u_roS_P1_S2_calculation(ctypes.byref(b.f), ctypes.byref(b.params), 0.0,
+                                                b.nx, 0, 1, 1) == 0
+        ga, gb = g.copy(), g.copy()
+        a.download(ga)
+        b.download(gb)
+        for n in FIELD_NAMES_ALL:
+            assert np.array_equal(ga[n], gb[n], equal_nan=True), n
+        n0 = lib.filt_step_zm_launches()
+        for s in (1, 2):
+            a.time_step(s * p.dt, 1)
+            staged_step(b, s * p.dt, 1)
+        assert lib.filt_step_zm_launches() == n0 + 2
+        a.download(ga)
+        b.download(gb)
+        for n in FIELD_NAMES_ALL:
+            assert np.array_equal(ga[n], gb[n], equal_nan=True), n
+    finally:
+        a.free()
+        b.free()
+        lib.filt_set_tile_config(2, 0, 0, 0, 0)
+    # a grid the kernel does not apply to (an extent below 3)
+    q = ref_params(20, 10, 2)
+    c = hip.grid(q)
+    try:
+        assert lib.filt_autotune_step(ctypes.byref(c.f), ctypes.byref(c.params), c.nx, 2,
+                                      None) == 6  # FILT_ERR_UNSUPPORTED
+    finally:
+        c.free()
 
 
 @pytest.mark.parametrize("media,poro", [(False, False), (True, False), (True, True)])
