@@ -186,11 +186,16 @@ def run_plain(args, p, perm, poro, split=False, steps=None):
                                                     ctypes.byref(path)), "path probe")
         names = STEP_STAGES if path.value == 1 else TWO_KERNEL_STAGES
         calls = None
-        if path.value == 0:
+        step_best = ctypes.c_double()
+        if path.value == 0 and args.step_tile:
+            t, kz, order = args.step_tile.split(",")
+            tx, ty = (int(v) for v in t.split("x"))
+            PKG.abi.check(lib, lib.filt_set_tile_config(2, tx, ty, int(kz), int(order)),
+                          "step tile")
+        elif path.value == 0:
             # the library's autotuner picks the stages 2-4 kernel's tile for
             # this grid (it times the kernel's variant without the folded
             # boundary, so the time step's own kernel launches only here)
-            step_best = ctypes.c_double()
             PKG.abi.check(lib, lib.filt_autotune_step(fref, pref, nx, 3, ctypes.byref(step_best)),
                           "step autotune")
     best = (ctypes.c_double * 2)()
@@ -200,14 +205,14 @@ def run_plain(args, p, perm, poro, split=False, steps=None):
         PKG.abi.check(lib, stage1(0.0), "stage 1")
         PKG.abi.check(lib, lib.filt_autotune_stencil(fref, pref, nx, 5, best), "autotune")
     tuned = {"autotuned": bool(args.autotune and split),
-             "step_autotuned": bool(not split and path.value == 0)}
+             "step_autotuned": bool(not split and path.value == 0 and not args.step_tile)}
     for kern, name in ((0, "roS"), (1, "u"), (2, "u_roS_newton"), (3, "step_full")):
         v = [ctypes.c_int() for _ in range(4)]
         lib.filt_get_tile_config(kern, nx, p.Ny, p.Nz, *[ctypes.byref(x) for x in v])
         tuned[name] = {"tile": [v[0].value, v[1].value], "kz": v[2].value, "order": v[3].value}
         if args.autotune and split and kern < 2:
             tuned[name]["tuned_ms"] = best[kern]
-        if kern == 2 and not split and path.value == 0:
+        if kern == 2 and not split and path.value == 0 and not args.step_tile:
             tuned[name]["tuned_ms"] = step_best.value
     def step_timed(t, ev):
         arr = (ctypes.c_void_p * 4)(*[e.h.value for e in ev])
@@ -951,6 +956,10 @@ def main():
     ap.add_argument("--tiling", default=None,
                     help="with --form: fixed tiling TXxTY[xCY],kz,order (e.g. 64x8x2,64,1) "
                          "instead of the autotuner")
+    ap.add_argument("--step-tile", default=None,
+                    help="fixed tile of the plain path's stages 2-4 kernel, TXxTY,kz,order "
+                         "(e.g. 128x8,64,0), instead of its autotuner: for profiler runs "
+                         "that must launch the same kernel")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -1046,9 +1055,6 @@ def main():
     mix_ceiling = stream_mix_ceiling(KERNEL_NAMES[dom], cells)
     if mix_ceiling:
         mix_ceiling["kernel_frac_of_mix"] = mix_ceiling["ms"] / dom_ms
-    dom_tiles = pr["tiles"].get(dom)
-    tiled_ceiling = (tiled_mix_ceiling(KERNEL_NAMES[dom], (p.NX, p.Ny, p.Nz), dom_tiles["tile"],
-                                       dom_tiles["kz"], dom_ms) if dom_tiles else None)
     step_bytes = sum(PLAIN_BYTES[s] for s in step_names) * cells
     line = {
         "metric": "cell-updates/s",
@@ -1087,7 +1093,6 @@ def main():
             "algorithmic_bytes_per_launch": dom_bytes,
             "kernel_ms": dom_ms,
             "stream_mix_ceiling": mix_ceiling,
-            "tiled_mix_ceiling": tiled_ceiling,
         },
         "stages_ms": stage_ms,
         "stages_ms_from": f"{pr['breakdown_steps']} steps after the timed region, with a HIP "
