@@ -471,7 +471,7 @@ def run_slab(args, nx, ny, nz, with_oracle):
     p, perm, poro = W.slab_workload(nx, ny, nz)
     gen_s = time.perf_counter() - t0
     steps = args.slab_steps
-    sub = types.SimpleNamespace(steps=steps, warmup=3, autotune=False)
+    sub = types.SimpleNamespace(steps=steps, warmup=3, autotune=False, step_tile=None)
     pr = run_plain(sub, p, perm, poro)
     cells = nx * ny * nz
     plain = cells * steps / pr["wall"]
