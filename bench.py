@@ -199,7 +199,13 @@ def run_plain(args, p, perm, poro, split=False, steps=None):
             PKG.abi.check(lib, lib.filt_autotune_step(fref, pref, nx, 3, ctypes.byref(step_best)),
                           "step autotune")
     best = (ctypes.c_double * 2)()
-    if args.autotune and split:
+    if split and getattr(args, "stencil_tile", None):
+        t, kz, order = args.stencil_tile.split(",")
+        tx, ty = (int(v) for v in t.split("x"))
+        for kern in (0, 1):
+            PKG.abi.check(lib, lib.filt_set_tile_config(kern, tx, ty, int(kz), int(order)),
+                          "stencil tile")
+    elif args.autotune and split:
         # stage 1 fills the inputs the stencil kernels read, then the
         # library's autotuner picks tile shapes for this grid
         PKG.abi.check(lib, stage1(0.0), "stage 1")
@@ -631,14 +637,19 @@ def run_divgrad(args, p, perm, poro):
     """The reference's tau-relaxed divgrad scheme (assign_rS, cpu.cpp:162-216)
     in place of the upwind roS stage, on the headline grid: the five stage
     calls per step (the stage path; the one-kernel fusion is upwind only),
-    timed like the stage-call leg, with the divgrad kernel's roofline; then
-    3 steps from the initial state against the oracle port, cell for cell."""
+    timed like the stage-call leg, with the divgrad kernel's roofline, the u
+    and divgrad kernels' tiles chosen by the library's stencil autotuner;
+    then 3 steps from the initial state against the oracle port, cell for
+    cell.  The tuned tiles are dropped afterwards, so later legs run the
+    built-in ones."""
     import copy
     import numpy as np
     q = copy.copy(p)
     q.roS_scheme = PKG.abi.FILT_SCHEME_DIVGRAD
     K = max(1, args.steps // 4)
-    r = run_plain(args, q, perm, poro, split=True, steps=K)
+    sub = copy.copy(args)
+    sub.autotune = not args.stencil_tile
+    r = run_plain(sub, q, perm, poro, split=True, steps=K)
     cells = q.NX * q.Ny * q.Nz
     ms = r["stage_ms"]["roS"]
     out = {"value": cells * K / r["wall"], "unit": "cell-updates/s",
@@ -653,10 +664,7 @@ def run_divgrad(args, p, perm, poro):
     flat = stream_mix_ceiling("k_roS_divgrad_zm", cells)
     if flat:
         flat["kernel_frac_of_mix"] = flat["ms"] / ms
-    tl = r["tiles"]["roS"]
     out["k_roS_divgrad_zm"]["stream_mix_ceiling"] = flat
-    out["k_roS_divgrad_zm"]["tiled_mix_ceiling"] = tiled_mix_ceiling(
-        "k_roS_divgrad_zm", (q.NX, q.Ny, q.Nz), tl["tile"], tl["kz"], ms)
     if not args.no_cpu:
         n = 3
         be = PKG.device.Backend()
@@ -675,6 +683,9 @@ def run_divgrad(args, p, perm, poro):
             "S2_bit_equal": bool(np.array_equal(S2.view(np.uint64), ref["S2"].view(np.uint64))),
             "P1_rel_linf": float(np.max(np.abs(P1 - ref["P1"])) / np.max(np.abs(ref["P1"]))),
             "S2_linf": float(np.max(np.abs(S2 - ref["S2"])))}
+    lib = PKG.abi.load_library()
+    for kern in (0, 1):  # back to the built-in tiles for the legs after this one
+        PKG.abi.check(lib, lib.filt_set_tile_config(kern, 0, 0, 0, 0), "tile reset")
     return out
 
 
@@ -956,6 +967,9 @@ def main():
     ap.add_argument("--tiling", default=None,
                     help="with --form: fixed tiling TXxTY[xCY],kz,order (e.g. 64x8x2,64,1) "
                          "instead of the autotuner")
+    ap.add_argument("--stencil-tile", default=None,
+                    help="fixed tile of the u and roS stencil kernels for the divgrad leg, "
+                         "TXxTY,kz,order, instead of their autotuner: for profiler runs")
     ap.add_argument("--step-tile", default=None,
                     help="fixed tile of the plain path's stages 2-4 kernel, TXxTY,kz,order "
                          "(e.g. 128x8,64,0), instead of its autotuner: for profiler runs "

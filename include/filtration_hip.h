@@ -314,12 +314,13 @@ int filt_save_exchange_data(double* dev_array, double* dev_buffer,
                             int size, const filt_params* p);
 
 /* Tile shape of the z-marching stencil kernels of u_calculation (kernel 1),
- * roS_calculation (kernel 0), the fused stages 2-4 (kernel 2) and the whole
- * step, stage 1 with stages 2-4 (kernel 3): tx*ty threads per block (64x4,
- * 32x8, 16x16 or 64x8), kz planes per z-chunk, order 0 = XCD-grouped tile
- * numbering, 1 = hardware order.  Used by grids with every extent >= 3.
- * The whole-step kernel also takes 32x16 and 64x16 (1024 threads), and
- * so does the stages 2-4 kernel, which takes 128x4, 128x8 and 256x4 too.  tx = 0 restores the built-in default
+ * roS_calculation (kernel 0; either scheme), the fused stages 2-4 (kernel 2)
+ * and the whole step, stage 1 with stages 2-4 (kernel 3): tx*ty threads per
+ * block (64x4, 32x8, 16x16 or 64x8; for kernels 0-2 also 128x4 and 128x8),
+ * kz planes per z-chunk, order 0 = XCD-grouped tile numbering, 1 = hardware
+ * order.  Used by grids with every extent >= 3.  The whole-step kernel also
+ * takes 32x16 and 64x16 (1024 threads), and so does the stages 2-4 kernel,
+ * which takes 256x4 too.  tx = 0 restores the built-in default
  * (roS 64x8, kz 64, order 1; u 64x8, kz 64, order 1; stages 2-4 64x8, kz 16,
  * order 1; the whole step off); tx = -1 switches the kernel off: the
  * one-cell-per-thread kernels for 0-1, the stages one by one for 2, stage 1
@@ -331,14 +332,20 @@ int filt_set_tile_config(int kernel, int tx, int ty, int kz, int order);
 int filt_get_tile_config(int kernel, int nx, int ny, int nz, int* tx, int* ty, int* kz,
                          int* order);
 
-/* Autotuner: times every candidate tile configuration of both stencil
- * kernels on the given device grid (after stage 1 has filled P2, Xi, ro),
- * reps launches each (median), and keeps the fastest per kernel for grids of
- * this shape.  Rewrites only roS*, u* with the values one call would give.
- * best_ms[0] / best_ms[1] receive the roS / u times when non-null.  The
- * tiled roS kernel needs h1, h2, h3 in [2^-100, 2^100] (it divides through a
- * refined reciprocal); outside that range roS runs untiled, is not tuned,
- * and best_ms[0] is 0.  filt_set_tile_config clears tuned entries. */
+/* Autotuner: times every candidate tile configuration (64x4, 32x8, 16x16,
+ * 64x8, 128x4, 128x8 x kz in {8, 16, 32, 64, Nz} x both orders) of both
+ * stencil kernels on the given device grid (after stage 1 has filled P2, Xi,
+ * ro), reps launches each (median), and keeps the fastest per kernel for
+ * grids of this shape.  Kernel 0 is the roS stage of the scheme p selects.
+ * Upwind: rewrites only roS*, u* with the values one call would give.
+ * Divgrad (its kernel reads roS*_old and rewrites them; timed on its t >= 2 dt
+ * branch): the four roS arrays are saved to a temporary device array and
+ * left as they were; u* holds the values one call gives.  best_ms[0] /
+ * best_ms[1] receive the roS / u times when non-null.  The tiled roS kernels
+ * need h1, h2, h3 in [2^-100, 2^100] (they divide through a refined
+ * reciprocal; the divgrad kernel also h^2, 2h, dt^2 and 2 dt^2); outside
+ * that range roS runs untiled, is not tuned, and best_ms[0] is 0.
+ * filt_set_tile_config clears tuned entries. */
 int filt_autotune_stencil(const filt_fields* dev, const filt_params* p, int localNx, int reps,
                           double* best_ms);
 

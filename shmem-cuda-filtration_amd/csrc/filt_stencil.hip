@@ -538,6 +538,8 @@ RoSFn pick_roS(int tx, int ty)
     if (tx == 32 && ty == 8) return k_roS_zm<32, 8>;
     if (tx == 16 && ty == 16) return k_roS_zm<16, 16>;
     if (tx == 64 && ty == 8) return k_roS_zm<64, 8>;
+    if (tx == 128 && ty == 4) return k_roS_zm<128, 4>;
+    if (tx == 128 && ty == 8) return k_roS_zm<128, 8>;
     return nullptr;
 }
 
@@ -555,6 +557,8 @@ DivgradFn pick_divgrad(int tx, int ty, bool poro)
     if (tx == 32 && ty == 8) return divgrad_of<32, 8>(poro);
     if (tx == 16 && ty == 16) return divgrad_of<16, 16>(poro);
     if (tx == 64 && ty == 8) return divgrad_of<64, 8>(poro);
+    if (tx == 128 && ty == 4) return divgrad_of<128, 4>(poro);
+    if (tx == 128 && ty == 8) return divgrad_of<128, 8>(poro);
     return nullptr;
 }
 
@@ -564,6 +568,8 @@ UFn pick_u(int tx, int ty)
     if (tx == 32 && ty == 8) return k_u_zm<32, 8>;
     if (tx == 16 && ty == 16) return k_u_zm<16, 16>;
     if (tx == 64 && ty == 8) return k_u_zm<64, 8>;
+    if (tx == 128 && ty == 4) return k_u_zm<128, 4>;
+    if (tx == 128 && ty == 8) return k_u_zm<128, 8>;
     return nullptr;
 }
 
@@ -739,7 +745,11 @@ extern "C" int filt_autotune_stencil(const filt_fields* dev, const filt_params* 
             return set_error(FILT_ERR_UNSUPPORTED, "grid does not use the tiled kernels");
         if (reps < 1)
             reps = 3;
-        static const int shapes[4][2] = {{64, 4}, {32, 8}, {16, 16}, {64, 8}};
+        const bool divgrad = p->roS_scheme == FILT_SCHEME_DIVGRAD;
+        if (divgrad && (!dev->u1x || !dev->u1y || !dev->u2x || !dev->u2y || !dev->roS1 ||
+                        !dev->roS2 || !dev->roS1_old || !dev->roS2_old))
+            return set_error(FILT_ERR_INVALID, "the divgrad scheme reads u and roS_old");
+        static const int shapes[6][2] = {{64, 4}, {32, 8}, {16, 16}, {64, 8}, {128, 4}, {128, 8}};
         std::vector<int> kzs = {8, 16, 32, 64};
         if (nz > 64)
             kzs.push_back(nz);
@@ -748,9 +758,33 @@ extern "C" int filt_autotune_stencil(const filt_fields* dev, const filt_params* 
         FILT_HIP_TRY(hipEventCreate(&e1));
         TileConfig best[2] = {cfg_for(0, nx, ny, nz), cfg_for(1, nx, ny, nz)};
         double best_t[2] = {1e30, 1e30};
-        // the tiled roS kernel divides through DivBy: outside its divisor
+        // the tiled roS kernels divide through DivBy: outside its divisor
         // range the stage runs untiled and only u is tuned
-        const bool roS_ok = divby_ok(p->h1) && divby_ok(p->h2) && divby_ok(p->h3);
+        const bool roS_ok = divby_ok(p->h1) && divby_ok(p->h2) && divby_ok(p->h3) &&
+                            (!divgrad || divgrad_tiled_ok(*p));
+        // The divgrad kernel reads roS1_old, roS2_old and rewrites them: all
+        // four roS arrays are saved, the two inputs restored before each of
+        // its launches, and all four restored at the end.
+        const size_t bytes = sizeof(double) * (size_t)nx * ny * nz;
+        double* const keep[4] = {dev->roS1_old, dev->roS2_old, dev->roS1, dev->roS2};
+        struct Buf {
+            double* p = nullptr;
+            ~Buf()
+            {
+                if (p) {
+                    (void)hipStreamSynchronize(stream());
+                    (void)hipFree(p);
+                }
+            }
+        } save;
+        if (divgrad && roS_ok) {
+            FILT_HIP_TRY(hipMalloc((void**)&save.p, 4 * bytes));
+            for (int a = 0; a < 4; a++)
+                FILT_HIP_TRY(hipMemcpyAsync((char*)save.p + a * bytes, keep[a], bytes,
+                                            hipMemcpyDeviceToDevice, stream()));
+        }
+        // the divgrad kernel is timed on its relaxed branch (t >= 2 dt)
+        const double t_div = 2. * p->dt;
         for (int kern = roS_ok ? 0 : 1; kern < 2; kern++)
             for (auto& s : shapes)
                 for (int kz : kzs)
@@ -759,8 +793,17 @@ extern "C" int filt_autotune_stencil(const filt_fields* dev, const filt_params* 
                         const TileMap m = make_map(nx, ny, nz, t);
                         std::vector<float> ms;
                         for (int r = 0; r <= reps; r++) {
+                            if (kern == 0 && divgrad)
+                                for (int a = 0; a < 2; a++)
+                                    FILT_HIP_TRY(hipMemcpyAsync(keep[a], (char*)save.p + a * bytes,
+                                                                bytes, hipMemcpyDeviceToDevice,
+                                                                stream()));
                             FILT_HIP_TRY(hipEventRecord(e0, stream()));
-                            if (kern == 0)
+                            if (kern == 0 && divgrad)
+                                hipLaunchKernelGGL(pick_divgrad(t.tx, t.ty, dev->poro != nullptr),
+                                                   dim3((unsigned)m.count()), dim3(t.tx, t.ty), 0,
+                                                   stream(), *dev, *p, nx, m, t.kz, t_div);
+                            else if (kern == 0)
                                 hipLaunchKernelGGL(pick_roS(t.tx, t.ty), dim3((unsigned)m.count()),
                                                    dim3(t.tx, t.ty), 0, stream(), *dev, *p, nx, m, t.kz);
                             else
@@ -784,6 +827,10 @@ extern "C" int filt_autotune_stencil(const filt_fields* dev, const filt_params* 
                     }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
+        if (save.p)
+            for (int a = 0; a < 4; a++)
+                FILT_HIP_TRY(hipMemcpyAsync(keep[a], (char*)save.p + a * bytes, bytes,
+                                            hipMemcpyDeviceToDevice, stream()));
         {
             std::lock_guard<std::mutex> lk(g_tune_mu);
             g_tuned[std::make_tuple(nx, ny, nz)] = std::make_pair(best[0], best[1]);

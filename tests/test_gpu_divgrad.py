@@ -36,7 +36,7 @@ def test_divgrad_stage_bitwise(hip, orc, dims, tstep):
         assert np.array_equal(out[n], g[n]), (n, dims, tstep)
 
 
-TILES = [(64, 4), (32, 8), (16, 16), (64, 8)]
+TILES = [(64, 4), (32, 8), (16, 16), (64, 8), (128, 4), (128, 8)]
 
 
 @pytest.mark.parametrize("tile", TILES)
@@ -47,7 +47,7 @@ def test_divgrad_tiled_every_config_bitwise(hip, orc, tile, kz, order):
     time test (plain Euler while t < 2 dt): the oracle's bits (partial tiles
     in x and y, chunk ends not aligned with the grid)."""
     lib = hip.lib
-    p = divgrad_params(67, 19, 23)
+    p = divgrad_params(137, 19, 23)
     try:
         assert lib.filt_set_tile_config(0, tile[0], tile[1], kz, order) == 0
         for poro in (False, True):
@@ -98,6 +98,41 @@ def test_divgrad_untiled_switch_and_divisor_range(hip, orc):
         outs.append(out)
     for n in ("roS1", "roS2", "roS1_old", "roS2_old"):
         assert np.array_equal(outs[0][n], outs[1][n]), n
+
+
+def test_stencil_autotuner_with_divgrad_keeps_roS_arrays(hip, orc):
+    """filt_autotune_stencil on a grid with the divgrad scheme times the
+    divgrad kernel as the roS stage; it leaves the four roS arrays as they
+    were (the kernel rewrites its own roS_old input) and u as one call gives;
+    a stage call afterwards, with the tuned tile, gives the oracle's bits."""
+    lib = hip.lib
+    p = divgrad_params(70, 33, 29)
+    g = random_state(p, np.random.default_rng(12), poro=True)
+    d = hip.grid(p, poro=True)
+    try:
+        d.upload(g)
+        best = (ctypes.c_double * 2)()
+        assert lib.filt_autotune_stencil(ctypes.byref(d.f), ctypes.byref(d.params), d.nx, 2,
+                                         best) == 0
+        assert best[0] > 0 and best[1] > 0
+        v = [ctypes.c_int() for _ in range(4)]
+        assert lib.filt_get_tile_config(0, 70, 33, 29, *[ctypes.byref(x) for x in v]) == 0
+        assert (v[0].value, v[1].value) in TILES
+        out = g.copy()
+        d.download(out)
+        want = g.copy()
+        orc.stage(2, want, p)
+        for n in FIELD_NAMES_ALL:
+            assert np.array_equal(out[n], want[n]), n
+        d.stage(3, t=4 * p.dt)
+        d.download(out)
+    finally:
+        d.free()
+        lib.filt_set_tile_config(0, 0, 0, 0, 0)
+        lib.filt_set_tile_config(1, 0, 0, 0, 0)
+    orc.stage(3, want, p, t=4 * p.dt)
+    for n in FIELD_NAMES_ALL:
+        assert np.array_equal(out[n], want[n]), n
 
 
 @pytest.mark.parametrize("dims,steps", [((20, 16, 12), 60), ((10, 9, 1), 30)])
