@@ -139,6 +139,26 @@ struct DivBy {
 
 __device__ __attribute__((noinline)) inline double div_rare(double n, double d) { return n / d; }
 
+// 1 / d bit for bit as the IEEE division, for a divisor that varies per
+// lane.  With numerator 1 the expansion above is, for 2^-100 <= |d| < 2^101,
+// y = refined_reciprocal(d) and then the remainder step with q = 1*y = y:
+// two FP64 operations after the reciprocal instead of the two scale steps,
+// the multiply, the remainder, the scaled fma and the fix-up, and three
+// fewer steps on the dependent chain.  The expansion is symmetric in the
+// sign of d, so |d| is what counts.  Other divisors (zeros, subnormals,
+// infinities, NaN, magnitudes outside the range) take the division out of
+// line.  tests/hip/divby_check compares it with 1 / d on 2^24 divisors.
+__device__ __forceinline__ double recip_ieee(double d)
+{
+    const double y = refined_reciprocal(d);
+    const double t = fma(d, y, -1.0);  // d*y - 1, rounded once
+    double f = fma(-t, y, y);
+    const unsigned e = ((unsigned)__double2hiint(d) >> 20) & 0x7ffu;
+    if (!(e - 923u <= 200u))  // |d| outside [2^-100, 2^101)
+        f = div_rare(1.0, d);
+    return f;
+}
+
 // The divisors DivBy accepts.
 __host__ __device__ inline bool divby_ok(double d)
 {
@@ -202,8 +222,9 @@ __device__ inline void newton_step(double P, double S, double P_k, double PkS, d
     const double F1S = (-1) * ro01 * (1 + b1 * (P - Pa));
     const double F2S = ro02 * (1 + b2 * (P + P_k - Pa + (S * PkS)));
     const double det = F1P * F2S - F1S * F2P;
-    Pn = P - (1 / det) * (F2S * F1 - F1S * F2);
-    Sn = S - (1 / det) * (F1P * F2 - F2P * F1);
+    const double inv = recip_ieee(det);  // 1 / det, bit for bit
+    Pn = P - inv * (F2S * F1 - F1S * F2);
+    Sn = S - inv * (F1P * F2 - F2P * F1);
 }
 
 // newton_step with P_k = PkS = 0, bit for bit, for P_atm != 0 and S > -inf.
@@ -224,8 +245,9 @@ __device__ inline void newton_step_nocap(double P, double S, double roS1, double
     const double F2P = ro02 * b2 * S;
     const double F1S = (-1) * ro01 * (1 + b1 * (P - Pa));
     const double det = F1P * F2S - F1S * F2P;
-    Pn = P - (1 / det) * (F2S * F1 - F1S * F2);
-    Sn = S - (1 / det) * (F1P * F2 - F2P * F1);
+    const double inv = recip_ieee(det);  // 1 / det, bit for bit
+    Pn = P - inv * (F2S * F1 - F1S * F2);
+    Sn = S - inv * (F1P * F2 - F2P * F1);
 }
 
 template <bool kLean, class Params>

@@ -2,8 +2,11 @@
 // n / d bit for bit on the GPU: 2^24 numerators (random bit patterns, the
 // range edges 2^-600 and 2^601, exact quotients, significands of all ones,
 // zeros, subnormals, infinities, NaN) against 64 divisors in [2^-100, 2^100]
-// (grid spacings, their doubles, and random ones).  Prints one JSON line and
-// exits nonzero on any difference.  Run by tests/test_gpu_plain.py.
+// (grid spacings, their doubles, and random ones); and recip_ieee against
+// 1 / d on 2^24 divisors (random bit patterns, random magnitudes in and
+// around [2^-100, 2^101) of either sign, the range edges, significands of all
+// ones, zeros, subnormals, infinities, NaN).  Prints one JSON line and exits
+// nonzero on any difference.  Run by tests/test_gpu_plain.py.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -35,6 +38,23 @@ __global__ void k_check(const double* num, long long count, const double* div, i
                 example[3 * c + 1] = d;
                 example[3 * c + 2] = got;
             }
+        }
+    }
+}
+
+__global__ void k_recip(const double* den, long long count, unsigned long long* mismatches,
+                        double* example)
+{
+    const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (i >= count)
+        return;
+    const double d = den[i];
+    const double got = filt::recip_ieee(d), want = 1.0 / d;
+    if (__double_as_longlong(got) != __double_as_longlong(want)) {
+        const unsigned long long c = atomicAdd(mismatches, 1ull);
+        if (c < 4) {
+            example[2 * c] = d;
+            example[2 * c + 1] = got;
         }
     }
 }
@@ -115,15 +135,59 @@ int main()
     double ex[12] = {};
     CHECK(hipMemcpy(&mis, d_mis, sizeof mis, hipMemcpyDeviceToHost));
     CHECK(hipMemcpy(ex, d_ex, sizeof ex, hipMemcpyDeviceToHost));
+
+    // reciprocals: the divisors go into the numerator buffer
+    std::mt19937_64 g3(4242);
+    for (long long i = 0; i < N; i++) {
+        const uint64_t u = g3();
+        const double sgn = (u >> 63) ? -1.0 : 1.0;
+        double v;
+        switch (i % 8) {
+        case 0: v = from_bits(u); break;
+        case 1:
+        case 2:
+        case 3:  // in and around the range, log-uniform
+            v = sgn * std::ldexp(1.0 + (double)(u >> 12) * 0x1p-52, (int)(u % 211) - 105);
+            break;
+        case 4:  // the range edges and their neighbours
+            v = sgn * std::nextafter(std::ldexp(1.0, (u & 1) ? 101 : -100),
+                                     (u & 2) ? INFINITY : 0.0) *
+                ((u & 4) ? 1.0 : std::ldexp(1.0, (int)((u >> 8) % 3) - 1));
+            break;
+        case 5: {  // significands of all ones, near the range
+            uint64_t b = 0x000fffffffffffffull - (u % 16);
+            b |= (uint64_t)(923 + (int)((u >> 10) % 203) - 1) << 52;
+            b ^= ((u >> 62) & 1) << 63;
+            v = from_bits(b);
+            break;
+        }
+        case 6: v = specials[u % 11] * ((u >> 40) & 1 ? -1.0 : 1.0); break;
+        default: v = sgn * ((double)(u >> 11) * 0x1p-53 + 1e-3) * 1e6; break;  // Newton-like
+        }
+        num[i] = v;
+    }
+    CHECK(hipMemcpy(d_num, num.data(), N * sizeof(double), hipMemcpyHostToDevice));
+    CHECK(hipMemset(d_mis, 0, sizeof(unsigned long long)));
+    hipLaunchKernelGGL(k_recip, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, 0, d_num, N,
+                       d_mis, d_ex);
+    CHECK(hipGetLastError());
+    unsigned long long rmis = 0;
+    double rex[8] = {};
+    CHECK(hipMemcpy(&rmis, d_mis, sizeof rmis, hipMemcpyDeviceToHost));
+    CHECK(hipMemcpy(rex, d_ex, sizeof rex, hipMemcpyDeviceToHost));
+
     std::printf("{\"numerators\": %lld, \"divisors\": %zu, \"mismatches\": %llu", N, div.size(),
                 mis);
     for (unsigned k = 0; k < (mis < 4 ? mis : 4); k++)
         std::printf(", \"example%u\": [\"%a\", \"%a\", \"%a\"]", k, ex[3 * k], ex[3 * k + 1],
                     ex[3 * k + 2]);
+    std::printf(", \"reciprocals\": %lld, \"recip_mismatches\": %llu", N, rmis);
+    for (unsigned k = 0; k < (rmis < 4 ? rmis : 4); k++)
+        std::printf(", \"recip_example%u\": [\"%a\", \"%a\"]", k, rex[2 * k], rex[2 * k + 1]);
     std::printf("}\n");
     (void)hipFree(d_num);
     (void)hipFree(d_div);
     (void)hipFree(d_ex);
     (void)hipFree(d_mis);
-    return mis != 0;
+    return mis != 0 || rmis != 0;
 }

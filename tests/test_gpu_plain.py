@@ -645,6 +645,7 @@ def test_divby_equals_ieee_division():
     line = r.stdout.strip().splitlines()[-1]
     res = json.loads(line)
     assert r.returncode == 0 and res["mismatches"] == 0, line
+    assert res["reciprocals"] == 1 << 24 and res["recip_mismatches"] == 0, line
 
 
 def test_stream_mix_ceilings_flat_and_tiled():
