@@ -80,6 +80,58 @@ __global__ __launch_bounds__(TX* TY) void k_ro_P2_Xi(filt_fields f, filt_params 
     }
 }
 
+// k_ro_P2_Xi for one medium (no media lookups) with 1 - S_wr, mu1 and mu2 in
+// DivBy's range: the three divisions by them go through DivBy, and the two
+// Brooks-Corey exponents (2 + 3 lam) / lam and (2 + lam) / lam are divided
+// once per block; thread 0 puts them in LDS before the block's one barrier.
+// The same expressions as stage1_cell, so the same bits.
+template <bool BC_A>
+__global__ __launch_bounds__(TX* TY) void k_ro_P2_Xi_one(filt_fields f, filt_params p, int nx,
+                                                         int rank, int size)
+{
+    __shared__ double u[5];  // reciprocals of 1 - S_wr, mu1, mu2; the exponents
+    const int t = threadIdx.x + TX * threadIdx.y;
+    if (t == 0) {
+        const double lam = p.lambda[0];
+        u[0] = refined_reciprocal(1. - p.S_wr[0]);
+        u[1] = refined_reciprocal(p.mu1);
+        u[2] = refined_reciprocal(p.mu2);
+        u[3] = (2. + 3. * lam) / lam;
+        u[4] = (2. + lam) / lam;
+    }
+    __syncthreads();
+    const Cell e = cell_of_thread(nx, p.Ny);
+    if (!e.valid || !is_active(e.i, nx, rank, size))
+        return;
+    const double swr = p.S_wr[0];
+    const double K = f.perm ? f.perm[e.c] : p.K[0];
+    const double P1 = f.P1[e.c], S2 = f.S2[e.c];
+    const DivBy dd{1. - swr, u[0]}, dmu1{p.mu1, u[1]}, dmu2{p.mu2, u[2]};
+    const double S_e = dd(1. - S2 - swr);
+    const double k1 = pow_spec(S_e, u[3]);
+    const double k2 = (1. - S_e) * (1. - S_e) * (1 - pow_spec(S_e, u[4]));
+    const double P_k = 0;  // capillary pressure disabled, cpu.cpp:63
+    const double P2 = P1 + P_k;
+    __builtin_nontemporal_store(P2, f.P2 + e.c);
+    __builtin_nontemporal_store(dmu1(-1. * K * k1), f.Xi1 + e.c);
+    __builtin_nontemporal_store(dmu2(-1. * K * k2), f.Xi2 + e.c);
+    __builtin_nontemporal_store(ro1_of(P1, p), f.ro1 + e.c);
+    __builtin_nontemporal_store(ro2_of(P2, p), f.ro2 + e.c);
+    if constexpr (BC_A) {
+        if (e.i == 0 && !is_interior(1, e.j, e.k, nx, p.Ny, p.Nz)) {
+            f.P1[e.c] = f.P1[e.c + 1];
+            f.S2[e.c] = f.S2[e.c + 1];
+        }
+    }
+}
+
+// Stage 1 takes k_ro_P2_Xi_one where it applies.
+bool stage1_one(const filt_fields& f, const filt_params& p)
+{
+    return !(f.media && p.n_media > 1) && divby_ok(1. - p.S_wr[0]) && divby_ok(p.mu1) &&
+           divby_ok(p.mu2);
+}
+
 // ---------------------------------------------------------------- stage 2 --
 // cpu.cpp:17-24, 77-160.  Bytes per cell: Xi1, Xi2, ro1, ro2, P1, P2 in,
 // u1x..u2z out (P neighbours from cache).
@@ -526,8 +578,9 @@ int filt_ro_P2_Xi_calculation(const filt_fields* dev, const filt_params* p, int 
 {
     int rc = launch_check(dev, p, localNx);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_ro_P2_Xi<false>, cell_grid(p, localNx), dim3(TX, TY), 0, stream(), *dev,
-                       *p, localNx, rank, size);
+    hipLaunchKernelGGL(stage1_one(*dev, *p) ? k_ro_P2_Xi_one<false> : k_ro_P2_Xi<false>,
+                       cell_grid(p, localNx), dim3(TX, TY), 0, stream(), *dev, *p, localNx, rank,
+                       size);
     FILT_LAUNCH_CHECK("k_ro_P2_Xi");
     return FILT_OK;
 }
@@ -660,8 +713,9 @@ int filt_time_step_timed(const filt_fields* dev, const filt_params* p, double t,
             (with_u && (!dev->u1x || !dev->u1y || !dev->u1z || !dev->u2x || !dev->u2y ||
                         !dev->u2z)))
             return set_error(FILT_ERR_INVALID, "null device fields");
-        hipLaunchKernelGGL(k_ro_P2_Xi<true>, cell_grid(p, localNx), dim3(TX, TY), 0, stream(),
-                           *dev, *p, localNx, rank, size);
+        hipLaunchKernelGGL(stage1_one(*dev, *p) ? k_ro_P2_Xi_one<true> : k_ro_P2_Xi<true>,
+                           cell_grid(p, localNx), dim3(TX, TY), 0, stream(), *dev, *p, localNx,
+                           rank, size);
         FILT_LAUNCH_CHECK("k_ro_P2_Xi");
         if ((rc = mark(1))) return rc;
         if ((rc = launch_step_zm(*dev, *p, localNx, with_u != 0, true))) return rc;

@@ -84,6 +84,37 @@ def test_stage_parity_per_cell_fields(hip, orc, stage):
         assert elem_rel(out[n], want[n]) <= 1e-13, n
 
 
+@pytest.mark.parametrize("params", ["ref", "bl"])
+@pytest.mark.parametrize("perm", [False, True])
+def test_stage1_one_medium_kernel_equals_general(hip, params, perm):
+    """Stage 1 for one medium (divisions by 1 - S_wr, mu1, mu2 through DivBy)
+    against the general kernel on the same state, bit for bit: the same grid
+    with a two-entry media table and a media array of zeros takes the general
+    kernel.  Saturations include 0, -0, S_wr-adjacent, 1, above 1 and NaN."""
+    mk = ref_params if params == "ref" else bl_params
+    p1 = mk(70, 13, 9)
+    p1.n_media = 1
+    g = random_state(p1, np.random.default_rng(23 + perm), media=False, perm=perm)
+    sp = [0.0, -0.0, 0.9, 0.9 - 1e-16, 0.9 + 1e-16, 1.0, 1.5, -0.2, np.nan, np.inf, 1e-300]
+    g["S2"].flat[:len(sp)] = sp
+    outs = []
+    for n_media in (1, 2):
+        p = mk(70, 13, 9)
+        p.n_media = n_media
+        d = hip.grid(p, perm=perm)
+        try:
+            d.upload(g)
+            d.stage(1)
+            out = g.copy()
+            d.download(out)
+        finally:
+            d.free()
+        outs.append(out)
+    for n in STAGE_OUT[1] + ("P1", "S2"):
+        a, b = outs[0][n], outs[1][n]
+        assert np.array_equal(a.view(np.uint64), b.view(np.uint64)), n
+
+
 @pytest.mark.parametrize("dims", [(3, 3, 3), (4, 7, 5), (17, 9, 11), (5, 5, 1), (6, 5, 2),
                                   (2, 6, 6), (9, 2, 7)])
 def test_boundary_bitwise_random_state(hip, orc, dims):
