@@ -196,7 +196,7 @@ def run_plain(args, p, perm, poro, split=False, steps=None):
             # the library's autotuner picks the stages 2-4 kernel's tile for
             # this grid (it times the kernel's variant without the folded
             # boundary, so the time step's own kernel launches only here)
-            PKG.abi.check(lib, lib.filt_autotune_step(fref, pref, nx, 3, ctypes.byref(step_best)),
+            PKG.abi.check(lib, lib.filt_autotune_step(fref, pref, nx, 5, ctypes.byref(step_best)),
                           "step autotune")
     best = (ctypes.c_double * 2)()
     if split and getattr(args, "stencil_tile", None):
