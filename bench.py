@@ -976,7 +976,7 @@ def main():
                          "that must launch the same kernel")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+        print(json.dumps({"error": "this benchmark runs on one GPU; --gpus must be 1 "
                                    "(multi-GPU sharding is described, not built)"}))
         raise SystemExit(2)
     if args.copy:

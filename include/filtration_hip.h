@@ -37,7 +37,7 @@ extern "C" {
 #define FILT_ERR_NOMEM 3       /* device allocation failed */
 #define FILT_ERR_UNSTABLE 4    /* the explicit-scheme stability guard refused */
 #define FILT_ERR_IO 5          /* file could not be opened or read */
-#define FILT_ERR_UNSUPPORTED 6 /* valid request this build does not implement */
+#define FILT_ERR_UNSUPPORTED 6 /* valid request the library does not implement */
 #define FILT_ERR_NOT_FOUND 7   /* unknown slot / handle */
 
 /* Largest media table.  The reference has two media (defines.h:22-26). */
