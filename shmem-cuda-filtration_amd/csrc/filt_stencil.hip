@@ -753,9 +753,16 @@ extern "C" int filt_autotune_stencil(const filt_fields* dev, const filt_params* 
         std::vector<int> kzs = {8, 16, 32, 64};
         if (nz > 64)
             kzs.push_back(nz);
-        hipEvent_t e0, e1;
-        FILT_HIP_TRY(hipEventCreate(&e0));
-        FILT_HIP_TRY(hipEventCreate(&e1));
+        struct Ev {  // destroyed on every return
+            hipEvent_t e = nullptr;
+            ~Ev()
+            {
+                if (e) (void)hipEventDestroy(e);
+            }
+        } ev0, ev1;
+        FILT_HIP_TRY(hipEventCreate(&ev0.e));
+        FILT_HIP_TRY(hipEventCreate(&ev1.e));
+        const hipEvent_t e0 = ev0.e, e1 = ev1.e;
         TileConfig best[2] = {cfg_for(0, nx, ny, nz), cfg_for(1, nx, ny, nz)};
         double best_t[2] = {1e30, 1e30};
         // the tiled roS kernels divide through DivBy: outside its divisor
@@ -783,6 +790,21 @@ extern "C" int filt_autotune_stencil(const filt_fields* dev, const filt_params* 
                 FILT_HIP_TRY(hipMemcpyAsync((char*)save.p + a * bytes, keep[a], bytes,
                                             hipMemcpyDeviceToDevice, stream()));
         }
+        // on an error return, the roS arrays go back to the state the call
+        // found (declared after `save`, so it runs before `save` is freed)
+        struct Restore {
+            double* const* dst;
+            const double* src;
+            size_t bytes;
+            bool armed;
+            ~Restore()
+            {
+                if (armed)
+                    for (int a = 0; a < 4; a++)
+                        (void)hipMemcpyAsync(dst[a], (const char*)src + a * bytes, bytes,
+                                             hipMemcpyDeviceToDevice, stream());
+            }
+        } restore{keep, save.p, bytes, save.p != nullptr};
         // the divgrad kernel is timed on its relaxed branch (t >= 2 dt)
         const double t_div = 2. * p->dt;
         for (int kern = roS_ok ? 0 : 1; kern < 2; kern++)
@@ -825,12 +847,12 @@ extern "C" int filt_autotune_stencil(const filt_fields* dev, const filt_params* 
                             best[kern] = t;
                         }
                     }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        if (save.p)
+        if (save.p) {
+            restore.armed = false;
             for (int a = 0; a < 4; a++)
                 FILT_HIP_TRY(hipMemcpyAsync(keep[a], (char*)save.p + a * bytes, bytes,
                                             hipMemcpyDeviceToDevice, stream()));
+        }
         {
             std::lock_guard<std::mutex> lk(g_tune_mu);
             g_tuned[std::make_tuple(nx, ny, nz)] = std::make_pair(best[0], best[1]);
@@ -878,6 +900,19 @@ extern "C" int filt_autotune_step(const filt_fields* dev, const filt_params* p, 
         } save;
         FILT_HIP_TRY(hipMalloc((void**)&save.p, bytes));
         FILT_HIP_TRY(hipMemcpyAsync(save.p, dev->S2, bytes, hipMemcpyDeviceToDevice, stream()));
+        // on an error return, S2 goes back to the state the call found
+        // (declared after `save`, so it runs before `save` is freed)
+        struct Restore {
+            double* dst;
+            const double* src;
+            size_t bytes;
+            bool armed = true;
+            ~Restore()
+            {
+                if (armed)
+                    (void)hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream());
+            }
+        } restore{dev->S2, save.p, bytes};
         struct Ev {
             hipEvent_t e = nullptr;
             ~Ev()
@@ -921,6 +956,7 @@ extern "C" int filt_autotune_step(const filt_fields* dev, const filt_params* p, 
                     }
                 }
         FILT_HIP_TRY(hipMemcpyAsync(dev->S2, save.p, bytes, hipMemcpyDeviceToDevice, stream()));
+        restore.armed = false;
         rc = launch_step_zm_cfg(*dev, *p, nx, true, false, best);
         if (rc)
             return rc;
