@@ -1,7 +1,9 @@
 // filt_stencil.hip -- z-marching LDS-tiled kernels of the plain path's two
 // stencil stages, u_calculation (cpu.cpp:77-160) and roS_calculation
-// (cpu.cpp:218-291), for grids with NX, Ny, Nz >= 3 whose arrays are below
-// 4 GiB each.
+// (cpu.cpp:218-291, and the divgrad scheme of cpu.cpp:162-216), for grids
+// with NX, Ny, Nz >= 3 whose arrays are below 4 GiB each; their tile
+// configurations, and the autotuners of these kernels and of the stages 2-4
+// kernel (filt_autotune_stencil, filt_autotune_step).
 //
 // Each block owns a TX x TY tile of columns and a chunk of kz planes.  Per
 // plane it writes its columns' values and a one-cell halo ring (read from
